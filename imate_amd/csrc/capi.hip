@@ -1,0 +1,1066 @@
+// C ABI of libslq_hip.so and the host side of the SLQ estimator.
+//
+// The Monte-Carlo loop follows imate's cTraceEstimator / cuTraceEstimator
+// (imate/_c_trace_estimator/c_trace_estimator.cpp:191-307,
+// imate/_cu_trace_estimator/cu_trace_estimator.cu:197-310) in its
+// reproducible one-device configuration: samples are processed in order, the
+// convergence test runs after every sample and sampling stops at the first
+// sample after which every inquiry has converged. The device computes a block
+// of 64 probes at a time; samples of a block that lie beyond the stopping
+// sample are discarded, so every output equals the one-thread reference run.
+// ConvergenceTools (convergence_tools.cpp:94-376) and erf_inv
+// (special_functions.cpp:63-104) are restated here in C++.
+
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <ctime>
+#include <limits>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "slq_internal.hpp"
+
+namespace slq {
+
+// ---------------------------------------------------------------- errors
+
+namespace {
+thread_local std::string g_last_error;
+}
+
+void set_error(const char* fmt, ...)
+{
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+}
+
+int hip_status(hipError_t e, const char* what, const char* file, int line)
+{
+    set_error("%s failed: %s (%s:%d)", what, hipGetErrorString(e), file, line);
+    if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) return SLQ_E_OOM;
+    if (e == hipErrorNoDevice || e == hipErrorInvalidDevice) return SLQ_E_NODEVICE;
+    return SLQ_E_HIP;
+}
+
+// ---------------------------------------------------------------- storage
+
+MatrixStore::~MatrixStore()
+{
+    if (csr.indptr) (void) hipFree(csr.indptr);
+    if (csr.indices) (void) hipFree(csr.indices);
+    if (csr.data) (void) hipFree(csr.data);
+    if (trans.indptr) (void) hipFree(trans.indptr);
+    if (trans.indices) (void) hipFree(trans.indices);
+    if (trans.data) (void) hipFree(trans.data);
+    if (data_f32) (void) hipFree(data_f32);
+    if (trans_data_f32) (void) hipFree(trans_data_f32);
+}
+
+Workspace::~Workspace()
+{
+    if (base) (void) hipFree(base);
+}
+
+int Workspace::reserve(size_t need)
+{
+    if (need <= bytes) return SLQ_OK;
+    if (base)
+    {
+        (void) hipFree(base);
+        base = nullptr;
+        bytes = 0;
+    }
+    SLQ_CHECK(hipMalloc(&base, need));
+    bytes = need;
+    return SLQ_OK;
+}
+
+// ---------------------------------------------------------------- profiling
+
+void prof_begin(slq_op* op, hipEvent_t* ev)
+{
+    *ev = nullptr;
+    if (!op->prof.enabled) return;
+    hipEvent_t e;
+    if (!op->prof.pool.empty())
+    {
+        e = op->prof.pool.back();
+        op->prof.pool.pop_back();
+    }
+    else if (hipEventCreate(&e) != hipSuccess)
+        return;
+    (void) hipEventRecord(e, op->stream);
+    *ev = e;
+}
+
+void prof_end(slq_op* op, hipEvent_t ev0, int kind)
+{
+    if (!ev0) return;
+    hipEvent_t e;
+    if (!op->prof.pool.empty())
+    {
+        e = op->prof.pool.back();
+        op->prof.pool.pop_back();
+    }
+    else if (hipEventCreate(&e) != hipSuccess)
+        return;
+    (void) hipEventRecord(e, op->stream);
+    (kind == 0 ? op->prof.spmm_pending : op->prof.update_pending).push_back({ev0, e});
+}
+
+void prof_collect(slq_op* op)
+{
+    for (int kind = 0; kind < 2; ++kind)
+    {
+        auto& pend = kind == 0 ? op->prof.spmm_pending : op->prof.update_pending;
+        for (auto& pr : pend)
+        {
+            float ms = 0.f;
+            if (hipEventSynchronize(pr.second) == hipSuccess &&
+                hipEventElapsedTime(&ms, pr.first, pr.second) == hipSuccess)
+            {
+                if (kind == 0)
+                {
+                    op->prof.spmm_ms += ms;
+                    op->prof.spmm_launches += 1;
+                }
+                else
+                {
+                    op->prof.update_ms += ms;
+                    op->prof.update_launches += 1;
+                }
+            }
+            op->prof.pool.push_back(pr.first);
+            op->prof.pool.push_back(pr.second);
+        }
+        pend.clear();
+    }
+}
+
+// ---------------------------------------------------------------- transpose
+
+namespace {
+
+__global__ void expand_rows_kernel(const int64_t* __restrict__ indptr, int64_t nrows,
+                                   int32_t* __restrict__ rowidx)
+{
+    const int64_t row = (int64_t) blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+    const int lane = threadIdx.x % 64;
+    if (row >= nrows) return;
+    for (int64_t k = indptr[row] + lane; k < indptr[row + 1]; k += 64)
+        rowidx[k] = (int32_t) row;
+}
+
+__global__ void iota_kernel(int32_t* __restrict__ p, int64_t n)
+{
+    const int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = (int32_t) i;
+}
+
+__global__ void col_count_kernel(const int32_t* __restrict__ sorted_cols, int64_t nnz,
+                                 int64_t* __restrict__ colptr)
+{
+    // colptr[c + 1] = number of entries with column c, from the sorted keys:
+    // an entry that starts a run of column c records the run end.
+    const int64_t k = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nnz) return;
+    const int32_t c = sorted_cols[k];
+    if (k == nnz - 1 || sorted_cols[k + 1] != c) colptr[c + 1] = k + 1;
+}
+
+__global__ void fill_colptr_kernel(int64_t* __restrict__ colptr, int64_t ncols)
+{
+    // run ends -> monotone pointers: empty columns inherit the previous end.
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    int64_t last = 0;
+    colptr[0] = 0;
+    for (int64_t c = 1; c <= ncols; ++c)
+    {
+        if (colptr[c] < last) colptr[c] = last;
+        last = colptr[c];
+    }
+}
+
+template <typename T>
+__global__ void gather_trans_kernel(const int32_t* __restrict__ perm,
+                                    const int32_t* __restrict__ rowidx,
+                                    const T* __restrict__ data, int64_t nnz,
+                                    int32_t* __restrict__ tind, T* __restrict__ tdata)
+{
+    const int64_t k = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nnz) return;
+    const int32_t src = perm[k];
+    tind[k] = rowidx[src];
+    tdata[k] = data[src];
+}
+
+__global__ void set_neg_kernel(int64_t* p, int64_t n)
+{
+    const int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = -1;
+}
+
+}  // namespace
+
+// CSR of A^T by a stable radix sort of the column keys: within a column the
+// rows stay ascending, so the transpose is deterministic.
+int build_transpose(hipStream_t s, const DevCSR& A, DevCSR& T)
+{
+    const int64_t nnz = A.nnz;
+    if (nnz >= (int64_t) 1 << 31)
+    {
+        set_error("transpose of a matrix with >= 2^31 non-zeros is not supported");
+        return SLQ_E_ARG;
+    }
+    const size_t vb = A.dtype == SLQ_F64 ? 8 : 4;
+    T.nrows = A.ncols;
+    T.ncols = A.nrows;
+    T.nnz = nnz;
+    T.dtype = A.dtype;
+    SLQ_CHECK(hipMalloc(&T.indptr, sizeof(int64_t) * (T.nrows + 1)));
+    SLQ_CHECK(hipMalloc(&T.indices, sizeof(int32_t) * (nnz > 0 ? nnz : 1)));
+    SLQ_CHECK(hipMalloc(&T.data, vb * (nnz > 0 ? nnz : 1)));
+    const int64_t n1 = T.nrows + 1;
+    hipLaunchKernelGGL(set_neg_kernel, dim3((unsigned) ((n1 + 255) / 256)), dim3(256),
+                       0, s, T.indptr, n1);
+    if (nnz > 0)
+    {
+        int32_t *rowidx = nullptr, *iota = nullptr, *keys = nullptr, *perm = nullptr;
+        void* tmp = nullptr;
+        size_t tmp_bytes = 0;
+        SLQ_CHECK(hipMalloc(&rowidx, sizeof(int32_t) * nnz));
+        SLQ_CHECK(hipMalloc(&iota, sizeof(int32_t) * nnz));
+        SLQ_CHECK(hipMalloc(&keys, sizeof(int32_t) * nnz));
+        SLQ_CHECK(hipMalloc(&perm, sizeof(int32_t) * nnz));
+        const unsigned g = (unsigned) ((nnz + 255) / 256);
+        hipLaunchKernelGGL(expand_rows_kernel, dim3((unsigned) ((A.nrows + 3) / 4)),
+                           dim3(256), 0, s, A.indptr, A.nrows, rowidx);
+        hipLaunchKernelGGL(iota_kernel, dim3(g), dim3(256), 0, s, iota, nnz);
+        int end_bit = 1;
+        while (end_bit < 31 && ((int64_t) 1 << end_bit) < A.ncols) ++end_bit;
+        SLQ_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, A.indices, keys,
+                                                     iota, perm, (int) nnz, 0, end_bit, s));
+        SLQ_CHECK(hipMalloc(&tmp, tmp_bytes > 0 ? tmp_bytes : 1));
+        SLQ_CHECK(hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, A.indices, keys,
+                                                     iota, perm, (int) nnz, 0, end_bit, s));
+        hipLaunchKernelGGL(col_count_kernel, dim3(g), dim3(256), 0, s, keys, nnz, T.indptr);
+        if (A.dtype == SLQ_F64)
+            hipLaunchKernelGGL(gather_trans_kernel<double>, dim3(g), dim3(256), 0, s, perm,
+                               rowidx, (const double*) A.data, nnz, T.indices,
+                               (double*) T.data);
+        else
+            hipLaunchKernelGGL(gather_trans_kernel<float>, dim3(g), dim3(256), 0, s, perm,
+                               rowidx, (const float*) A.data, nnz, T.indices,
+                               (float*) T.data);
+        SLQ_CHECK(hipGetLastError());
+        SLQ_CHECK(hipStreamSynchronize(s));
+        (void) hipFree(rowidx);
+        (void) hipFree(iota);
+        (void) hipFree(keys);
+        (void) hipFree(perm);
+        (void) hipFree(tmp);
+    }
+    hipLaunchKernelGGL(fill_colptr_kernel, dim3(1), dim3(1), 0, s, T.indptr, T.nrows);
+    SLQ_CHECK(hipGetLastError());
+    SLQ_CHECK(hipStreamSynchronize(s));
+    return SLQ_OK;
+}
+
+// ---------------------------------------------------------------- statistics
+
+namespace {
+
+double sign_d(double x) { return (x > 0) - (x < 0); }
+
+// erf_inv, special_functions.cpp:63-104: rational initial guess with the
+// published coefficients the reference cites, then two Newton steps.
+double erf_inv(double x)
+{
+    if (x == 1.0 || x == -1.0) return sign_d(x) * INFINITY;
+    const double a[4] = {0.886226899, -1.645349621, 0.914624893, -0.140543331};
+    const double b[5] = {1.0, -2.118377725, 1.442710462, -0.329097515, 0.012229801};
+    const double c[4] = {-1.970840454, -1.62490649, 3.429567803, 1.641345311};
+    const double d[3] = {1.0, 3.543889200, 1.637067800};
+    double z = sign_d(x) * x;
+    double r;
+    if (z <= 0.7)
+    {
+        const double x2 = z * z;
+        r = z * (((a[3] * x2 + a[2]) * x2 + a[1]) * x2 + a[0]);
+        r /= (((b[4] * x2 + b[3]) * x2 + b[2]) * x2 + b[1]) * x2 + b[0];
+    }
+    else
+    {
+        const double y = std::sqrt(-std::log((1.0 - z) / 2.0));
+        r = (((c[3] * y + c[2]) * y + c[1]) * y + c[0]);
+        r /= ((d[2] * y + d[1]) * y + d[0]);
+    }
+    r = r * sign_d(x);
+    z = z * sign_d(x);
+    r -= (std::erf(r) - z) / ((2.0 / std::sqrt(M_PI)) * std::exp(-r * r));
+    r -= (std::erf(r) - z) / ((2.0 / std::sqrt(M_PI)) * std::exp(-r * r));
+    return r;
+}
+
+// ConvergenceTools::check_convergence, convergence_tools.cpp:94-194.
+int check_convergence(const double* samples, int q, int min_num_samples,
+                      const int32_t* processed, int num_processed,
+                      double confidence, double atol, double rtol,
+                      double* error, int32_t* num_used, int32_t* converged)
+{
+    if (num_processed < min_num_samples)
+    {
+        for (int j = 0; j < q; ++j)
+        {
+            error[j] = INFINITY;
+            converged[j] = 0;
+            num_used[j] = num_processed;
+        }
+        return 0;
+    }
+    const double z = std::sqrt(2) * erf_inv(confidence);
+    for (int j = 0; j < q; ++j)
+    {
+        if (converged[j] != 0) continue;
+        double s = 0.0;
+        for (int i = 0; i < num_processed; ++i) s += samples[(size_t) processed[i] * q + j];
+        const double mean = s / num_processed;
+        double sd;
+        if (num_processed > 1)
+        {
+            s = 0.0;
+            for (int i = 0; i < num_processed; ++i)
+            {
+                const double d = samples[(size_t) processed[i] * q + j];
+                s += (d - mean) * (d - mean);
+            }
+            sd = std::sqrt(s / (num_processed - 1.0));
+        }
+        else
+            sd = INFINITY;
+        error[j] = z * sd / std::sqrt((double) num_processed);
+        if (error[j] < std::max(atol, rtol * mean)) converged[j] = 1;
+        num_used[j] = num_processed;
+    }
+    for (int j = 0; j < q; ++j)
+        if (converged[j] == 0) return 0;
+    return 1;
+}
+
+// ConvergenceTools::average_estimates, convergence_tools.cpp:255-376.
+void average_estimates(const double* samples, int q, double confidence,
+                       double outlier, const int32_t* num_used,
+                       const int32_t* processed, int32_t* num_outliers,
+                       double* trace, double* error)
+{
+    const double ez = std::sqrt(2) * erf_inv(confidence);
+    const double oz = std::sqrt(2.0) * erf_inv(1.0 - outlier);
+    for (int j = 0; j < q; ++j)
+    {
+        const int N = num_used[j];
+        std::vector<char> out(N > 0 ? N : 1, 0);
+        num_outliers[j] = 0;
+        double s = 0.0;
+        for (int i = 0; i < N; ++i) s += samples[(size_t) processed[i] * q + j];
+        double mean = s / N;
+        double sd;
+        if (N > 1)
+        {
+            s = 0.0;
+            for (int i = 0; i < N; ++i)
+            {
+                const double d = samples[(size_t) processed[i] * q + j] - mean;
+                s += d * d;
+            }
+            sd = std::sqrt(s / (N - 1.0));
+        }
+        else
+            sd = INFINITY;
+        const double half = oz * sd;
+        for (int i = 0; i < N; ++i)
+        {
+            const double d = samples[(size_t) processed[i] * q + j] - mean;
+            if (std::abs(d) > half)
+            {
+                out[i] = 1;
+                num_outliers[j] += 1;
+            }
+        }
+        const int no = num_outliers[j];
+        s = 0.0;
+        for (int i = 0; i < N; ++i)
+            if (!out[i]) s += samples[(size_t) processed[i] * q + j];
+        mean = s / (N - no);
+        if (N > 1 + no)
+        {
+            s = 0.0;
+            for (int i = 0; i < N; ++i)
+                if (!out[i])
+                {
+                    const double d = samples[(size_t) processed[i] * q + j] - mean;
+                    s += d * d;
+                }
+            sd = std::sqrt(s / (N - no - 1.0));
+        }
+        else
+            sd = INFINITY;
+        trace[j] = mean;
+        error[j] = ez * sd / std::sqrt((double) (N - no));
+    }
+}
+
+int64_t clock_seed()
+{
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return (int64_t) (((uint64_t) ts.tv_sec * 1000000000ULL + (uint64_t) ts.tv_nsec) &
+                      0x7fffffffffffffffULL);
+}
+
+// Workspace carve-up for one estimate / recurrence call.
+struct Plan {
+    int64_t len = 0;        // probe length
+    int64_t W = 0;          // words per probe
+    int vbytes = 8;         // value bytes of the run
+    int m = 0;
+    int q = 1;
+    size_t off_words = 0, off_X0 = 0, off_alpha = 0, off_beta = 0, off_sizes = 0,
+           off_samples = 0, off_params = 0, off_scratch = 0, total = 0;
+    size_t scratch_bytes = 0;
+};
+
+size_t align256(size_t b) { return (b + 255) & ~(size_t) 255; }
+
+Plan make_plan(slq_op* op, int gram, int m, int ortho, int q, int vbytes)
+{
+    Plan p;
+    p.len = gram ? op->A->csr.ncols : op->A->csr.nrows;
+    p.W = p.len / 64 + 1;
+    p.vbytes = vbytes;
+    p.m = m;
+    p.q = q;
+    size_t o = 0;
+    p.off_words = o;   o += align256(sizeof(uint64_t) * kBlock * p.W);
+    p.off_X0 = o;      o += align256((size_t) vbytes * kBlock * p.len);
+    p.off_alpha = o;   o += align256(sizeof(double) * kBlock * m);
+    p.off_beta = o;    o += align256(sizeof(double) * kBlock * m);
+    p.off_sizes = o;   o += align256(sizeof(int32_t) * kBlock);
+    p.off_samples = o; o += align256(sizeof(double) * kBlock * q);
+    p.off_params = o;  o += align256(sizeof(double) * q);
+    p.off_scratch = o;
+    p.scratch_bytes = recurrence_scratch_bytes(op, gram, m, ortho, vbytes);
+    o += p.scratch_bytes;
+    p.total = o;
+    return p;
+}
+
+template <typename U>
+U* at(slq_op* op, size_t off) { return (U*) ((char*) op->ws.base + off); }
+
+int value_dtype(const slq_op* op, int precision)
+{
+    return (precision == SLQ_PREC_MIXED || op->A->csr.dtype == SLQ_F32) ? SLQ_F32 : SLQ_F64;
+}
+
+int prepare(slq_op* op, int gram, int precision)
+{
+    if (gram)
+    {
+        if (!op->A->has_trans)
+        {
+            SLQ_TRY(build_transpose(op->stream, op->A->csr, op->A->trans));
+            op->A->has_trans = true;
+        }
+        if (op->kind == OP_AFFINE_B && !op->B->has_trans)
+        {
+            SLQ_TRY(build_transpose(op->stream, op->B->csr, op->B->trans));
+            op->B->has_trans = true;
+        }
+    }
+    if (precision == SLQ_PREC_MIXED)
+    {
+        SLQ_TRY(ensure_f32(op->stream, *op->A, gram != 0));
+        if (op->kind == OP_AFFINE_B) SLQ_TRY(ensure_f32(op->stream, *op->B, gram != 0));
+    }
+    return SLQ_OK;
+}
+
+// Probes of samples [first, first + count) into the block X0.
+int make_probes(slq_op* op, const Plan& p, const uint64_t base[4], int64_t first,
+                int count, int vdtype)
+{
+    SLQ_TRY(rng_words(op->stream, base, p.W, first, count, at<uint64_t>(op, p.off_words)));
+    SLQ_TRY(rng_expand(op->stream, at<uint64_t>(op, p.off_words), p.W, p.len, count,
+                       vdtype, at<void>(op, p.off_X0)));
+    return SLQ_OK;
+}
+
+int recurrence_block(slq_op* op, const Plan& p, double t, int gram, double tol,
+                     int ortho, int precision)
+{
+    RecurrenceArgs ra;
+    ra.op = op;
+    ra.t = t;
+    ra.gram = gram;
+    ra.m = p.m;
+    ra.tol = tol;
+    ra.ortho = ortho;
+    ra.precision = precision;
+    ra.X0 = at<void>(op, p.off_X0);
+    ra.alpha = at<double>(op, p.off_alpha);
+    ra.beta = at<double>(op, p.off_beta);
+    ra.sizes = at<int32_t>(op, p.off_sizes);
+    return run_recurrence(ra, at<void>(op, p.off_scratch), p.scratch_bytes);
+}
+
+struct DeviceGuard {
+    int prev = -1;
+    explicit DeviceGuard(int dev)
+    {
+        if (hipGetDevice(&prev) == hipSuccess && prev != dev) (void) hipSetDevice(dev);
+        else prev = -1;
+    }
+    ~DeviceGuard()
+    {
+        if (prev >= 0) (void) hipSetDevice(prev);
+    }
+};
+
+}  // namespace
+
+// ---------------------------------------------------------------- copy kernel
+
+namespace {
+__global__ __launch_bounds__(256) void copy_kernel(const uint4* __restrict__ in,
+                                                   uint4* __restrict__ out, int64_t n)
+{
+    for (int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t) gridDim.x * blockDim.x)
+        out[i] = in[i];
+}
+}  // namespace
+
+int copy_bandwidth(int64_t bytes, int iters, double* ms)
+{
+    void *a = nullptr, *b = nullptr;
+    const int64_t n = bytes / 16;
+    SLQ_CHECK(hipMalloc(&a, n * 16));
+    SLQ_CHECK(hipMalloc(&b, n * 16));
+    SLQ_CHECK(hipMemset(a, 1, n * 16));
+    hipEvent_t e0, e1;
+    SLQ_CHECK(hipEventCreate(&e0));
+    SLQ_CHECK(hipEventCreate(&e1));
+    const int grid = 256 * 16;
+    hipLaunchKernelGGL(copy_kernel, dim3(grid), dim3(256), 0, 0, (const uint4*) a, (uint4*) b, n);
+    SLQ_CHECK(hipEventRecord(e0, 0));
+    for (int i = 0; i < iters; ++i)
+        hipLaunchKernelGGL(copy_kernel, dim3(grid), dim3(256), 0, 0, (const uint4*) a,
+                           (uint4*) b, n);
+    SLQ_CHECK(hipEventRecord(e1, 0));
+    SLQ_CHECK(hipEventSynchronize(e1));
+    float t = 0.f;
+    SLQ_CHECK(hipEventElapsedTime(&t, e0, e1));
+    *ms = t / (iters > 0 ? iters : 1);
+    (void) hipEventDestroy(e0);
+    (void) hipEventDestroy(e1);
+    (void) hipFree(a);
+    (void) hipFree(b);
+    return SLQ_OK;
+}
+
+}  // namespace slq
+
+using namespace slq;
+
+// ================================================================= C ABI
+
+extern "C" {
+
+int slq_last_error(char* buf, size_t len)
+{
+    if (!buf || len == 0) return SLQ_E_ARG;
+    std::snprintf(buf, len, "%s", g_last_error.c_str());
+    return SLQ_OK;
+}
+
+int slq_device_properties(int* num_devices, int* num_multiprocessors,
+                          int* num_threads_per_multiprocessor, char* name,
+                          size_t name_len)
+{
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n < 1)
+    {
+        set_error("no HIP device found (%s)", hipGetErrorString(e));
+        return SLQ_E_NODEVICE;
+    }
+    hipDeviceProp_t prop;
+    SLQ_CHECK(hipGetDeviceProperties(&prop, 0));
+    if (num_devices) *num_devices = n;
+    if (num_multiprocessors) *num_multiprocessors = prop.multiProcessorCount;
+    if (num_threads_per_multiprocessor)
+        *num_threads_per_multiprocessor = prop.maxThreadsPerMultiProcessor;
+    if (name && name_len) std::snprintf(name, name_len, "%s", prop.gcnArchName);
+    return SLQ_OK;
+}
+
+int slq_op_create_csr(int64_t nrows, int64_t ncols, int64_t nnz,
+                      const int64_t* indptr, const int32_t* indices,
+                      const void* data, int dtype, slq_op** out)
+{
+    if (!out || nrows < 1 || ncols < 1 || nnz < 0 || !indptr ||
+        (nnz > 0 && (!indices || !data)) || (dtype != SLQ_F64 && dtype != SLQ_F32))
+    {
+        set_error("slq_op_create_csr: invalid argument");
+        return SLQ_E_ARG;
+    }
+    if (ncols > INT32_MAX)
+    {
+        set_error("slq_op_create_csr: more than 2^31-1 columns");
+        return SLQ_E_ARG;
+    }
+    if (indptr[0] != 0 || indptr[nrows] != nnz)
+    {
+        set_error("slq_op_create_csr: indptr[0] must be 0 and indptr[nrows] == nnz");
+        return SLQ_E_ARG;
+    }
+    // Every kernel gathers X[column]: reject pointers or columns out of range
+    // here rather than let a kernel read outside its buffers.
+    for (int64_t r = 0; r < nrows; ++r)
+        if (indptr[r + 1] < indptr[r])
+        {
+            set_error("slq_op_create_csr: indptr decreases at row %lld", (long long) r);
+            return SLQ_E_ARG;
+        }
+    for (int64_t k = 0; k < nnz; ++k)
+        if (indices[k] < 0 || (int64_t) indices[k] >= ncols)
+        {
+            set_error("slq_op_create_csr: column index %d out of range at %lld",
+                      indices[k], (long long) k);
+            return SLQ_E_ARG;
+        }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+    {
+        set_error("no HIP device found");
+        return SLQ_E_NODEVICE;
+    }
+    slq_op* op = new (std::nothrow) slq_op();
+    if (!op)
+    {
+        set_error("host allocation failed");
+        return SLQ_E_OOM;
+    }
+    (void) hipGetDevice(&op->device);
+    op->A = std::make_shared<MatrixStore>();
+    DevCSR& c = op->A->csr;
+    c.nrows = nrows;
+    c.ncols = ncols;
+    c.nnz = nnz;
+    c.dtype = dtype;
+    const size_t vb = dtype == SLQ_F64 ? 8 : 4;
+    auto fail = [&](int code) {
+        delete op;
+        return code;
+    };
+    hipError_t e;
+    if ((e = hipStreamCreateWithFlags(&op->stream, hipStreamNonBlocking)) != hipSuccess)
+        return fail(hip_status(e, "hipStreamCreate", __FILE__, __LINE__));
+    if ((e = hipMalloc(&c.indptr, sizeof(int64_t) * (nrows + 1))) != hipSuccess ||
+        (e = hipMalloc(&c.indices, sizeof(int32_t) * (nnz > 0 ? nnz : 1))) != hipSuccess ||
+        (e = hipMalloc(&c.data, vb * (nnz > 0 ? nnz : 1))) != hipSuccess)
+        return fail(hip_status(e, "hipMalloc", __FILE__, __LINE__));
+    if ((e = hipMemcpy(c.indptr, indptr, sizeof(int64_t) * (nrows + 1),
+                       hipMemcpyHostToDevice)) != hipSuccess ||
+        (nnz > 0 && (e = hipMemcpy(c.indices, indices, sizeof(int32_t) * nnz,
+                                   hipMemcpyHostToDevice)) != hipSuccess) ||
+        (nnz > 0 && (e = hipMemcpy(c.data, data, vb * nnz, hipMemcpyHostToDevice)) != hipSuccess))
+        return fail(hip_status(e, "hipMemcpy", __FILE__, __LINE__));
+    *out = op;
+    return SLQ_OK;
+}
+
+int slq_op_create_affine(slq_op* A, slq_op* B, slq_op** out)
+{
+    if (!A || !out || A->kind != OP_MATRIX || (B && B->kind != OP_MATRIX))
+    {
+        set_error("slq_op_create_affine: A (and B) must be matrix handles");
+        return SLQ_E_ARG;
+    }
+    if (B && (B->A->csr.nrows != A->A->csr.nrows || B->A->csr.ncols != A->A->csr.ncols ||
+              B->A->csr.dtype != A->A->csr.dtype))
+    {
+        set_error("slq_op_create_affine: B must have the shape and dtype of A");
+        return SLQ_E_ARG;
+    }
+    slq_op* op = new (std::nothrow) slq_op();
+    if (!op)
+    {
+        set_error("host allocation failed");
+        return SLQ_E_OOM;
+    }
+    op->device = A->device;
+    op->kind = B ? OP_AFFINE_B : OP_AFFINE_I;
+    op->A = A->A;
+    if (B) op->B = B->A;
+    hipError_t e = hipStreamCreateWithFlags(&op->stream, hipStreamNonBlocking);
+    if (e != hipSuccess)
+    {
+        delete op;
+        return hip_status(e, "hipStreamCreate", __FILE__, __LINE__);
+    }
+    *out = op;
+    return SLQ_OK;
+}
+
+int slq_op_destroy(slq_op* op)
+{
+    if (!op) return SLQ_OK;
+    if (op->stream) (void) hipStreamSynchronize(op->stream);
+    prof_collect(op);
+    for (hipEvent_t e : op->prof.pool) (void) hipEventDestroy(e);
+    if (op->stream) (void) hipStreamDestroy(op->stream);
+    delete op;
+    return SLQ_OK;
+}
+
+int slq_estimate(slq_op* op, const double* params, int32_t q, int32_t func,
+                 int32_t gram, double exponent, int32_t orthogonalize,
+                 int64_t seed, int32_t m, double tol, int32_t min_s,
+                 int32_t max_s, double atol, double rtol, double confidence,
+                 double outlier, int32_t block_size, int32_t precision,
+                 double* trace, double* error, double* samples,
+                 int32_t* processed, int32_t* num_used, int32_t* num_outliers,
+                 int32_t* converged, int32_t* all_converged,
+                 double* alg_wall_time)
+{
+    if (!op || q < 1 || m < 1 || min_s < 1 || max_s < min_s || !trace || !error ||
+        !samples || !processed || !num_used || !num_outliers || !converged ||
+        func < SLQ_FUNC_LOG || func > SLQ_FUNC_IDENT ||
+        (precision != SLQ_PREC_NATIVE && precision != SLQ_PREC_MIXED))
+    {
+        set_error("slq_estimate: invalid argument");
+        return SLQ_E_ARG;
+    }
+    if (op->kind != OP_MATRIX && !params)
+    {
+        set_error("slq_estimate: an affine operator needs parameters");
+        return SLQ_E_ARG;
+    }
+    if (op->kind == OP_MATRIX && q != 1)
+    {
+        set_error("slq_estimate: a matrix operator has exactly one inquiry");
+        return SLQ_E_ARG;
+    }
+    if (!gram && op->A->csr.nrows != op->A->csr.ncols)
+    {
+        set_error("slq_estimate: Lanczos needs a square operator");
+        return SLQ_E_ARG;
+    }
+    DeviceGuard guard(op->device);
+    const int bs = (block_size <= 0 || block_size > kBlock) ? kBlock : block_size;
+    const bool relation = op->kind == OP_AFFINE_I;
+    const int required = relation ? 1 : q;
+    const int vdt = value_dtype(op, precision);
+    if (seed < 0) seed = clock_seed();
+
+    SLQ_TRY(prepare(op, gram, precision));
+    Plan p = make_plan(op, gram, m, orthogonalize, q, vdt == SLQ_F64 ? 8 : 4);
+    SLQ_TRY(op->ws.reserve(p.total));
+    double* params_dev = at<double>(op, p.off_params);
+    if (op->kind != OP_MATRIX)
+        SLQ_CHECK(hipMemcpyAsync(params_dev, params, sizeof(double) * q,
+                                 hipMemcpyHostToDevice, op->stream));
+    uint64_t base[4];
+    thread_state(seed, 0, base);
+    const double nq = (double) p.len;
+
+    for (int64_t i = 0; i < (int64_t) max_s * q; ++i)
+        samples[i] = std::numeric_limits<double>::quiet_NaN();
+    for (int j = 0; j < q; ++j)
+    {
+        converged[j] = 0;
+        num_used[j] = 0;
+        num_outliers[j] = 0;
+    }
+    std::vector<double> block_out((size_t) kBlock * q);
+    int nproc = 0;
+    int all_conv = 0;
+
+    SLQ_CHECK(hipStreamSynchronize(op->stream));
+    const auto t0 = std::chrono::steady_clock::now();
+    for (int64_t s0 = 0; s0 < max_s && !all_conv; s0 += bs)
+    {
+        const int cnt = (int) std::min<int64_t>(bs, max_s - s0);
+        SLQ_TRY(make_probes(op, p, base, s0, cnt, vdt));
+        for (int j = 0; j < required; ++j)
+        {
+            if (!relation && converged[j]) continue;   // masked below anyway
+            const double t = op->kind != OP_MATRIX ? params[j] : 0.0;
+            SLQ_TRY(recurrence_block(op, p, t, gram, tol, orthogonalize, precision));
+            SLQ_TRY(run_quadrature(op->stream, at<double>(op, p.off_alpha),
+                                   at<double>(op, p.off_beta), at<int32_t>(op, p.off_sizes),
+                                   cnt, m, gram, func, exponent, nq,
+                                   relation ? params_dev : nullptr, q, relation ? 1 : 0, j,
+                                   q, at<double>(op, p.off_samples), nullptr, nullptr));
+        }
+        SLQ_CHECK(hipMemcpyAsync(block_out.data(), at<double>(op, p.off_samples),
+                                 sizeof(double) * cnt * q, hipMemcpyDeviceToHost,
+                                 op->stream));
+        SLQ_CHECK(hipStreamSynchronize(op->stream));
+        for (int pi = 0; pi < cnt && !all_conv; ++pi)
+        {
+            const int64_t i = s0 + pi;
+            if (relation && q > 1)
+                for (int j = 0; j < q; ++j) converged[j] = 0;
+            for (int j = 0; j < q; ++j)
+                if (!converged[j]) samples[i * q + j] = block_out[(size_t) pi * q + j];
+            processed[nproc++] = (int32_t) i;
+            all_conv = check_convergence(samples, q, min_s, processed, nproc, confidence,
+                                         atol, rtol, error, num_used, converged);
+        }
+    }
+    const auto t1 = std::chrono::steady_clock::now();
+    if (alg_wall_time) *alg_wall_time = std::chrono::duration<double>(t1 - t0).count();
+    for (int i = nproc; i < max_s; ++i) processed[i] = 0;
+    average_estimates(samples, q, confidence, outlier, num_used, processed, num_outliers,
+                      trace, error);
+    if (all_converged) *all_converged = all_conv;
+    prof_collect(op);
+    return SLQ_OK;
+}
+
+int slq_rademacher(int64_t seed, int64_t n, int64_t first_sample, int32_t count,
+                   int dtype, void* out, uint64_t* words)
+{
+    if (n < 1 || count < 1 || count > kBlock || first_sample < 0 || !out ||
+        (dtype != SLQ_F64 && dtype != SLQ_F32))
+    {
+        set_error("slq_rademacher: invalid argument (count must be 1..64)");
+        return SLQ_E_ARG;
+    }
+    if (seed < 0) seed = clock_seed();
+    const int64_t W = n / 64 + 1;
+    const size_t vb = dtype == SLQ_F64 ? 8 : 4;
+    uint64_t* dw = nullptr;
+    void* dx = nullptr;
+    SLQ_CHECK(hipMalloc(&dw, sizeof(uint64_t) * W * kBlock));
+    SLQ_CHECK(hipMalloc(&dx, vb * n * kBlock));
+    uint64_t base[4];
+    thread_state(seed, 0, base);
+    int r = rng_words(0, base, W, first_sample, count, dw);
+    if (r == SLQ_OK) r = rng_expand(0, dw, W, n, count, dtype, dx);
+    std::vector<char> host(vb * n * kBlock);
+    if (r == SLQ_OK)
+    {
+        hipError_t e = hipMemcpy(host.data(), dx, vb * n * kBlock, hipMemcpyDeviceToHost);
+        if (e == hipSuccess && words)
+            e = hipMemcpy(words, dw, sizeof(uint64_t) * W * count, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) r = hip_status(e, "hipMemcpy", __FILE__, __LINE__);
+    }
+    (void) hipFree(dw);
+    (void) hipFree(dx);
+    if (r != SLQ_OK) return r;
+    for (int p = 0; p < count; ++p)
+        for (int64_t i = 0; i < n; ++i)
+            std::memcpy((char*) out + vb * ((size_t) p * n + i),
+                        host.data() + vb * ((size_t) i * kBlock + p), vb);
+    return SLQ_OK;
+}
+
+int slq_op_spmm(slq_op* op, double t, int32_t b, int32_t transpose, const void* X,
+                void* Y)
+{
+    if (!op || !X || !Y || b < 1 || b > kBlock)
+    {
+        set_error("slq_op_spmm: invalid argument (b must be 1..64)");
+        return SLQ_E_ARG;
+    }
+    DeviceGuard guard(op->device);
+    if (transpose) SLQ_TRY(prepare(op, 1, SLQ_PREC_NATIVE));
+    const DevCSR& c = op->A->csr;
+    const int64_t in_len = transpose ? c.nrows : c.ncols;
+    const int64_t out_len = transpose ? c.ncols : c.nrows;
+    const int vdt = c.dtype;
+    const size_t vb = vdt == SLQ_F64 ? 8 : 4;
+    void *dx = nullptr, *dy = nullptr;
+    SLQ_CHECK(hipMalloc(&dx, vb * in_len * kBlock));
+    SLQ_CHECK(hipMalloc(&dy, vb * out_len * kBlock));
+    std::vector<char> hx(vb * in_len * kBlock, 0);
+    for (int64_t i = 0; i < in_len; ++i)
+        std::memcpy(hx.data() + vb * i * kBlock, (const char*) X + vb * i * b, vb * b);
+    int r = SLQ_OK;
+    hipError_t e = hipMemcpy(dx, hx.data(), hx.size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess) r = hip_status(e, "hipMemcpy", __FILE__, __LINE__);
+    if (r == SLQ_OK) r = run_spmm(op, t, transpose, dx, dy, vdt);
+    std::vector<char> hy(vb * out_len * kBlock);
+    if (r == SLQ_OK)
+    {
+        e = hipStreamSynchronize(op->stream);
+        if (e == hipSuccess) e = hipMemcpy(hy.data(), dy, hy.size(), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) r = hip_status(e, "hipMemcpy", __FILE__, __LINE__);
+    }
+    (void) hipFree(dx);
+    (void) hipFree(dy);
+    prof_collect(op);
+    if (r != SLQ_OK) return r;
+    for (int64_t i = 0; i < out_len; ++i)
+        std::memcpy((char*) Y + vb * i * b, hy.data() + vb * i * kBlock, vb * b);
+    return SLQ_OK;
+}
+
+int slq_recurrences(slq_op* op, double t, int32_t gram, int64_t seed,
+                    int64_t first_sample, int32_t count, int32_t m, double tol,
+                    int32_t ortho, int32_t precision, double* alpha, double* beta,
+                    int32_t* sizes)
+{
+    if (!op || count < 1 || count > kBlock || m < 1 || !alpha || !beta || !sizes ||
+        first_sample < 0)
+    {
+        set_error("slq_recurrences: invalid argument (count must be 1..64)");
+        return SLQ_E_ARG;
+    }
+    if (!gram && op->A->csr.nrows != op->A->csr.ncols)
+    {
+        set_error("slq_recurrences: Lanczos needs a square operator");
+        return SLQ_E_ARG;
+    }
+    DeviceGuard guard(op->device);
+    if (seed < 0) seed = clock_seed();
+    const int vdt = value_dtype(op, precision);
+    SLQ_TRY(prepare(op, gram, precision));
+    Plan p = make_plan(op, gram, m, ortho, 1, vdt == SLQ_F64 ? 8 : 4);
+    SLQ_TRY(op->ws.reserve(p.total));
+    uint64_t base[4];
+    thread_state(seed, 0, base);
+    SLQ_TRY(make_probes(op, p, base, first_sample, count, vdt));
+    SLQ_TRY(recurrence_block(op, p, t, gram, tol, ortho, precision));
+    std::vector<double> ha((size_t) m * kBlock), hb((size_t) m * kBlock);
+    std::vector<int32_t> hs(kBlock);
+    SLQ_CHECK(hipMemcpyAsync(ha.data(), at<double>(op, p.off_alpha), sizeof(double) * m * kBlock,
+                             hipMemcpyDeviceToHost, op->stream));
+    SLQ_CHECK(hipMemcpyAsync(hb.data(), at<double>(op, p.off_beta), sizeof(double) * m * kBlock,
+                             hipMemcpyDeviceToHost, op->stream));
+    SLQ_CHECK(hipMemcpyAsync(hs.data(), at<int32_t>(op, p.off_sizes), sizeof(int32_t) * kBlock,
+                             hipMemcpyDeviceToHost, op->stream));
+    SLQ_CHECK(hipStreamSynchronize(op->stream));
+    prof_collect(op);
+    for (int c = 0; c < count; ++c)
+    {
+        sizes[c] = hs[c] > 0 ? hs[c] : m;
+        for (int j = 0; j < m; ++j)
+        {
+            alpha[(size_t) c * m + j] = ha[(size_t) j * kBlock + c];
+            beta[(size_t) c * m + j] = hb[(size_t) j * kBlock + c];
+        }
+    }
+    return SLQ_OK;
+}
+
+int slq_quadrature(const double* alpha, const double* beta, const int32_t* sizes,
+                   int32_t count, int32_t m, int32_t gram, int32_t func,
+                   double exponent, double n, const double* params, int32_t q,
+                   int32_t shift, double* samples, double* theta, double* tau)
+{
+    if (!alpha || !beta || !sizes || count < 1 || count > kBlock || m < 1 || q < 1 ||
+        !samples || (shift && !params))
+    {
+        set_error("slq_quadrature: invalid argument (count must be 1..64)");
+        return SLQ_E_ARG;
+    }
+    std::vector<double> ha((size_t) m * kBlock, 0.0), hb((size_t) m * kBlock, 0.0);
+    std::vector<int32_t> hs(kBlock, 1);
+    for (int c = 0; c < count; ++c)
+    {
+        hs[c] = sizes[c];
+        for (int j = 0; j < m; ++j)
+        {
+            ha[(size_t) j * kBlock + c] = alpha[(size_t) c * m + j];
+            hb[(size_t) j * kBlock + c] = beta[(size_t) c * m + j];
+        }
+    }
+    const int qs = shift ? q : 1;
+    double *da = nullptr, *db = nullptr, *dsmp = nullptr, *dp = nullptr, *dth = nullptr,
+           *dta = nullptr;
+    int32_t* ds = nullptr;
+    SLQ_CHECK(hipMalloc(&da, sizeof(double) * ha.size()));
+    SLQ_CHECK(hipMalloc(&db, sizeof(double) * hb.size()));
+    SLQ_CHECK(hipMalloc(&ds, sizeof(int32_t) * kBlock));
+    SLQ_CHECK(hipMalloc(&dsmp, sizeof(double) * kBlock * qs));
+    SLQ_CHECK(hipMalloc(&dp, sizeof(double) * q));
+    SLQ_CHECK(hipMalloc(&dth, sizeof(double) * kBlock * m));
+    SLQ_CHECK(hipMalloc(&dta, sizeof(double) * kBlock * m));
+    SLQ_CHECK(hipMemcpy(da, ha.data(), sizeof(double) * ha.size(), hipMemcpyHostToDevice));
+    SLQ_CHECK(hipMemcpy(db, hb.data(), sizeof(double) * hb.size(), hipMemcpyHostToDevice));
+    SLQ_CHECK(hipMemcpy(ds, hs.data(), sizeof(int32_t) * kBlock, hipMemcpyHostToDevice));
+    if (params) SLQ_CHECK(hipMemcpy(dp, params, sizeof(double) * q, hipMemcpyHostToDevice));
+    int r = run_quadrature(0, da, db, ds, count, m, gram, func, exponent, n,
+                           params ? dp : nullptr, q, shift, 0, qs, dsmp, dth, dta);
+    if (r == SLQ_OK)
+    {
+        hipError_t e = hipDeviceSynchronize();
+        if (e == hipSuccess)
+            e = hipMemcpy(samples, dsmp, sizeof(double) * count * qs, hipMemcpyDeviceToHost);
+        if (e == hipSuccess && theta)
+            e = hipMemcpy(theta, dth, sizeof(double) * count * m, hipMemcpyDeviceToHost);
+        if (e == hipSuccess && tau)
+            e = hipMemcpy(tau, dta, sizeof(double) * count * m, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) r = hip_status(e, "hipMemcpy", __FILE__, __LINE__);
+    }
+    (void) hipFree(da);
+    (void) hipFree(db);
+    (void) hipFree(ds);
+    (void) hipFree(dsmp);
+    (void) hipFree(dp);
+    (void) hipFree(dth);
+    (void) hipFree(dta);
+    return r;
+}
+
+int slq_profile_enable(slq_op* op, int32_t enable)
+{
+    if (!op) return SLQ_E_ARG;
+    prof_collect(op);
+    op->prof.enabled = enable != 0;
+    op->prof.spmm_ms = 0.0;
+    op->prof.spmm_launches = 0;
+    op->prof.update_ms = 0.0;
+    op->prof.update_launches = 0;
+    return SLQ_OK;
+}
+
+int slq_profile_read(slq_op* op, double* spmm_ms, int64_t* spmm_launches,
+                     double* update_ms, int64_t* update_launches)
+{
+    if (!op) return SLQ_E_ARG;
+    prof_collect(op);
+    if (spmm_ms) *spmm_ms = op->prof.spmm_ms;
+    if (spmm_launches) *spmm_launches = op->prof.spmm_launches;
+    if (update_ms) *update_ms = op->prof.update_ms;
+    if (update_launches) *update_launches = op->prof.update_launches;
+    return SLQ_OK;
+}
+
+int slq_copy_bandwidth(int64_t bytes, int32_t iters, double* ms_per_launch)
+{
+    if (bytes < 16 || iters < 1 || !ms_per_launch)
+    {
+        set_error("slq_copy_bandwidth: invalid argument");
+        return SLQ_E_ARG;
+    }
+    return copy_bandwidth(bytes, iters, ms_per_launch);
+}
+
+}  // extern "C"

@@ -1,0 +1,264 @@
+// Rademacher probe generation on the device, bit-exact to imate's per-thread
+// xoshiro256** stream.
+//
+// Stream definition (reference): thread 0's generator is xoshiro256** seeded
+// by four SplitMix64 draws from state (seed << 32) | seed and advanced by one
+// jump() (imate/_random_generator/split_mix_64.cpp:28-64,
+// random_number_generator.cpp:126-136). A probe of length n consumes
+// W = floor(n/64) + 1 consecutive words; bit j of word i is entry 64 i + j,
+// 1 -> +1 and 0 -> -1 (random_array_generator.cpp:98-164).
+//
+// Parallel generation: the state transition of xoshiro256** is linear over
+// GF(2), so the state k steps ahead is M^k s. The host builds the 256 x 256
+// bit matrices M^(2^e), e = 0..63, once; each device thread jumps straight to
+// its (probe, chunk) offset by multiplying the base state with the matrices of
+// the set bits of the offset, then steps its chunk of words sequentially.
+// A second kernel expands the packed words into the [len][64] probe block.
+
+#include <mutex>
+#include <vector>
+
+#include "slq_internal.hpp"
+
+namespace slq {
+
+namespace {
+
+constexpr int kChunkWords = 64;   // words generated per thread after the jump
+
+inline uint64_t rotl_h(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
+
+// State transition of one xoshiro256** step (the output is not needed).
+inline void advance_h(uint64_t s[4])
+{
+    const uint64_t t = s[1] << 17;
+    s[2] ^= s[0];
+    s[3] ^= s[1];
+    s[1] ^= s[2];
+    s[0] ^= s[3];
+    s[2] ^= t;
+    s[3] = rotl_h(s[3], 45);
+}
+
+inline uint64_t next_h(uint64_t s[4])
+{
+    const uint64_t result = rotl_h(s[1] * 5, 7) * 9;
+    advance_h(s);
+    return result;
+}
+
+// Column-major bit matrix: col[j] = image of the unit state e_j.
+struct BitMatrix {
+    uint64_t col[256][4];
+};
+
+void apply_h(const BitMatrix& M, const uint64_t x[4], uint64_t y[4])
+{
+    uint64_t r[4] = {0, 0, 0, 0};
+    for (int j = 0; j < 256; ++j)
+    {
+        if ((x[j >> 6] >> (j & 63)) & 1)
+        {
+            r[0] ^= M.col[j][0];
+            r[1] ^= M.col[j][1];
+            r[2] ^= M.col[j][2];
+            r[3] ^= M.col[j][3];
+        }
+    }
+    y[0] = r[0]; y[1] = r[1]; y[2] = r[2]; y[3] = r[3];
+}
+
+std::vector<BitMatrix>& jump_tables()
+{
+    static std::vector<BitMatrix> tables;
+    static std::once_flag once;
+    std::call_once(once, [] {
+        tables.resize(64);
+        for (int j = 0; j < 256; ++j)
+        {
+            uint64_t s[4] = {0, 0, 0, 0};
+            s[j >> 6] = (uint64_t) 1 << (j & 63);
+            advance_h(s);
+            for (int w = 0; w < 4; ++w) tables[0].col[j][w] = s[w];
+        }
+        for (int e = 1; e < 64; ++e)
+            for (int j = 0; j < 256; ++j)
+                apply_h(tables[e - 1], tables[e - 1].col[j], tables[e].col[j]);
+    });
+    return tables;
+}
+
+// One device copy of the tables per device.
+uint64_t* device_tables()
+{
+    static std::mutex mu;
+    static std::vector<uint64_t*> per_device;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+    std::lock_guard<std::mutex> lock(mu);
+    if ((int) per_device.size() <= dev) per_device.resize(dev + 1, nullptr);
+    if (!per_device[dev])
+    {
+        auto& t = jump_tables();
+        size_t bytes = sizeof(BitMatrix) * t.size();
+        uint64_t* d = nullptr;
+        if (hipMalloc(&d, bytes) != hipSuccess) return nullptr;
+        if (hipMemcpy(d, t.data(), bytes, hipMemcpyHostToDevice) != hipSuccess)
+        {
+            (void) hipFree(d);
+            return nullptr;
+        }
+        per_device[dev] = d;
+    }
+    return per_device[dev];
+}
+
+__device__ __forceinline__ uint64_t rotl_d(uint64_t x, int k)
+{
+    return (x << k) | (x >> (64 - k));
+}
+
+// y = M x over GF(2), M column-major [256][4] in global memory (the same
+// column is read by every lane of a wavefront at once).
+__device__ __forceinline__ void apply_d(const uint64_t* __restrict__ M,
+                                        uint64_t& x0, uint64_t& x1,
+                                        uint64_t& x2, uint64_t& x3)
+{
+    uint64_t r0 = 0, r1 = 0, r2 = 0, r3 = 0;
+    const uint64_t xs[4] = {x0, x1, x2, x3};
+#pragma unroll
+    for (int w = 0; w < 4; ++w)
+    {
+        const uint64_t xw = xs[w];
+        const uint64_t* c = M + (size_t) w * 64 * 4;
+        for (int b = 0; b < 64; ++b)
+        {
+            const uint64_t mask = 0 - ((xw >> b) & 1);
+            r0 ^= c[b * 4 + 0] & mask;
+            r1 ^= c[b * 4 + 1] & mask;
+            r2 ^= c[b * 4 + 2] & mask;
+            r3 ^= c[b * 4 + 3] & mask;
+        }
+    }
+    x0 = r0; x1 = r1; x2 = r2; x3 = r3;
+}
+
+// Thread (p, c): words [c*L, min((c+1)*L, W)) of probe p, i.e. stream words
+// starting at offset (first_sample + p) * W + c * L from the base state.
+__global__ __launch_bounds__(256) void rng_words_kernel(
+    const uint64_t* __restrict__ tables, uint64_t s0, uint64_t s1, uint64_t s2,
+    uint64_t s3, int64_t W, int64_t first_sample, int32_t count,
+    int64_t chunks, uint64_t* __restrict__ words)
+{
+    const int64_t gid = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t p = gid / chunks;
+    const int64_t c = gid - p * chunks;
+    if (p >= count) return;
+    const uint64_t offset = (uint64_t)(first_sample + p) * (uint64_t) W +
+                            (uint64_t)(c * kChunkWords);
+    uint64_t x0 = s0, x1 = s1, x2 = s2, x3 = s3;
+    for (int e = 0; e < 64; ++e)
+        if ((offset >> e) & 1) apply_d(tables + (size_t) e * 256 * 4, x0, x1, x2, x3);
+    uint64_t* out = words + p * W;
+    const int64_t beg = c * kChunkWords;
+    const int64_t end = (beg + kChunkWords < W) ? beg + kChunkWords : W;
+    for (int64_t i = beg; i < end; ++i)
+    {
+        out[i] = rotl_d(x1 * 5, 7) * 9;
+        const uint64_t t = x1 << 17;
+        x2 ^= x0;
+        x3 ^= x1;
+        x1 ^= x2;
+        x0 ^= x3;
+        x2 ^= t;
+        x3 = rotl_d(x3, 45);
+    }
+}
+
+// X[i][p] = +1 / -1 from bit (i mod 64) of word floor(i/64) of probe p; zero
+// for p >= count. One wavefront per row i, lane = probe.
+template <typename T>
+__global__ __launch_bounds__(256) void rng_expand_kernel(
+    const uint64_t* __restrict__ words, int64_t W, int64_t len, int32_t count,
+    T* __restrict__ X)
+{
+    const int64_t gid = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t i = gid / kBlock;
+    const int p = (int) (gid - i * kBlock);
+    if (i >= len) return;
+    T v = 0;
+    if (p < count)
+    {
+        const uint64_t w = words[(int64_t) p * W + (i >> 6)];
+        v = ((w >> (i & 63)) & 1) ? T(1) : T(-1);
+    }
+    X[gid] = v;
+}
+
+}  // namespace
+
+void thread_state(int64_t seed, int thread_id, uint64_t out[4])
+{
+    uint64_t sd = (uint64_t) seed;
+    uint64_t st = (sd << 32) | sd;
+    for (int i = 0; i < 4; ++i)
+    {
+        uint64_t z = (st += 0x9e3779b97f4a7c15ULL);
+        z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+        z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+        out[i] = z ^ (z >> 31);
+    }
+    // jump(): 2^128 steps, xoshiro_256_star_star.cpp:100-133 (published
+    // jump polynomial of xoshiro256**).
+    static const uint64_t J[4] = {0x180ec6d33cfd0abaULL, 0xd5a61266f0c9392cULL,
+                                  0xa9582618e03fc9aaULL, 0x39abdc4529b1661cULL};
+    for (int rep = 0; rep < thread_id + 1; ++rep)
+    {
+        uint64_t a[4] = {0, 0, 0, 0};
+        for (int i = 0; i < 4; ++i)
+            for (int b = 0; b < 64; ++b)
+            {
+                if (J[i] & ((uint64_t) 1 << b))
+                    for (int w = 0; w < 4; ++w) a[w] ^= out[w];
+                next_h(out);
+            }
+        for (int w = 0; w < 4; ++w) out[w] = a[w];
+    }
+}
+
+int rng_words(hipStream_t s, const uint64_t base[4], int64_t W,
+              int64_t first_sample, int32_t count, uint64_t* words)
+{
+    uint64_t* tables = device_tables();
+    if (!tables)
+    {
+        set_error("could not upload the xoshiro256** jump tables");
+        return SLQ_E_HIP;
+    }
+    if (count <= 0) return SLQ_OK;
+    const int64_t chunks = (W + kChunkWords - 1) / kChunkWords;
+    const int64_t threads = chunks * count;
+    const int64_t grid = (threads + 255) / 256;
+    hipLaunchKernelGGL(rng_words_kernel, dim3((unsigned) grid), dim3(256), 0, s,
+                       tables, base[0], base[1], base[2], base[3], W,
+                       first_sample, count, chunks, words);
+    SLQ_CHECK(hipGetLastError());
+    return SLQ_OK;
+}
+
+int rng_expand(hipStream_t s, const uint64_t* words, int64_t W, int64_t len,
+               int32_t count, int dtype, void* X)
+{
+    const int64_t threads = len * kBlock;
+    const int64_t grid = (threads + 255) / 256;
+    if (dtype == SLQ_F64)
+        hipLaunchKernelGGL(rng_expand_kernel<double>, dim3((unsigned) grid),
+                           dim3(256), 0, s, words, W, len, count, (double*) X);
+    else
+        hipLaunchKernelGGL(rng_expand_kernel<float>, dim3((unsigned) grid),
+                           dim3(256), 0, s, words, W, len, count, (float*) X);
+    SLQ_CHECK(hipGetLastError());
+    return SLQ_OK;
+}
+
+}  // namespace slq

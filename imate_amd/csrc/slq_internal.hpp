@@ -1,0 +1,160 @@
+// Internal declarations of libslq_hip.so (gfx950). Not part of the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <memory>
+#include <vector>
+
+#include "slq_hip.h"
+
+namespace slq {
+
+// ---------------------------------------------------------------- errors
+
+void set_error(const char* fmt, ...);
+int hip_status(hipError_t e, const char* what, const char* file, int line);
+
+#define SLQ_CHECK(expr)                                                      \
+    do {                                                                     \
+        hipError_t slq_e_ = (expr);                                          \
+        if (slq_e_ != hipSuccess)                                            \
+            return ::slq::hip_status(slq_e_, #expr, __FILE__, __LINE__);     \
+    } while (0)
+
+#define SLQ_TRY(expr)                                                        \
+    do {                                                                     \
+        int slq_r_ = (expr);                                                 \
+        if (slq_r_ != SLQ_OK) return slq_r_;                                 \
+    } while (0)
+
+// ---------------------------------------------------------------- layout
+
+// Probe blocks are row-major [len][kBlock]: entry (i, p) of probe p sits at
+// i * kBlock + p, so one wavefront reads or writes a whole row of the block
+// (kBlock consecutive values) per instruction and a CSR gather of column c
+// fetches the contiguous row c of the block.
+constexpr int kBlock = 64;
+constexpr int kWave = 64;
+
+// Persistent-style grids: workgroups of kThreads threads (kWavesPerWG
+// wavefronts), each taking a contiguous run of rows.
+constexpr int kThreads = 512;
+constexpr int kWavesPerWG = kThreads / kWave;
+constexpr int kXcds = 8;           // MI355X: 8 XCDs, workgroups round-robin
+constexpr int kMaxGrid = 2048;     // upper bound of reduction partials
+
+// ---------------------------------------------------------------- matrices
+
+// A CSR matrix in device memory: int64 row pointers, int32 column indices,
+// values of type dtype (SLQ_F64 / SLQ_F32). Owned by a MatrixStore.
+struct DevCSR {
+    int64_t nrows = 0;
+    int64_t ncols = 0;
+    int64_t nnz = 0;
+    int dtype = SLQ_F64;
+    int64_t* indptr = nullptr;
+    int32_t* indices = nullptr;
+    void* data = nullptr;
+};
+
+// Device copy of a matrix plus derived forms built on demand: the transpose
+// (CSR of A^T, used as a gather for A^T u) and an FP32 copy of the values of
+// an FP64 matrix (for the mixed-precision form).
+struct MatrixStore {
+    DevCSR csr;
+    DevCSR trans;          // valid when has_trans
+    bool has_trans = false;
+    float* data_f32 = nullptr;        // FP32 values of csr (mixed precision)
+    float* trans_data_f32 = nullptr;  // FP32 values of trans
+    ~MatrixStore();
+};
+
+// Device buffers reused across calls on one handle.
+struct Workspace {
+    size_t bytes = 0;
+    void* base = nullptr;
+    ~Workspace();
+    int reserve(size_t need);
+};
+
+struct ProfileState {
+    bool enabled = false;
+    double spmm_ms = 0.0;
+    int64_t spmm_launches = 0;
+    double update_ms = 0.0;
+    int64_t update_launches = 0;
+    std::vector<hipEvent_t> pool;            // free events
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> spmm_pending;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> update_pending;
+};
+
+}  // namespace slq
+
+enum OpKind { OP_MATRIX = 0, OP_AFFINE_I = 1, OP_AFFINE_B = 2 };
+
+struct slq_op {
+    int kind = OP_MATRIX;
+    std::shared_ptr<slq::MatrixStore> A;
+    std::shared_ptr<slq::MatrixStore> B;   // OP_AFFINE_B only
+    hipStream_t stream = nullptr;
+    int device = 0;
+    slq::Workspace ws;
+    slq::ProfileState prof;
+};
+
+namespace slq {
+
+// ---------------------------------------------------------------- kernels
+
+// Rademacher probes: words [count][W] of the thread-0 stream starting at
+// sample first_sample, expanded into X [len][kBlock] (probes >= count zero).
+int rng_words(hipStream_t s, const uint64_t base_state[4], int64_t W,
+              int64_t first_sample, int32_t count, uint64_t* words);
+int rng_expand(hipStream_t s, const uint64_t* words, int64_t W, int64_t len,
+               int32_t count, int dtype, void* X);
+void thread_state(int64_t seed, int thread_id, uint64_t out[4]);
+
+// Lanczos / Golub-Kahn over one probe block; outputs alpha/beta [m][kBlock]
+// (FP64, device) and sizes [kBlock] (device).
+struct RecurrenceArgs {
+    slq_op* op = nullptr;
+    double t = 0.0;             // affine parameter
+    int gram = 0;
+    int m = 0;
+    double tol = 0.0;
+    int ortho = 0;
+    int precision = SLQ_PREC_NATIVE;
+    const void* X0 = nullptr;   // probe block [len][kBlock] (value type of the run)
+    double* alpha = nullptr;    // device [m][kBlock]
+    double* beta = nullptr;     // device [m][kBlock]
+    int32_t* sizes = nullptr;   // device [kBlock]
+};
+int run_recurrence(const RecurrenceArgs& a, void* scratch, size_t scratch_bytes);
+size_t recurrence_scratch_bytes(const slq_op* op, int gram, int m, int ortho,
+                                int value_bytes);
+
+// Device eigensolve + quadrature for `count` probes.
+int run_quadrature(hipStream_t s, const double* alpha, const double* beta,
+                   const int32_t* sizes, int count, int m, int gram, int func,
+                   double exponent, double n, const double* params_dev, int q,
+                   int shift, int j_out, int q_stride, double* samples_dev,
+                   double* theta_dev, double* tau_dev);
+
+// Plain block SpMM (tests): Y = (A + tB) X or its transpose.
+int run_spmm(slq_op* op, double t, int transpose, const void* X, void* Y,
+             int value_dtype);
+
+// Transpose of a device CSR (stable: rows ascending within a column).
+int build_transpose(hipStream_t s, const DevCSR& A, DevCSR& T);
+int ensure_f32(hipStream_t s, MatrixStore& M, bool need_trans);
+
+// Profiling hooks around kernel launches.
+void prof_begin(slq_op* op, hipEvent_t* ev);
+void prof_end(slq_op* op, hipEvent_t ev0, int kind);
+void prof_collect(slq_op* op);
+
+int copy_bandwidth(int64_t bytes, int iters, double* ms);
+
+}  // namespace slq

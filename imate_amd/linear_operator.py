@@ -1,0 +1,240 @@
+"""Linear operators of the drop-in: ``LinearOperator``, ``Matrix`` and
+``AffineMatrixFunction``, with the interface of
+imate/linear_operator/{linear_operator,matrix,affine_matrix_function}.py.
+
+The operator is uploaded lazily, once, on first GPU use (like
+``Matrix.initialize``, matrix.py:286-352) and reused by every later call, so
+passing a ``Matrix`` instead of a scipy matrix keeps A resident in HBM across
+calls (the usage imate recommends, logdet/_slq_method.pyx:218-222).
+
+Only the GPU path exists in this package: ``get_linear_operator(gpu=False)``
+raises, since the CPU path of imate is not part of the MI355X build.
+"""
+
+import ctypes
+
+import numpy
+import scipy.sparse
+
+from . import _lib
+
+__all__ = ['LinearOperator', 'Matrix', 'AffineMatrixFunction']
+
+
+class DeviceOperator(object):
+    """Owner of one ``slq_op*`` handle (device copy of the operator)."""
+
+    def __init__(self, handle, parents=()):
+        self.handle = handle
+        self._parents = parents  # keep referenced handles alive
+
+    @staticmethod
+    def from_matrix(A):
+        lib = _lib.load()
+        if A.ndim != 2:
+            raise ValueError('Input matrix should be a 2-dimensional array.')
+        code = _lib.dtype_code(A.dtype)
+        if scipy.sparse.issparse(A):
+            M = scipy.sparse.csr_matrix(A)
+        else:
+            M = scipy.sparse.csr_matrix(numpy.asarray(A))
+        if not M.has_sorted_indices:
+            M = M.copy()
+            M.sort_indices()
+        indptr = numpy.ascontiguousarray(M.indptr, dtype=numpy.int64)
+        indices = numpy.ascontiguousarray(M.indices, dtype=numpy.int32)
+        data = numpy.ascontiguousarray(M.data, dtype=A.dtype)
+        h = ctypes.c_void_p()
+        _lib.check(lib.slq_op_create_csr(
+            M.shape[0], M.shape[1], M.nnz, _lib.ptr(indptr),
+            _lib.ptr(indices), _lib.ptr(data), code, ctypes.byref(h)),
+            'slq_op_create_csr')
+        return DeviceOperator(h)
+
+    @staticmethod
+    def affine(A_dev, B_dev=None):
+        lib = _lib.load()
+        h = ctypes.c_void_p()
+        _lib.check(lib.slq_op_create_affine(
+            A_dev.handle, B_dev.handle if B_dev is not None else None,
+            ctypes.byref(h)), 'slq_op_create_affine')
+        return DeviceOperator(h, parents=(A_dev, B_dev))
+
+    def get_device_properties(self):
+        return _lib.device_properties()
+
+    def __del__(self):
+        h = getattr(self, 'handle', None)
+        if h is not None and h.value and _lib._lib is not None:
+            _lib._lib.slq_op_destroy(h)
+            self.handle = None
+
+
+class LinearOperator(object):
+    """Base class (imate/linear_operator/linear_operator.py:14-207)."""
+
+    def __init__(self):
+        self.Aop = None
+        self.cpu_Aop = None
+        self.gpu_Aop = None
+        self.gpu = False
+        self.num_gpu_devices = 0
+        self.initialized_on_cpu = False
+        self.initialized_on_gpu = False
+        self.data_type_name = None
+        self.num_parameters = None
+
+    def initialize(self, gpu, num_gpu_devices):
+        raise NotImplementedError('This method should be called by a derived' +
+                                  ' class.')
+
+    def get_num_rows(self):
+        raise NotImplementedError('This method should be called by a derived' +
+                                  ' class.')
+
+    def get_num_columns(self):
+        raise NotImplementedError('This method should be called by a derived' +
+                                  ' class.')
+
+    def is_sparse(self):
+        raise NotImplementedError('This method should be called by a derived' +
+                                  ' class.')
+
+    def get_nnz(self):
+        raise NotImplementedError('This method should be called by a derived' +
+                                  ' class.')
+
+    def get_density(self):
+        raise NotImplementedError('This method should be called by a derived' +
+                                  ' class.')
+
+    def get_linear_operator(self, gpu=False, num_gpu_devices=0):
+        """The device operator (linear_operator.py:114-136)."""
+        self.initialize(gpu, num_gpu_devices)
+        if gpu:
+            if self.gpu_Aop is None or not self.initialized_on_gpu:
+                raise RuntimeError('Matrix is not initialized on gpu.')
+            self.Aop = self.gpu_Aop
+        else:
+            if self.cpu_Aop is None or not self.initialized_on_cpu:
+                raise RuntimeError('Matrix is not initialized on cpu.')
+            self.Aop = self.cpu_Aop
+        return self.Aop
+
+    def set_data_type_name(self, A):
+        if A.dtype == numpy.float32:
+            self.data_type_name = b'float32'
+        elif A.dtype == numpy.float64:
+            self.data_type_name = b'float64'
+        elif str(A.dtype) in ('float128', 'longdouble'):
+            self.data_type_name = b'float128'
+        else:
+            raise TypeError('Data type should be "float32", "float64", or ' +
+                            '"float128".')
+
+    def get_data_type_name(self):
+        if self.data_type_name is None:
+            raise RuntimeError('"data_type_name" is None.')
+        return self.data_type_name
+
+    def get_num_parameters(self):
+        if self.num_parameters is None:
+            raise RuntimeError('"num_parameters" is None.')
+        return self.num_parameters
+
+    def _initialize_cpu(self):
+        raise NotImplementedError(
+            'imate_amd implements the GPU path only (gpu=True); the CPU '
+            'path (gpu=False) is served by imate itself.')
+
+    def _nnz(self, A):
+        if scipy.sparse.issparse(A):
+            return A.nnz
+        return A.shape[0] * A.shape[1]
+
+
+class Matrix(LinearOperator):
+    """A matrix as a linear operator (imate/linear_operator/matrix.py)."""
+
+    def __init__(self, A):
+        super(Matrix, self).__init__()
+        self.A = A
+        self.num_parameters = 0
+        self.set_data_type_name(self.A)
+
+    def initialize(self, gpu, num_gpu_devices):
+        if gpu:
+            if not self.initialized_on_gpu:
+                self.gpu_Aop = DeviceOperator.from_matrix(self.A)
+                self.initialized_on_gpu = True
+        else:
+            self._initialize_cpu()
+        self.gpu = gpu
+        self.num_gpu_devices = num_gpu_devices
+
+    def get_num_rows(self):
+        return self.A.shape[0]
+
+    def get_num_columns(self):
+        return self.A.shape[1]
+
+    def is_sparse(self):
+        return scipy.sparse.issparse(self.A)
+
+    def get_nnz(self):
+        return self._nnz(self.A)
+
+    def get_density(self):
+        if self.is_sparse():
+            return self.get_nnz() / \
+                (self.get_num_rows() * self.get_num_columns())
+        return 1.0
+
+
+class AffineMatrixFunction(LinearOperator):
+    """``A + t B`` with parameter t; ``B=None`` means the identity, for which
+    one Lanczos run per probe serves every t by a Ritz shift
+    (imate/linear_operator/affine_matrix_function.py)."""
+
+    def __init__(self, A, B=None):
+        super(AffineMatrixFunction, self).__init__()
+        self.A = A
+        self.B = B
+        self.num_parameters = 1
+        self.set_data_type_name(self.A)
+        self._A_dev = None
+        self._B_dev = None
+
+    def initialize(self, gpu, num_gpu_devices):
+        if gpu:
+            if not self.initialized_on_gpu:
+                self._A_dev = DeviceOperator.from_matrix(self.A)
+                if self.B is not None:
+                    B = self.B
+                    if B.dtype != self.A.dtype:
+                        B = B.astype(self.A.dtype)
+                    self._B_dev = DeviceOperator.from_matrix(B)
+                self.gpu_Aop = DeviceOperator.affine(self._A_dev, self._B_dev)
+                self.initialized_on_gpu = True
+        else:
+            self._initialize_cpu()
+        self.gpu = gpu
+        self.num_gpu_devices = num_gpu_devices
+
+    def get_num_rows(self):
+        return self.A.shape[0]
+
+    def get_num_columns(self):
+        return self.A.shape[1]
+
+    def is_sparse(self):
+        return scipy.sparse.issparse(self.A)
+
+    def get_nnz(self):
+        return self._nnz(self.A)
+
+    def get_density(self):
+        if self.is_sparse():
+            return self.get_nnz() / \
+                (self.get_num_rows() * self.get_num_columns())
+        return 1.0

@@ -1,0 +1,20 @@
+"""``logdet(A, gram, p, return_info, method, **options)`` (imate/logdet/logdet.py:23-398).
+
+logdet(A^p) = p log|det A| by stochastic Lanczos quadrature with
+f = log (imate/functions/logarithm.cpp:42-45); with ``gram=True`` the Gramian
+A^T A is used through Golub-Kahn bidiagonalization. Only ``method='slq'`` is
+part of this build.
+"""
+
+from ._slq import slq_method
+
+
+def logdet(A, gram=False, p=1.0, return_info=False, method='cholesky',
+           **options):
+    if method == 'slq':
+        return slq_method('log', A, gram=gram, p=p, return_info=return_info,
+                          **options)
+    if method in ('eigenvalue', 'cholesky'):
+        raise NotImplementedError(
+            'Method "%s" is not part of imate_amd; use method="slq".' % method)
+    raise ValueError('Method "%s" is invalid.' % (method))

@@ -1,0 +1,182 @@
+/*
+ * slq_hip.h — C ABI of libslq_hip.so, the MI355X (gfx950) implementation of
+ * stochastic Lanczos quadrature (SLQ) trace estimation behind
+ * imate.logdet / traceinv / trace / schatten(A, method='slq', gpu=True).
+ *
+ * Plain C types only: host pointers and sizes in, status codes out. Every
+ * call is blocking; the library copies host arrays during the call and keeps
+ * no host pointer. Device memory lives in slq_op handles (one HIP stream per
+ * handle). Calls on one handle are not re-entrant; separate handles may be
+ * used from separate threads.
+ *
+ * Each entry point names the reference interface it replaces
+ * (imate 0.25.2, paths relative to the reference root).
+ */
+#ifndef SLQ_HIP_H_
+#define SLQ_HIP_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes (negative = error). The message of the last error of the
+ * calling thread is returned by slq_last_error(). Replaces the abort() and
+ * assert() exits of imate/_cu_linear_operator/cu_linear_operator.cu:77-85,
+ * 251-270 and imate/_c_trace_estimator/c_orthogonalization.cpp. */
+#define SLQ_OK 0
+#define SLQ_E_ARG -1       /* invalid argument (ValueError / TypeError) */
+#define SLQ_E_NODEVICE -2  /* no HIP device */
+#define SLQ_E_OOM -3       /* device allocation failed */
+#define SLQ_E_HIP -4       /* any other HIP runtime error */
+#define SLQ_E_NOTSPD -5    /* a Ritz value left the domain of f (e.g. log) */
+
+/* Value types (the matrix data type; imate accepts float32/float64 on GPU,
+ * py_cu_matrix.pyx:136-142). */
+#define SLQ_F64 0
+#define SLQ_F32 1
+
+/* Matrix functions of imate/functions: applied as f(theta^p) in the
+ * quadrature, c_trace_estimator.cpp:616-617. */
+#define SLQ_FUNC_LOG 0      /* functions/logarithm.cpp:42-45 (logdet) */
+#define SLQ_FUNC_INV 1      /* functions/inverse.cpp:41-44   (traceinv) */
+#define SLQ_FUNC_IDENT 2    /* functions/identity.cpp:41-44  (trace) */
+
+/* Precision modes of the recurrences. */
+#define SLQ_PREC_NATIVE 0   /* vectors and SpMM in the matrix dtype, FP64 recurrences */
+#define SLQ_PREC_MIXED 1    /* FP32 SpMM / basis with FP64 alpha, beta and eigensolve */
+
+typedef struct slq_op slq_op;
+
+/* Last error message of the calling thread (always NUL-terminated). */
+int slq_last_error(char* buf, size_t len);
+
+/* Device count and properties of device 0.
+ * Replaces pycuLinearOperator.get_device_properties
+ * (imate/_cu_linear_operator/py_cu_linear_operator.pyx:195) and
+ * imate/_cuda_utilities/query_device.cu:35-48. */
+int slq_device_properties(int* num_devices, int* num_multiprocessors,
+                          int* num_threads_per_multiprocessor,
+                          char* name, size_t name_len);
+
+/* Upload a CSR matrix (sorted column indices) to the device.
+ * Replaces pycuMatrix / cuCSRMatrix<T> construction and the host-to-device
+ * upload (imate/_cu_linear_operator/py_cu_matrix.pyx:118-200,365-407;
+ * cu_csr_matrix.cu:52-84,164-224), with 64-bit row pointers instead of the
+ * reference's int32 (types.h:60). dtype is SLQ_F64 or SLQ_F32 and gives the
+ * element type of `data`. */
+int slq_op_create_csr(int64_t nrows, int64_t ncols, int64_t nnz,
+                      const int64_t* indptr, const int32_t* indices,
+                      const void* data, int dtype, slq_op** out);
+
+/* Affine operator A + tB over existing handles (B == NULL means B = I, for
+ * which the Ritz values of A + tI follow from one Lanczos run by a shift).
+ * The new handle references A's (and B's) device arrays; destroy it before
+ * them. Replaces pycuAffineMatrixFunction / cuCSRAffineMatrixFunction
+ * (imate/_cu_linear_operator/py_cu_affine_matrix_function.pyx:31,171;
+ * cu_csr_affine_matrix_function.cu:29-211). */
+int slq_op_create_affine(slq_op* A, slq_op* B, slq_op** out);
+
+/* Free a handle (pycuMatrix.__dealloc__, cu_csr_matrix.cu:94-154). */
+int slq_op_destroy(slq_op* op);
+
+/* SLQ trace estimation of f((A + t B)^p) (or of (A^T A)^p with gram != 0,
+ * by Golub-Kahn bidiagonalization).
+ * Replaces pycu_trace_estimator / cuTraceEstimator<T>::cu_trace_estimator
+ * (imate/_cu_trace_estimator/py_cu_trace_estimator.pyx:36-141,263-376;
+ * cu_trace_estimator.cu:197-310,419-~670). Arguments have the meaning of
+ * imate's trace_estimator (imate/_trace_estimator/trace_estimator.pyx:38-390):
+ *   params[num_inquiries]  t values of an affine operator (ignored otherwise)
+ *   func, exponent         f and p of f(theta^p)
+ *   orthogonalize          0 none, <0 or >= lanczos_degree full, else window
+ *   seed                   >= 0 reproducible; < 0 uses the clock
+ * Probe vectors are the Rademacher vectors of imate's thread-0 xoshiro256**
+ * stream (sample i uses words [i*W, (i+1)*W), W = floor(len/64) + 1), i.e. the
+ * reference's reproducible configuration with one device.
+ * block_size: probes per device block (0 = library default);
+ * precision: SLQ_PREC_NATIVE or SLQ_PREC_MIXED.
+ * Outputs (caller-allocated host arrays, like trace_estimator.pyx:151-174):
+ *   trace[q], error[q], samples[max_num_samples][q] (row-major, processing
+ *   order, NaN where not computed), processed_samples_indices[max],
+ *   num_samples_used[q], num_outliers[q], converged[q], *all_converged,
+ *   *alg_wall_time (seconds, the sampling loop only). */
+int slq_estimate(slq_op* op, const double* params, int32_t num_inquiries,
+                 int32_t func, int32_t gram, double exponent,
+                 int32_t orthogonalize, int64_t seed, int32_t lanczos_degree,
+                 double lanczos_tol, int32_t min_num_samples,
+                 int32_t max_num_samples, double error_atol, double error_rtol,
+                 double confidence_level, double outlier_significance_level,
+                 int32_t block_size, int32_t precision,
+                 double* trace, double* error, double* samples,
+                 int32_t* processed_samples_indices, int32_t* num_samples_used,
+                 int32_t* num_outliers, int32_t* converged,
+                 int32_t* all_converged, double* alg_wall_time);
+
+/* ---- component entry points (tests, calibration and benchmarks) ---- */
+
+/* `count` consecutive Rademacher probes of length n, starting at sample
+ * `first_sample` of the thread-0 stream of `seed`, written probe-major into
+ * host `out[count][n]` (dtype SLQ_F64 / SLQ_F32), and the raw stream words
+ * into `words[count][floor(n/64)+1]` when non-NULL.
+ * Replaces RandomArrayGenerator<T>::generate_random_array
+ * (imate/_random_generator/random_array_generator.cpp:62-165) fed by
+ * RandomNumberGenerator (random_number_generator.cpp:70-153). */
+int slq_rademacher(int64_t seed, int64_t n, int64_t first_sample,
+                   int32_t count, int dtype, void* out, uint64_t* words);
+
+/* Y = op * X (transpose != 0: op^T * X) for a block of b vectors stored
+ * row-major X[len][b], in the op's dtype, host buffers.
+ * Replaces cuCSRMatrix::dot / transpose_dot and the affine variants
+ * (imate/_cu_linear_operator/cu_csr_matrix.cu:277-318,373-414;
+ * cu_csr_affine_matrix_function.cu:123-155), one vector at a time there. */
+int slq_op_spmm(slq_op* op, double t, int32_t b, int32_t transpose,
+                const void* X, void* Y);
+
+/* Recurrence coefficients of `count` consecutive probes (from first_sample
+ * of the thread-0 stream of seed) at parameter t: alpha[count][m],
+ * beta[count][m] (FP64) and sizes[count].
+ * Replaces cu_lanczos_tridiagonalization / cu_golub_kahn_bidiagonalization
+ * (imate/_cu_trace_estimator/cu_lanczos_tridiagonalization.cu:119-248;
+ * cu_golub_kahn_bidiagonalization.cu:113-242). */
+int slq_recurrences(slq_op* op, double t, int32_t gram, int64_t seed,
+                    int64_t first_sample, int32_t count, int32_t lanczos_degree,
+                    double lanczos_tol, int32_t orthogonalize,
+                    int32_t precision, double* alpha, double* beta,
+                    int32_t* sizes);
+
+/* Device eigensolve + Gauss quadrature of `count` recurrences:
+ * theta/tau of the tridiagonal (gram == 0: alpha diagonal, beta sub-diagonal)
+ * or of the bidiagonal's squared singular values (gram != 0), then
+ * samples[count][q] = n * sum_i tau_i^2 f((theta_i - params[0] + params[j])^p)
+ * (shift only when shift != 0; else one inquiry). Outputs theta/tau
+ * [count][m] when non-NULL.
+ * Replaces Diagonalization::eigh_tridiagonal / svd_bidiagonal (LAPACK ?stev /
+ * ?bdsdc, imate/_c_trace_estimator/diagonalization.cpp:88-118,189-231) and the
+ * quadrature of cu_trace_estimator.cu:~555-630. */
+int slq_quadrature(const double* alpha, const double* beta,
+                   const int32_t* sizes, int32_t count, int32_t m,
+                   int32_t gram, int32_t func, double exponent, double n,
+                   const double* params, int32_t q, int32_t shift,
+                   double* samples, double* theta, double* tau);
+
+/* Per-kernel timing with HIP events on the handle's stream.
+ * enable != 0 starts collecting (and clears the totals). slq_profile_read
+ * returns the summed duration (ms) and launch count of the SpMM kernel and of
+ * the update kernel since enabling. No reference counterpart: imate times
+ * only the whole loop (imate/_cuda_utilities/cuda_timer.cu:28-84). */
+int slq_profile_enable(slq_op* op, int32_t enable);
+int slq_profile_read(slq_op* op, double* spmm_ms, int64_t* spmm_launches,
+                     double* update_ms, int64_t* update_launches);
+
+/* Device copy of `bytes` bytes (a plain streaming kernel) timed with HIP
+ * events over `iters` launches; used to calibrate rocprof byte counters and
+ * the achievable HBM bandwidth. No reference counterpart. */
+int slq_copy_bandwidth(int64_t bytes, int32_t iters, double* ms_per_launch);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SLQ_HIP_H_ */
