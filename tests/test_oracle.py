@@ -1,0 +1,288 @@
+"""Pins the CPU oracle (CPU only, no GPU).
+
+1. Against fixtures written from the reference's own compiled CPU sources
+   (tests/golden/ref_vectors.npz, see make_golden.py): RNG streams,
+   erf_inv, Lanczos / Golub-Kahn recurrences (all re-orthogonalisation modes,
+   affine A + tI and A + tB, FP32 and FP64) and the convergence tools, all
+   bit for bit.
+2. Against the reference build directly (oracle/_ref) when it is present.
+3. Against mathematics: published SplitMix64 outputs, jump() equal to 2^128
+   steps via GF(2) matrix powers, Gauss quadrature exactness, GK on A equal
+   to Lanczos on A^T A, breakdown on matrices with few distinct eigenvalues,
+   scipy's erfinv, Toeplitz closed forms and a Cholesky log-determinant.
+"""
+
+import math
+import os
+
+import numpy
+import pytest
+import scipy.sparse
+import scipy.special
+
+from oracle import oracle
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLD = os.path.join(HERE, 'golden', 'ref_vectors.npz')
+
+
+@pytest.fixture(scope='module')
+def gold():
+    return numpy.load(GOLD, allow_pickle=False)
+
+
+def _mat(g, name):
+    shape = tuple(g['mat_%s_shape' % name])
+    return scipy.sparse.csr_matrix((g['mat_%s_data' % name],
+                                    g['mat_%s_indices' % name],
+                                    g['mat_%s_indptr' % name]), shape=shape)
+
+
+# ---------------------------------------------------------------- fixtures
+
+def test_rng_streams_match_reference(gold):
+    c = 0
+    while 'rng_%d_case' % c in gold:
+        seed, nt, tid = (int(v) for v in gold['rng_%d_case' % c])
+        s = oracle.thread_state(seed, tid)
+        w = oracle.xoshiro_words(s, 40)
+        assert numpy.array_equal(w, gold['rng_%d_words' % c]), (seed, nt, tid)
+        c += 1
+    assert c >= 6
+
+
+def test_erf_inv_matches_reference(gold):
+    for x, y in zip(gold['erfinv_x'], gold['erfinv_y']):
+        assert oracle.erf_inv(float(x)) == float(y)
+
+
+def test_recurrences_match_reference(gold):
+    c = 0
+    while 'rec_%d_spec' % c in gold:
+        spec = [str(v) for v in gold['rec_%d_spec' % c]]
+        mname, bname, gram, ortho, kind, t, dt, m, ns, seed = spec
+        dtype = numpy.dtype(dt)
+        A = _mat(gold, mname)
+        B = _mat(gold, bname) if bname else None
+        op = oracle.Operator(A, dtype=dtype, B=B, affine=int(kind) > 0,
+                             t=float(t))
+        a, b, sz = oracle.recurrences(op, int(seed), int(ns), int(m),
+                                      float(numpy.finfo(dtype).eps),
+                                      int(ortho), gram=bool(int(gram)))
+        assert numpy.array_equal(sz, gold['rec_%d_sizes' % c]), spec
+        assert numpy.array_equal(a, gold['rec_%d_alpha' % c]), spec
+        assert numpy.array_equal(b, gold['rec_%d_beta' % c]), spec
+        c += 1
+    assert c >= 14
+
+
+def test_convergence_tools_match_reference(gold):
+    S = gold['conv_samples']
+    processed = gold['conv_processed']
+    q = S.shape[1]
+    err = numpy.zeros(q)
+    used = numpy.zeros(q, numpy.int32)
+    conv = numpy.zeros(q, numpy.int32)
+    flags = []
+    for k in range(1, S.shape[0] + 1):
+        flags.append(oracle.check_convergence(S, 10, processed, k, 0.95, 0.0,
+                                              0.05, err, used, conv))
+    assert flags == list(gold['conv_flags'])
+    assert numpy.array_equal(err, gold['conv_error'])
+    assert numpy.array_equal(used, gold['conv_used'])
+    assert numpy.array_equal(conv, gold['conv_converged'])
+    nout = numpy.zeros(q, numpy.int32)
+    trace = numpy.zeros(q)
+    err2 = numpy.zeros(q)
+    oracle.average_estimates(S, 0.95, 0.01, gold['avg_used'], processed,
+                             nout, trace, err2)
+    assert numpy.array_equal(nout, gold['avg_num_outliers'])
+    assert numpy.array_equal(trace, gold['avg_trace'])
+    assert numpy.array_equal(err2, gold['avg_error'])
+    assert nout.sum() >= 1   # the planted outliers were found
+
+
+@pytest.mark.skipif(oracle.ref_lib() is None, reason='reference not built')
+def test_oracle_equals_reference_build_sweep():
+    """Wider sweep straight against the reference build (where present)."""
+    import ctypes
+    sys_path_gold = os.path.join(HERE, 'golden')
+    import sys
+    sys.path.insert(0, sys_path_gold)
+    from make_golden import ref_recurrences
+    R = oracle.ref_lib()
+    rng = numpy.random.default_rng(5)
+    for trial in range(6):
+        n = int(rng.integers(2, 130))
+        M = scipy.sparse.random(n, n, density=0.1, random_state=trial)
+        A = (M @ M.T + scipy.sparse.eye(n)).tocsr()
+        for gram, ortho in ((0, 0), (0, -1), (1, 0), (1, 3)):
+            a, b, sz = ref_recurrences(R, A, gram, trial, 3, 12,
+                                       float(numpy.finfo(float).eps), ortho)
+            op = oracle.Operator(A)
+            a2, b2, sz2 = oracle.recurrences(op, trial, 3, 12,
+                                             float(numpy.finfo(float).eps),
+                                             ortho, gram=bool(gram))
+            assert numpy.array_equal(sz, sz2)
+            assert numpy.array_equal(a, a2)
+            assert numpy.array_equal(b, b2)
+    del ctypes
+
+
+# ---------------------------------------------------------------- RNG maths
+
+def test_splitmix64_published_values():
+    # SplitMix64 from state 0 (seed 0 gives state (0 << 32) | 0 = 0)
+    assert list(oracle.splitmix(0, 3)) == [
+        0xe220a8397b1dcdaf, 0x6e789e6aa1b965f4, 0x06c45d188009454f]
+
+
+def _step_py(s):
+    """xoshiro256** state transition written out independently."""
+    M64 = (1 << 64) - 1
+    s0, s1, s2, s3 = s
+    t = (s1 << 17) & M64
+    s2 ^= s0
+    s3 ^= s1
+    s1 ^= s2
+    s0 ^= s3
+    s2 ^= t
+    s3 = ((s3 << 45) | (s3 >> 19)) & M64
+    return [s0, s1, s2, s3]
+
+
+def _bits(s):
+    return numpy.array([(s[w] >> b) & 1 for w in range(4) for b in range(64)],
+                       dtype=numpy.int64)
+
+
+def _from_bits(v):
+    out = [0, 0, 0, 0]
+    for j in range(256):
+        if v[j]:
+            out[j // 64] |= 1 << (j % 64)
+    return out
+
+
+def test_jump_is_2_pow_128_steps():
+    M = numpy.zeros((256, 256), dtype=numpy.int64)
+    for j in range(256):
+        e = [0, 0, 0, 0]
+        e[j // 64] = 1 << (j % 64)
+        M[:, j] = _bits(_step_py(e))
+    P = M.copy()
+    for _ in range(128):
+        P = (P @ P) % 2
+    s = oracle.thread_state(99, -1)  # no jump: raw SplitMix64 state
+    s_py = [int(v) for v in s]
+    expect = _from_bits((P @ _bits(s_py)) % 2)
+    oracle.xoshiro_jump(s)
+    assert [int(v) for v in s] == expect
+
+
+def test_rademacher_layout_and_tail_word():
+    for n in (1, 63, 64, 65, 128, 200):
+        W = n // 64 + 1
+        s = oracle.thread_state(4, 0)
+        words = oracle.xoshiro_words(s, 2 * W)
+        probes = oracle.rademacher(4, n, 2)
+        for p in range(2):
+            for i in range(n):
+                bit = (int(words[p * W + i // 64]) >> (i % 64)) & 1
+                assert probes[p, i] == (1.0 if bit else -1.0)
+
+
+# ---------------------------------------------------------------- Lanczos maths
+
+def _slq_one(A, func, p=1.0, m=10, seed=3, **kw):
+    return oracle.slq(A, func=func, exponent=p, lanczos_degree=m,
+                      min_num_samples=4, max_num_samples=4, seed=seed, **kw)
+
+
+def test_scaled_identity_stops_after_one_step():
+    n, c = 37, 3.5
+    A = scipy.sparse.identity(n, format='csr') * c
+    op = oracle.Operator(A)
+    a, b, sz = oracle.recurrences(op, 1, 3, 10, float(numpy.finfo(float).eps), 0)
+    assert list(sz) == [1, 1, 1]
+    r = _slq_one(A, 'log')
+    assert numpy.allclose(r['samples'][:, 0], n * math.log(c), rtol=1e-14)
+
+
+def test_few_distinct_eigenvalues_break_down():
+    # three distinct eigenvalues: the Krylov space has dimension 3, so with a
+    # tolerance above round-off the recurrence stops after 3 steps and the
+    # 3-node Gauss rule is exact for the 3-point spectral measure.
+    d = numpy.repeat([1.0, 2.0, 5.0], 40)
+    A = scipy.sparse.diags(d).tocsr()
+    op = oracle.Operator(A)
+    a, b, sz = oracle.recurrences(op, 2, 5, 20, 1e-10, 0)
+    assert list(sz) == [3] * 5
+    r = oracle.slq(A, func='log', lanczos_degree=20, lanczos_tol=1e-10,
+                   min_num_samples=3, max_num_samples=3, seed=2)
+    exact = 40 * (math.log(1.0) + math.log(2.0) + math.log(5.0))
+    assert numpy.allclose(r['samples'][:, 0], exact, rtol=1e-12)
+
+
+def test_gauss_quadrature_exact_for_low_degree():
+    from imate_amd.sample_matrices import sparse_correlation_matrix
+    A = sparse_correlation_matrix(300, nnz_per_row=15, seed=2)
+    probes = oracle.rademacher(5, 300, 4)
+    r1 = _slq_one(A, 'identity', p=1.0, m=6, seed=5)
+    r2 = _slq_one(A, 'identity', p=2.0, m=6, seed=5)
+    for i in range(4):
+        v = probes[i]
+        assert abs(r1['samples'][i, 0] - v @ (A @ v)) < 1e-10 * abs(v @ (A @ v))
+        Av = A @ v
+        assert abs(r2['samples'][i, 0] - Av @ Av) < 1e-10 * abs(Av @ Av)
+
+
+def test_golub_kahn_equals_lanczos_on_gramian():
+    rng = numpy.random.default_rng(0)
+    G = scipy.sparse.random(120, 120, density=0.08, random_state=4) + \
+        scipy.sparse.eye(120) * 2
+    G = G.tocsr()
+    AtA = (G.T @ G).tocsr()
+    r_gk = _slq_one(G, 'log', gram=True, m=12)
+    r_l = _slq_one(AtA, 'log', m=12)
+    assert numpy.allclose(r_gk['samples'], r_l['samples'], rtol=1e-8)
+    del rng
+
+
+def test_erf_inv_against_scipy():
+    # two Newton steps on erf: ~1e-15 in the body, erf's flatness costs a
+    # few digits only in the far tail (|x| -> 1)
+    for x in numpy.linspace(-0.999999, 0.999999, 101):
+        tol = 1e-13 if abs(x) <= 0.9999 else 1e-11
+        assert abs(oracle.erf_inv(x) - scipy.special.erfinv(x)) <= \
+            tol * max(1.0, abs(scipy.special.erfinv(x)))
+
+
+@pytest.mark.parametrize('gram', [False, True])
+def test_toeplitz_closed_forms(gram):
+    from imate_amd.sample_matrices import (toeplitz, toeplitz_logdet,
+                                           toeplitz_traceinv)
+    n = 50
+    # logdet of A^T A for the bidiagonal A (gram) or of the SPD tridiagonal
+    A = toeplitz(2.0, 1.0, n, gram=not gram)
+    r = oracle.slq(A, func='log', gram=gram, lanczos_degree=30,
+                   orthogonalize=-1, min_num_samples=100,
+                   max_num_samples=200, seed=11)
+    exact = toeplitz_logdet(2.0, 1.0, n, gram=True)
+    assert abs(r['trace'][0] - exact) < 3 * r['error'][0] + 1e-9 * abs(exact)
+    r = oracle.slq(A, func='inv', gram=gram, lanczos_degree=30,
+                   orthogonalize=-1, min_num_samples=100,
+                   max_num_samples=200, seed=12)
+    exact = toeplitz_traceinv(2.0, 1.0, n, gram=True)
+    assert abs(r['trace'][0] - exact) < 3 * r['error'][0] + 1e-9 * abs(exact)
+
+
+def test_cholesky_logdet_c1():
+    from imate_amd.sample_matrices import sparse_correlation_matrix
+    A = sparse_correlation_matrix(4096, nnz_per_row=40, seed=0)
+    L = numpy.linalg.cholesky(A.toarray())
+    exact = 2.0 * numpy.sum(numpy.log(numpy.diag(L)))
+    r = oracle.slq(A, func='log', lanczos_degree=20, min_num_samples=100,
+                   max_num_samples=100, seed=0)
+    assert abs(r['trace'][0] - exact) <= 0.01 * abs(exact)
+    assert abs(r['trace'][0] - exact) <= 3 * r['error'][0]
