@@ -19,6 +19,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <ctime>
 #include <limits>
@@ -56,8 +57,19 @@ int hip_status(hipError_t e, const char* what, const char* file, int line)
 
 // ---------------------------------------------------------------- storage
 
+static void free_tiles(DevTiles& t)
+{
+    if (t.ptr) (void) hipFree(t.ptr);
+    if (t.cols) (void) hipFree(t.cols);
+    if (t.vals) (void) hipFree(t.vals);
+    if (t.vals_f32) (void) hipFree(t.vals_f32);
+    t = DevTiles();
+}
+
 MatrixStore::~MatrixStore()
 {
+    free_tiles(tiles);
+    free_tiles(trans_tiles);
     if (csr.indptr) (void) hipFree(csr.indptr);
     if (csr.indices) (void) hipFree(csr.indices);
     if (csr.data) (void) hipFree(csr.data);
@@ -84,6 +96,78 @@ int Workspace::reserve(size_t need)
     }
     SLQ_CHECK(hipMalloc(&base, need));
     bytes = need;
+    return SLQ_OK;
+}
+
+// ---------------------------------------------------------------- tiles
+
+int build_tiles(int64_t nrows, int64_t ncols, const int64_t* indptr,
+                const int32_t* indices, const void* data, int dtype, int R,
+                double min_density, DevTiles& out)
+{
+    (void) ncols;
+    out = DevTiles();
+    const int64_t nnz = indptr[nrows];
+    if (nnz == 0) return SLQ_OK;
+    const int64_t T = (nrows + R - 1) / R;
+    std::vector<int64_t> tptr(T + 1, 0);
+    std::vector<int32_t> cols;
+    cols.reserve((size_t) (nnz / 2 + 16));
+    std::vector<int32_t> buf;
+    for (int64_t t = 0; t < T; ++t)
+    {
+        buf.clear();
+        const int64_t r1 = std::min<int64_t>(nrows, (t + 1) * R);
+        for (int64_t row = t * R; row < r1; ++row)
+            buf.insert(buf.end(), indices + indptr[row], indices + indptr[row + 1]);
+        std::sort(buf.begin(), buf.end());
+        buf.erase(std::unique(buf.begin(), buf.end()), buf.end());
+        cols.insert(cols.end(), buf.begin(), buf.end());
+        tptr[t + 1] = (int64_t) cols.size();
+    }
+    const int64_t U = (int64_t) cols.size();
+    const double density = (double) nnz / ((double) U * R);
+    if (density < min_density) return SLQ_OK;
+    const size_t vb = dtype == SLQ_F64 ? 8 : 4;
+    std::vector<char> vals((size_t) U * R * vb, 0);
+    for (int64_t t = 0; t < T; ++t)
+    {
+        const int32_t* cb = cols.data() + tptr[t];
+        const int32_t* ce = cols.data() + tptr[t + 1];
+        const int64_t r1 = std::min<int64_t>(nrows, (t + 1) * R);
+        for (int64_t row = t * R; row < r1; ++row)
+        {
+            const int r = (int) (row - t * R);
+            const int32_t* p = cb;
+            for (int64_t k = indptr[row]; k < indptr[row + 1]; ++k)
+            {
+                p = std::lower_bound(p, ce, indices[k]);
+                const size_t slot = (size_t) (p - cols.data()) * R + r;
+                if (dtype == SLQ_F64)
+                    ((double*) vals.data())[slot] += ((const double*) data)[k];
+                else
+                    ((float*) vals.data())[slot] += ((const float*) data)[k];
+            }
+        }
+    }
+    DevTiles d;
+    d.R = R;
+    d.ntiles = T;
+    d.nslabs = U;
+    hipError_t e;
+    if ((e = hipMalloc(&d.ptr, sizeof(int64_t) * (T + 1))) != hipSuccess ||
+        (e = hipMalloc(&d.cols, sizeof(int32_t) * U)) != hipSuccess ||
+        (e = hipMalloc(&d.vals, vals.size())) != hipSuccess ||
+        (e = hipMemcpy(d.ptr, tptr.data(), sizeof(int64_t) * (T + 1),
+                       hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(d.cols, cols.data(), sizeof(int32_t) * U,
+                       hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(d.vals, vals.data(), vals.size(), hipMemcpyHostToDevice)) != hipSuccess)
+    {
+        free_tiles(d);
+        return hip_status(e, "tile upload", __FILE__, __LINE__);
+    }
+    out = d;
     return SLQ_OK;
 }
 
@@ -483,6 +567,26 @@ int prepare(slq_op* op, int gram, int precision)
             SLQ_TRY(build_transpose(op->stream, op->A->csr, op->A->trans));
             op->A->has_trans = true;
         }
+        if (!op->A->trans_tiles_tried && op->A->tiles.R > 0)
+        {
+            // tiled form of A^T, built on the host from the device transpose
+            op->A->trans_tiles_tried = true;
+            const DevCSR& T = op->A->trans;
+            const size_t vb = T.dtype == SLQ_F64 ? 8 : 4;
+            std::vector<int64_t> ip(T.nrows + 1);
+            std::vector<int32_t> ix(T.nnz > 0 ? T.nnz : 1);
+            std::vector<char> dv(vb * (T.nnz > 0 ? T.nnz : 1));
+            SLQ_CHECK(hipMemcpy(ip.data(), T.indptr, sizeof(int64_t) * (T.nrows + 1),
+                                hipMemcpyDeviceToHost));
+            if (T.nnz > 0)
+            {
+                SLQ_CHECK(hipMemcpy(ix.data(), T.indices, sizeof(int32_t) * T.nnz,
+                                    hipMemcpyDeviceToHost));
+                SLQ_CHECK(hipMemcpy(dv.data(), T.data, vb * T.nnz, hipMemcpyDeviceToHost));
+            }
+            SLQ_TRY(build_tiles(T.nrows, T.ncols, ip.data(), ix.data(), dv.data(), T.dtype,
+                                op->A->tiles.R, 0.25, op->A->trans_tiles));
+        }
         if (op->kind == OP_AFFINE_B && !op->B->has_trans)
         {
             SLQ_TRY(build_transpose(op->stream, op->B->csr, op->B->trans));
@@ -688,6 +792,16 @@ int slq_op_create_csr(int64_t nrows, int64_t ncols, int64_t nnz,
                                    hipMemcpyHostToDevice)) != hipSuccess) ||
         (nnz > 0 && (e = hipMemcpy(c.data, data, vb * nnz, hipMemcpyHostToDevice)) != hipSuccess))
         return fail(hip_status(e, "hipMemcpy", __FILE__, __LINE__));
+    {
+        const char* env = std::getenv("SLQ_TILE_ROWS");
+        const int R = env ? std::atoi(env) : 8;
+        if (R == 8 || R == 16)
+        {
+            int r = build_tiles(nrows, ncols, indptr, indices, data, dtype, R, 0.25,
+                                op->A->tiles);
+            if (r != SLQ_OK) return fail(r);
+        }
+    }
     *out = op;
     return SLQ_OK;
 }

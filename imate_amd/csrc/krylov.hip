@@ -123,7 +123,7 @@ __device__ __forceinline__ void wg_partial(double v, double* __restrict__ partia
 {
     __shared__ double red[kWavesPerWG][kWave];
     const int lane = threadIdx.x % kWave;
-    const int wave = threadIdx.x / kWave;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     red[wave][lane] = v;
     __syncthreads();
     if (wave == 0)
@@ -154,7 +154,7 @@ __device__ __forceinline__ bool last_block_total(const double* __restrict__ part
     if (!is_last) return false;
     __threadfence();
     const int lane = threadIdx.x % kWave;
-    const int wave = threadIdx.x / kWave;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     const int G = gridDim.x;
     double s0 = 0.0, s1 = 0.0;
     int g = wave;
@@ -238,7 +238,7 @@ template <typename T>
 __global__ __launch_bounds__(kThreads) void spmm_kernel(SpmmArgs<T> a)
 {
     const int lane = threadIdx.x % kWave;
-    const int wave = threadIdx.x / kWave;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     int64_t r0, r1;
     row_run(a.nrows, r0, r1);
     const T s = a.xscale ? (T) a.xscale[lane] : T(1);
@@ -289,6 +289,104 @@ __global__ __launch_bounds__(kThreads) void spmm_kernel(SpmmArgs<T> a)
     finish_norm(total, lane, a.j, a.tol_sqrt_n, a.red_out, a.scale_out, a.sizes);
 }
 
+
+// Row-tiled variant of spmm_kernel: one wavefront per tile of R rows, R
+// accumulators per lane; every gathered operand row X[u] feeds the R values
+// of column u (the slab is read with scalar loads since its address is
+// uniform across the wavefront). The epilogue is the CSR kernel's, row by row.
+template <typename T, int R>
+__global__ __launch_bounds__(kThreads) void tiled_spmm_kernel(
+    SpmmArgs<T> a, int64_t ntiles, const int64_t* __restrict__ tptr,
+    const int32_t* __restrict__ tcols, const T* __restrict__ tvals)
+{
+    const int lane = threadIdx.x % kWave;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    int64_t t0, t1;
+    row_run(ntiles, t0, t1);
+    const T s = a.xscale ? (T) a.xscale[lane] : T(1);
+    const T tt = (T) a.t;
+    T sub_c = T(0), sub_s = T(0);
+    if (a.Sub)
+    {
+        sub_c = (T) a.sub_coef[lane];
+        sub_s = (T) a.sub_scale[lane];
+    }
+    const T* __restrict__ X = a.X;
+    double red = 0.0;
+    for (int64_t t = t0 + wave; t < t1; t += kWavesPerWG)
+    {
+        T acc[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) acc[r] = T(0);
+        const int64_t beg = tptr[t];
+        const int64_t end = tptr[t + 1];
+        for (int64_t base = beg; base < end; base += kWave)
+        {
+            const int64_t rem = end - base;
+            const int cnt = rem < kWave ? (int) rem : kWave;
+            const int cl = lane < cnt ? tcols[base + lane] : 0;
+            const T* __restrict__ vb = tvals + base * R;
+            int k = 0;
+            for (; k + 4 <= cnt; k += 4)
+            {
+                const int c0 = readlane(cl, k);
+                const int c1 = readlane(cl, k + 1);
+                const int c2 = readlane(cl, k + 2);
+                const int c3 = readlane(cl, k + 3);
+                const T x0 = s * X[(int64_t) c0 * kBlock + lane];
+                const T x1 = s * X[(int64_t) c1 * kBlock + lane];
+                const T x2 = s * X[(int64_t) c2 * kBlock + lane];
+                const T x3 = s * X[(int64_t) c3 * kBlock + lane];
+                const T* v = vb + (int64_t) k * R;
+#pragma unroll
+                for (int r = 0; r < R; ++r) acc[r] = fma(v[r], x0, acc[r]);
+#pragma unroll
+                for (int r = 0; r < R; ++r) acc[r] = fma(v[R + r], x1, acc[r]);
+#pragma unroll
+                for (int r = 0; r < R; ++r) acc[r] = fma(v[2 * R + r], x2, acc[r]);
+#pragma unroll
+                for (int r = 0; r < R; ++r) acc[r] = fma(v[3 * R + r], x3, acc[r]);
+            }
+            for (; k < cnt; ++k)
+            {
+                const int c = readlane(cl, k);
+                const T x = s * X[(int64_t) c * kBlock + lane];
+                const T* v = vb + (int64_t) k * R;
+#pragma unroll
+                for (int r = 0; r < R; ++r) acc[r] = fma(v[r], x, acc[r]);
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+        {
+            const int64_t row = t * R + r;
+            if (row >= a.nrows) break;
+            const int64_t o = row * kBlock + lane;
+            T y = acc[r];
+            T v = T(0);
+            if (a.Vout || a.red_mode == 1 || (a.shift_mode == 1 && row < a.shift_len))
+                v = s * X[o];
+            if (a.Vout) a.Vout[o] = v;
+            if (a.shift_mode == 1 && row < a.shift_len) y = y + tt * v;
+            if (a.Sub) y = y - sub_c * (sub_s * a.Sub[o]);
+            a.Y[o] = y;
+            if (a.red_mode == 1) red += (double) v * (double) y;
+            else if (a.red_mode == 2) red += (double) y * (double) y;
+        }
+    }
+    if (a.red_mode == 0) return;
+    wg_partial(red, a.partial);
+    double total;
+    if (!last_block_total(a.partial, a.counter, total)) return;
+    if (wave != 0) return;
+    if (a.red_mode == 1 || !a.finish_sqrt)
+    {
+        a.red_out[lane] = total;
+        return;
+    }
+    finish_norm(total, lane, a.j, a.tol_sqrt_n, a.red_out, a.scale_out, a.sizes);
+}
+
 // Lanczos three-term update r = (w - alpha v_j) - beta_{j-1} v_{j-1}, in
 // place over w, with the sum of squares of r (the order of the two
 // subtractions follows c_lanczos_tridiagonalization.cpp:196-209).
@@ -314,7 +412,7 @@ template <typename T>
 __global__ __launch_bounds__(kThreads) void lanczos_update_kernel(UpdateArgs<T> a)
 {
     const int lane = threadIdx.x % kWave;
-    const int wave = threadIdx.x / kWave;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     int64_t r0, r1;
     row_run(a.nrows, r0, r1);
     const double al = a.alpha[lane];
@@ -365,7 +463,7 @@ template <typename T>
 __global__ __launch_bounds__(kThreads) void gs_kernel(GsArgs<T> a)
 {
     const int lane = threadIdx.x % kWave;
-    const int wave = threadIdx.x / kWave;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     int64_t r0, r1;
     row_run(a.nrows, r0, r1);
     const int64_t stride = a.nrows * kBlock;
@@ -527,13 +625,39 @@ SpmmArgs<T> spmm_defaults()
     return s;
 }
 
+// Tiled form of the operand matrix for this launch, or nullptr (CSR path):
+// only for A and A + tI (the general A + tB keeps the two CSR streams).
 template <typename T>
-int launch_spmm(slq_op* op, const SpmmArgs<T>& s)
+const DevTiles* tiles_for(slq_op* op, bool trans)
+{
+    if (op->kind == OP_AFFINE_B) return nullptr;
+    const DevTiles& t = trans ? op->A->trans_tiles : op->A->tiles;
+    if (t.R == 0) return nullptr;
+    if (sizeof(T) == 4 && op->A->csr.dtype == SLQ_F64 && !t.vals_f32) return nullptr;
+    return &t;
+}
+
+template <typename T>
+int launch_spmm(slq_op* op, const SpmmArgs<T>& s, const DevTiles* tl = nullptr)
 {
     hipEvent_t ev = nullptr;
     prof_begin(op, &ev);
-    hipLaunchKernelGGL(spmm_kernel<T>, dim3(grid_for(s.nrows)), dim3(kThreads),
-                       0, op->stream, s);
+    if (tl)
+    {
+        const T* tv = (sizeof(T) == 4 && op->A->csr.dtype == SLQ_F64)
+                          ? (const T*) tl->vals_f32 : (const T*) tl->vals;
+        if (tl->R == 8)
+            hipLaunchKernelGGL((tiled_spmm_kernel<T, 8>), dim3(grid_for(tl->ntiles)),
+                               dim3(kThreads), 0, op->stream, s, tl->ntiles, tl->ptr,
+                               tl->cols, tv);
+        else
+            hipLaunchKernelGGL((tiled_spmm_kernel<T, 16>), dim3(grid_for(tl->ntiles)),
+                               dim3(kThreads), 0, op->stream, s, tl->ntiles, tl->ptr,
+                               tl->cols, tv);
+    }
+    else
+        hipLaunchKernelGGL(spmm_kernel<T>, dim3(grid_for(s.nrows)), dim3(kThreads),
+                           0, op->stream, s);
     SLQ_CHECK(hipGetLastError());
     prof_end(op, ev, 0);
     return SLQ_OK;
@@ -700,7 +824,7 @@ int lanczos(const RecurrenceArgs& ra, Bump& bump)
         s.partial = partial;
         s.counter = counter;
         s.red_out = ra.alpha + (size_t) j * kBlock;
-        SLQ_TRY(launch_spmm<T>(op, s));
+        SLQ_TRY(launch_spmm<T>(op, s, tiles_for<T>(op, false)));
 
         UpdateArgs<T> u;
         std::memset(&u, 0, sizeof(u));
@@ -825,7 +949,7 @@ int golub_kahn(const RecurrenceArgs& ra, Bump& bump)
         s.sizes = usizes;            // a small alpha never ends the recurrence
         s.j = j;
         s.tol_sqrt_n = -1.0;
-        SLQ_TRY(launch_spmm<T>(op, s));
+        SLQ_TRY(launch_spmm<T>(op, s, tiles_for<T>(op, false)));
         if (reorth)
         {
             const int num_ortho = j < buffer ? j : buffer - 1;
@@ -859,7 +983,7 @@ int golub_kahn(const RecurrenceArgs& ra, Bump& bump)
         t.sizes = ra.sizes;
         t.j = j;
         t.tol_sqrt_n = tol_sqrt_n;
-        SLQ_TRY(launch_spmm<T>(op, t));
+        SLQ_TRY(launch_spmm<T>(op, t, tiles_for<T>(op, true)));
         if (reorth)
         {
             const int num_ortho = j < buffer ? j : buffer - 1;
@@ -944,7 +1068,7 @@ int run_spmm(slq_op* op, double t, int transpose, const void* X, void* Y,
         s.t = t;
         s.X = (const float*) X;
         s.Y = (float*) Y;
-        return launch_spmm<float>(op, s);
+        return launch_spmm<float>(op, s, tiles_for<float>(op, transpose != 0));
     }
     SpmmArgs<double> s = spmm_defaults<double>();
     fill_matrix<double>(s, A, Bm, transpose != 0, false);
@@ -953,12 +1077,26 @@ int run_spmm(slq_op* op, double t, int transpose, const void* X, void* Y,
     s.t = t;
     s.X = (const double*) X;
     s.Y = (double*) Y;
-    return launch_spmm<double>(op, s);
+    return launch_spmm<double>(op, s, tiles_for<double>(op, transpose != 0));
+}
+
+int tiles_f32(hipStream_t s, DevTiles& t, int dtype)
+{
+    if (dtype != SLQ_F64 || t.R == 0 || t.vals_f32) return SLQ_OK;
+    const int64_t n = t.nslabs * t.R;
+    SLQ_CHECK(hipMalloc(&t.vals_f32, sizeof(float) * n));
+    const int64_t g = (n + 255) / 256;
+    hipLaunchKernelGGL((convert_kernel<double, float>), dim3((unsigned) g), dim3(256), 0,
+                       s, (const double*) t.vals, t.vals_f32, n);
+    SLQ_CHECK(hipGetLastError());
+    return SLQ_OK;
 }
 
 int ensure_f32(hipStream_t s, MatrixStore& M, bool need_trans)
 {
     if (M.csr.dtype != SLQ_F64) return SLQ_OK;
+    SLQ_TRY(tiles_f32(s, M.tiles, M.csr.dtype));
+    if (need_trans) SLQ_TRY(tiles_f32(s, M.trans_tiles, M.csr.dtype));
     if (!M.data_f32)
     {
         SLQ_CHECK(hipMalloc(&M.data_f32, sizeof(float) * (M.csr.nnz > 0 ? M.csr.nnz : 1)));

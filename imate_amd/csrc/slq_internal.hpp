@@ -59,17 +59,44 @@ struct DevCSR {
     void* data = nullptr;
 };
 
+// Row-tiled "column slab" form of a CSR matrix for the SpMM. Rows are grouped
+// in tiles of R consecutive rows; tile t lists the sorted union of the column
+// indices of its rows (cols[ptr[t] .. ptr[t+1])) and, for each such column u,
+// the R values a[t*R + r][u] (zero where absent) stored contiguously
+// (vals[u * R + r]). One gathered operand row X[u] then feeds R FMAs instead
+// of one, which is what keeps the gather traffic of a 64-probe block out of
+// the cache bandwidth limit on spatially ordered matrices.
+struct DevTiles {
+    int R = 0;                 // 0 = not built / not profitable
+    int64_t ntiles = 0;
+    int64_t nslabs = 0;        // total unique (tile, column) pairs
+    int64_t* ptr = nullptr;    // [ntiles + 1]
+    int32_t* cols = nullptr;   // [nslabs]
+    void* vals = nullptr;      // [nslabs * R], dtype of the matrix
+    float* vals_f32 = nullptr; // FP32 copy (mixed precision)
+};
+
 // Device copy of a matrix plus derived forms built on demand: the transpose
-// (CSR of A^T, used as a gather for A^T u) and an FP32 copy of the values of
-// an FP64 matrix (for the mixed-precision form).
+// (CSR of A^T, used as a gather for A^T u), an FP32 copy of the values of an
+// FP64 matrix (for the mixed-precision form) and the row-tiled form.
 struct MatrixStore {
     DevCSR csr;
     DevCSR trans;          // valid when has_trans
     bool has_trans = false;
     float* data_f32 = nullptr;        // FP32 values of csr (mixed precision)
     float* trans_data_f32 = nullptr;  // FP32 values of trans
+    DevTiles tiles;                   // of csr
+    DevTiles trans_tiles;             // of trans
+    bool trans_tiles_tried = false;
     ~MatrixStore();
 };
+
+// Host construction of the tiled form (R in {8, 16}); returns R = 0 in
+// `out` when fewer than `min_density` of the slab entries would be non-zero.
+int build_tiles(int64_t nrows, int64_t ncols, const int64_t* indptr,
+                const int32_t* indices, const void* data, int dtype, int R,
+                double min_density, DevTiles& out);
+int tiles_f32(hipStream_t s, DevTiles& t, int dtype);
 
 // Device buffers reused across calls on one handle.
 struct Workspace {
