@@ -1,6 +1,6 @@
-// Block Krylov recurrences on gfx950: CSR SpMM over a 64-probe block with
-// fused Lanczos / Golub-Kahn epilogues, deterministic grid reductions and
-// classical Gram-Schmidt (twice) re-orthogonalisation.
+// Block Krylov recurrences on gfx950: CSR / row-tiled SpMM over a 64-probe
+// block with fused Lanczos / Golub-Kahn epilogues, deterministic grid
+// reductions and classical Gram-Schmidt (twice) re-orthogonalisation.
 //
 // Reference algorithms (one vector at a time there):
 //   Lanczos     imate/_c_trace_estimator/c_lanczos_tridiagonalization.cpp:117-235
@@ -11,21 +11,27 @@
 //               c_csr_affine_matrix_function.cpp (A x + t x, A x + t B x)
 //
 // Layout: every vector block is [rows][64] row-major, lane p of a wavefront
-// owns probe p. A CSR row is processed by one wavefront: the row's column
-// indices and values are read once, coalesced, by the lanes, broadcast with
-// v_readlane, and each non-zero gathers the contiguous 64-probe row of the
-// operand block (512 B in FP64). Rows are handed out in contiguous runs per
-// workgroup, with workgroup ids mapped XCD-major so that neighbouring runs
-// (which share columns of a spatially ordered matrix) land on the same XCD's
-// L2.
+// owns probe p, so a row of a block is one coalesced 512 B (FP64) access and
+// the gather of column c reads the contiguous row c of the operand block.
 //
-// Reductions (alpha, beta, Gram-Schmidt coefficients) are per-probe sums over
-// all rows: each workgroup writes one partial row, and either the last
-// workgroup to finish (atomic ticket) or a follow-up kernel adds the partials
-// in a fixed order, so results are bitwise reproducible run to run.
+// Work distribution ("XCD windows"): workgroup ids are dealt round-robin to
+// the 8 XCDs (id mod 8). XCD x owns the contiguous eighth x of the rows (or
+// row tiles), and its workgroups sweep that range together, one window of
+// (workgroups per XCD) x (wavefronts per workgroup) units at a time. Units
+// processed concurrently on an XCD are therefore neighbours; on a spatially
+// ordered matrix their gathered operand rows overlap and stay in that XCD's
+// 4 MB L2.
+//
+// Reductions (alpha, beta, norms) are per-probe sums over all rows, computed
+// deterministically: each workgroup writes one partial row; the last
+// workgroup of each group of 32 (agent-scope atomic ticket after a fence)
+// adds its group's partials in a fixed order; the last group adds the group
+// partials. The result is bitwise reproducible run to run and the serial
+// tail is two rounds of at most four loads per lane.
 
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -34,6 +40,9 @@
 namespace slq {
 
 namespace {
+
+constexpr int kGroup = 32;                         // workgroups per level-1 group
+constexpr int kMaxGroups = (kMaxGrid + kGroup - 1) / kGroup;
 
 // ---------------------------------------------------------------- helpers
 
@@ -56,17 +65,32 @@ __device__ __forceinline__ int readlane(int v, int k)
     return __builtin_amdgcn_readlane(v, k);
 }
 
-// Contiguous row run [r0, r1) of this workgroup. Workgroup ids are dealt
-// round-robin to the 8 XCDs; the remapping gives XCD x the runs
-// x*G/8 .. (x+1)*G/8 - 1. G is a multiple of 8.
-__device__ __forceinline__ void row_run(int64_t nrows, int64_t& r0, int64_t& r1)
+__device__ __forceinline__ int wave_id()
+{
+    return __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+}
+
+// Units [0, N) swept in XCD windows (see the header). G is a multiple of 8.
+struct Sweep {
+    int64_t first;
+    int64_t end;
+    int64_t stride;
+};
+
+__device__ __forceinline__ Sweep sweep(int64_t N, int wave)
 {
     const int64_t G = gridDim.x;
     const int64_t bid = blockIdx.x;
-    const int64_t run = (bid % kXcds) * (G / kXcds) + bid / kXcds;
-    const int64_t per = (nrows + G - 1) / G;
-    r0 = run * per;
-    r1 = r0 + per < nrows ? r0 + per : nrows;
+    const int64_t x = bid % kXcds;
+    const int64_t local = bid / kXcds;
+    const int64_t per_xcd = G / kXcds;
+    const int64_t span = (N + kXcds - 1) / kXcds;
+    const int64_t b = x * span;
+    Sweep s;
+    s.end = b + span < N ? b + span : N;
+    s.first = b + local * kWavesPerWG + wave;
+    s.stride = per_xcd * kWavesPerWG;
+    return s;
 }
 
 // sum_k a_ik * (s * X[c_k][lane]) over the CSR row [beg, end).
@@ -118,12 +142,24 @@ __device__ __forceinline__ T row_gather(int64_t beg, int64_t end,
     return acc;
 }
 
-// Workgroup sum of one double per lane: partial[blockIdx.x][lane].
-__device__ __forceinline__ void wg_partial(double v, double* __restrict__ partial)
+// ---------------------------------------------------------------- reduction
+
+struct RedBuf {
+    double* partial;            // [G][64]
+    double* gpartial;           // [ceil(G/32)][64]
+    unsigned int* counters;     // [1 + ceil(G/32)], zero between launches
+};
+
+// Per-lane sum of v over the whole grid (see the header). Returns true in
+// exactly one workgroup, whose wave 0 then holds the totals in `total`.
+__device__ __forceinline__ bool grid_total(double v, const RedBuf& rb, double& total)
 {
     __shared__ double red[kWavesPerWG][kWave];
+    __shared__ int flag;
     const int lane = threadIdx.x % kWave;
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const int wave = wave_id();
+    const int G = gridDim.x;
+    const int bid = blockIdx.x;
     red[wave][lane] = v;
     __syncthreads();
     if (wave == 0)
@@ -131,47 +167,51 @@ __device__ __forceinline__ void wg_partial(double v, double* __restrict__ partia
         double s = 0.0;
 #pragma unroll
         for (int w = 0; w < kWavesPerWG; ++w) s += red[w][lane];
-        partial[(int64_t) blockIdx.x * kWave + lane] = s;
+        rb.partial[(int64_t) bid * kWave + lane] = s;
     }
-}
-
-// After wg_partial: returns true in the last workgroup to finish, which then
-// holds the per-lane totals (fixed summation order) in `total` of wave 0.
-__device__ __forceinline__ bool last_block_total(const double* __restrict__ partial,
-                                                 unsigned int* __restrict__ counter,
-                                                 double& total)
-{
-    __shared__ bool is_last;
-    __shared__ double red[kWavesPerWG][kWave];
+    const int group = bid / kGroup;
+    const int g0 = group * kGroup;
+    const int gsz = (G - g0) < kGroup ? (G - g0) : kGroup;
+    const int ngroups = (G + kGroup - 1) / kGroup;
     __threadfence();
     __syncthreads();
     if (threadIdx.x == 0)
-    {
-        const unsigned int ticket = atomicAdd(counter, 1u);
-        is_last = (ticket == gridDim.x - 1);
-    }
+        flag = (atomicAdd(&rb.counters[1 + group], 1u) == (unsigned int) (gsz - 1));
     __syncthreads();
-    if (!is_last) return false;
+    if (!flag) return false;
     __threadfence();
-    const int lane = threadIdx.x % kWave;
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
-    const int G = gridDim.x;
-    double s0 = 0.0, s1 = 0.0;
-    int g = wave;
-    for (; g + kWavesPerWG < G; g += 2 * kWavesPerWG)
+    double s = 0.0;
+    for (int r = wave; r < gsz; r += kWavesPerWG)
+        s += rb.partial[(int64_t) (g0 + r) * kWave + lane];
+    __syncthreads();
+    red[wave][lane] = s;
+    __syncthreads();
+    if (wave == 0)
     {
-        s0 += partial[(int64_t) g * kWave + lane];
-        s1 += partial[(int64_t) (g + kWavesPerWG) * kWave + lane];
+        double t = 0.0;
+#pragma unroll
+        for (int w = 0; w < kWavesPerWG; ++w) t += red[w][lane];
+        rb.gpartial[group * kWave + lane] = t;
+        if (lane == 0) rb.counters[1 + group] = 0u;
     }
-    if (g < G) s0 += partial[(int64_t) g * kWave + lane];
-    red[wave][lane] = s0 + s1;
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0)
+        flag = (atomicAdd(&rb.counters[0], 1u) == (unsigned int) (ngroups - 1));
+    __syncthreads();
+    if (!flag) return false;
+    __threadfence();
+    s = 0.0;
+    for (int r = wave; r < ngroups; r += kWavesPerWG) s += rb.gpartial[r * kWave + lane];
+    __syncthreads();
+    red[wave][lane] = s;
     __syncthreads();
     total = 0.0;
     if (wave == 0)
     {
 #pragma unroll
         for (int w = 0; w < kWavesPerWG; ++w) total += red[w][lane];
-        if (lane == 0) *counter = 0u;
+        if (lane == 0) rb.counters[0] = 0u;
     }
     return true;
 }
@@ -196,11 +236,11 @@ __device__ __forceinline__ void finish_norm(double total, int lane, int j,
     scale_out[lane] = (sz == 0) ? 1.0 / b : 0.0;
 }
 
-// ---------------------------------------------------------------- kernels
+// ---------------------------------------------------------------- SpMM
 
 template <typename T>
 struct SpmmArgs {
-    // A (rows of the output) and optionally B of A + tB
+    // A (rows of the output) and optionally B of A + tB (CSR)
     const int64_t* indptr;
     const int32_t* indices;
     const T* vals;
@@ -211,76 +251,50 @@ struct SpmmArgs {
     int64_t shift_len;      // rows that get + t x[i] (identity shift)
     int shift_mode;         // 0 none, 1 + t x (B = I), 2 + t B x
     double t;
-    // gathered operand and its per-probe scale
-    const T* X;
-    const double* xscale;   // nullptr -> 1
-    // optional: Vout[i] = xscale * X[i] (square operators)
-    T* Vout;
-    // optional epilogue subtraction: y -= sub_coef[p] * (sub_scale[p] * Sub[i])
-    const T* Sub;
+    const double* xscale;   // per-probe scale of the gathered operand (nullptr -> 1)
+    // epilogue subtraction: y -= sub_coef[p] * (sub_scale[p] * Sub[i])
     const double* sub_coef;
     const double* sub_scale;
-    T* Y;
     // reduction: 0 none, 1 sum v*y -> red_out (alpha), 2 sum y*y -> norm
     int red_mode;
-    double* partial;
-    unsigned int* counter;
-    double* red_out;        // alpha[j] or norm output
-    // norm finishing (red_mode 2)
-    int finish_sqrt;        // 1: red_out = sqrt, scale_out = 1/sqrt (+ breakdown)
+    RedBuf rb;
+    double* red_out;
+    int finish_sqrt;        // red_mode 2: red_out = sqrt, scale_out = 1/sqrt
     double* scale_out;
     int32_t* sizes;
     int j;
     double tol_sqrt_n;
 };
 
+// Epilogue of one output row: optional v = s X[row] (written to Vout), the
+// affine shift, the subtraction, the store and the reduction term.
 template <typename T>
-__global__ __launch_bounds__(kThreads) void spmm_kernel(SpmmArgs<T> a)
+__device__ __forceinline__ void spmm_epilogue(const SpmmArgs<T>& a, int64_t row, int lane,
+                                              T y, T s, T tt, T sub_c, T sub_s,
+                                              const T* __restrict__ X,
+                                              T* __restrict__ Y, T* __restrict__ Vout,
+                                              const T* __restrict__ Sub, double& red)
 {
-    const int lane = threadIdx.x % kWave;
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
-    int64_t r0, r1;
-    row_run(a.nrows, r0, r1);
-    const T s = a.xscale ? (T) a.xscale[lane] : T(1);
-    const T tt = (T) a.t;
-    T sub_c = T(0), sub_s = T(0);
-    if (a.Sub)
-    {
-        sub_c = (T) a.sub_coef[lane];
-        sub_s = (T) a.sub_scale[lane];
-    }
-    double red = 0.0;
-    for (int64_t row = r0 + wave; row < r1; row += kWavesPerWG)
-    {
-        const int64_t beg = a.indptr[row];
-        const int64_t end = a.indptr[row + 1];
-        T y = row_gather<T>(beg, end, a.indices, a.vals, a.X, s, lane);
-        const int64_t o = row * kBlock + lane;
-        T v = T(0);
-        if (a.Vout || a.shift_mode == 1 || a.red_mode == 1)
-            v = (row < a.shift_len || a.Vout || a.red_mode == 1) ? s * a.X[o] : T(0);
-        if (a.Vout) a.Vout[o] = v;
-        if (a.shift_mode == 1)
-        {
-            // y - (-t) x: imate adds t x through subtract_scaled_vector(-t)
-            if (row < a.shift_len) y = y + tt * v;
-        }
-        else if (a.shift_mode == 2)
-        {
-            const T yb = row_gather<T>(a.b_indptr[row], a.b_indptr[row + 1],
-                                       a.b_indices, a.b_vals, a.X, s, lane);
-            y = y + tt * yb;
-        }
-        if (a.Sub) y = y - sub_c * (sub_s * a.Sub[o]);
-        a.Y[o] = y;
-        if (a.red_mode == 1) red += (double) v * (double) y;
-        else if (a.red_mode == 2) red += (double) y * (double) y;
-    }
+    const int64_t o = row * kBlock + lane;
+    const bool shift_row = a.shift_mode == 1 && row < a.shift_len;
+    T v = T(0);
+    if (Vout || a.red_mode == 1 || shift_row) v = s * X[o];
+    if (Vout) Vout[o] = v;
+    // imate adds t x through subtract_scaled_vector(-t): y - (-t) x
+    if (shift_row) y = y + tt * v;
+    if (Sub) y = y - sub_c * (sub_s * Sub[o]);
+    Y[o] = y;
+    if (a.red_mode == 1) red += (double) v * (double) y;
+    else if (a.red_mode == 2) red += (double) y * (double) y;
+}
+
+template <typename T>
+__device__ __forceinline__ void spmm_finish(const SpmmArgs<T>& a, double red, int lane)
+{
     if (a.red_mode == 0) return;
-    wg_partial(red, a.partial);
     double total;
-    if (!last_block_total(a.partial, a.counter, total)) return;
-    if (wave != 0) return;
+    if (!grid_total(red, a.rb, total)) return;
+    if (wave_id() != 0) return;
     if (a.red_mode == 1 || !a.finish_sqrt)
     {
         a.red_out[lane] = total;
@@ -289,31 +303,52 @@ __global__ __launch_bounds__(kThreads) void spmm_kernel(SpmmArgs<T> a)
     finish_norm(total, lane, a.j, a.tol_sqrt_n, a.red_out, a.scale_out, a.sizes);
 }
 
+// CSR form: one wavefront per row.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void spmm_kernel(SpmmArgs<T> a,
+                                                        const T* __restrict__ X,
+                                                        T* __restrict__ Y,
+                                                        T* __restrict__ Vout,
+                                                        const T* __restrict__ Sub)
+{
+    const int lane = threadIdx.x % kWave;
+    const int wave = wave_id();
+    const Sweep sw = sweep(a.nrows, wave);
+    const T s = a.xscale ? (T) a.xscale[lane] : T(1);
+    const T tt = (T) a.t;
+    const T sub_c = Sub ? (T) a.sub_coef[lane] : T(0);
+    const T sub_s = Sub ? (T) a.sub_scale[lane] : T(0);
+    double red = 0.0;
+    for (int64_t row = sw.first; row < sw.end; row += sw.stride)
+    {
+        T y = row_gather<T>(a.indptr[row], a.indptr[row + 1], a.indices, a.vals, X, s, lane);
+        if (a.shift_mode == 2)
+            y = y + tt * row_gather<T>(a.b_indptr[row], a.b_indptr[row + 1], a.b_indices,
+                                       a.b_vals, X, s, lane);
+        spmm_epilogue<T>(a, row, lane, y, s, tt, sub_c, sub_s, X, Y, Vout, Sub, red);
+    }
+    spmm_finish<T>(a, red, lane);
+}
 
-// Row-tiled variant of spmm_kernel: one wavefront per tile of R rows, R
-// accumulators per lane; every gathered operand row X[u] feeds the R values
-// of column u (the slab is read with scalar loads since its address is
-// uniform across the wavefront). The epilogue is the CSR kernel's, row by row.
+// Row-tiled form: one wavefront per tile of R rows, R accumulators per lane.
+// Each gathered operand row X[u] feeds the R values of column u, read with
+// scalar loads (the slab address is wavefront-uniform, and a scalar operand
+// feeds the FMA directly).
 template <typename T, int R>
 __global__ __launch_bounds__(kThreads) void tiled_spmm_kernel(
-    SpmmArgs<T> a, int64_t ntiles, const int64_t* __restrict__ tptr,
+    SpmmArgs<T> a, const T* __restrict__ X, T* __restrict__ Y, T* __restrict__ Vout,
+    const T* __restrict__ Sub, int64_t ntiles, const int64_t* __restrict__ tptr,
     const int32_t* __restrict__ tcols, const T* __restrict__ tvals)
 {
     const int lane = threadIdx.x % kWave;
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
-    int64_t t0, t1;
-    row_run(ntiles, t0, t1);
+    const int wave = wave_id();
+    const Sweep sw = sweep(ntiles, wave);
     const T s = a.xscale ? (T) a.xscale[lane] : T(1);
     const T tt = (T) a.t;
-    T sub_c = T(0), sub_s = T(0);
-    if (a.Sub)
-    {
-        sub_c = (T) a.sub_coef[lane];
-        sub_s = (T) a.sub_scale[lane];
-    }
-    const T* __restrict__ X = a.X;
+    const T sub_c = Sub ? (T) a.sub_coef[lane] : T(0);
+    const T sub_s = Sub ? (T) a.sub_scale[lane] : T(0);
     double red = 0.0;
-    for (int64_t t = t0 + wave; t < t1; t += kWavesPerWG)
+    for (int64_t t = sw.first; t < sw.end; t += sw.stride)
     {
         T acc[R];
 #pragma unroll
@@ -360,47 +395,28 @@ __global__ __launch_bounds__(kThreads) void tiled_spmm_kernel(
         for (int r = 0; r < R; ++r)
         {
             const int64_t row = t * R + r;
-            if (row >= a.nrows) break;
-            const int64_t o = row * kBlock + lane;
-            T y = acc[r];
-            T v = T(0);
-            if (a.Vout || a.red_mode == 1 || (a.shift_mode == 1 && row < a.shift_len))
-                v = s * X[o];
-            if (a.Vout) a.Vout[o] = v;
-            if (a.shift_mode == 1 && row < a.shift_len) y = y + tt * v;
-            if (a.Sub) y = y - sub_c * (sub_s * a.Sub[o]);
-            a.Y[o] = y;
-            if (a.red_mode == 1) red += (double) v * (double) y;
-            else if (a.red_mode == 2) red += (double) y * (double) y;
+            if (row < a.nrows)
+                spmm_epilogue<T>(a, row, lane, acc[r], s, tt, sub_c, sub_s, X, Y, Vout, Sub,
+                                 red);
         }
     }
-    if (a.red_mode == 0) return;
-    wg_partial(red, a.partial);
-    double total;
-    if (!last_block_total(a.partial, a.counter, total)) return;
-    if (wave != 0) return;
-    if (a.red_mode == 1 || !a.finish_sqrt)
-    {
-        a.red_out[lane] = total;
-        return;
-    }
-    finish_norm(total, lane, a.j, a.tol_sqrt_n, a.red_out, a.scale_out, a.sizes);
+    spmm_finish<T>(a, red, lane);
 }
+
+// ---------------------------------------------------------------- update
 
 // Lanczos three-term update r = (w - alpha v_j) - beta_{j-1} v_{j-1}, in
 // place over w, with the sum of squares of r (the order of the two
-// subtractions follows c_lanczos_tridiagonalization.cpp:196-209).
-template <typename T>
+// subtractions follows c_lanczos_tridiagonalization.cpp:196-209). Each lane
+// owns two consecutive probes of a row (one 16 B access per array for FP64),
+// so a wavefront covers two rows per instruction; two such row pairs per
+// iteration keep six loads in flight.
 struct UpdateArgs {
     int64_t nrows;
-    T* R;                  // in: w, out: r
-    const T* Vj;
-    const T* Vjm1;         // nullptr at j = 0
     const double* alpha;   // [64]
-    const double* beta;    // [64] beta_{j-1}
+    const double* beta;    // [64] beta_{j-1} (nullptr at j = 0)
     int do_norm;           // 1: reduce |r|^2 and finish beta_j
-    double* partial;
-    unsigned int* counter;
+    RedBuf rb;
     double* beta_out;
     double* scale_out;
     int32_t* sizes;
@@ -409,28 +425,100 @@ struct UpdateArgs {
 };
 
 template <typename T>
-__global__ __launch_bounds__(kThreads) void lanczos_update_kernel(UpdateArgs<T> a)
+struct Pair;
+template <>
+struct Pair<double> {
+    using V = double2;
+};
+template <>
+struct Pair<float> {
+    using V = float2;
+};
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void lanczos_update_kernel(
+    UpdateArgs a, T* __restrict__ R, const T* __restrict__ Vj, const T* __restrict__ Vjm1)
 {
+    using V2 = typename Pair<T>::V;
     const int lane = threadIdx.x % kWave;
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
-    int64_t r0, r1;
-    row_run(a.nrows, r0, r1);
-    const double al = a.alpha[lane];
-    const double be = a.Vjm1 ? a.beta[lane] : 0.0;
-    double red = 0.0;
-    for (int64_t row = r0 + wave; row < r1; row += kWavesPerWG)
+    const int wave = wave_id();
+    // unit = pair of rows; lane l covers row 2u + (l >= 32), probes 2(l%32), +1
+    const int64_t npairs = (a.nrows + 1) / 2;
+    const Sweep sw = sweep(npairs, wave);
+    const int half = lane >> 5;
+    const int p0 = 2 * (lane & 31);
+    const double al0 = a.alpha[p0], al1 = a.alpha[p0 + 1];
+    const double be0 = Vjm1 ? a.beta[p0] : 0.0, be1 = Vjm1 ? a.beta[p0 + 1] : 0.0;
+    double red0 = 0.0, red1 = 0.0;
+    V2* __restrict__ R2 = reinterpret_cast<V2*>(R);
+    const V2* __restrict__ Vj2 = reinterpret_cast<const V2*>(Vj);
+    const V2* __restrict__ Vm2 = reinterpret_cast<const V2*>(Vjm1);
+    for (int64_t u = sw.first; u < sw.end; u += 2 * sw.stride)
     {
-        const int64_t o = row * kBlock + lane;
-        double r = (double) a.R[o] - al * (double) a.Vj[o];
-        if (a.Vjm1) r = r - be * (double) a.Vjm1[o];
-        const T rt = (T) r;
-        a.R[o] = rt;
-        red += (double) rt * (double) rt;
+        const int64_t u2 = u + sw.stride;
+        const int64_t ra = 2 * u + half;
+        const int64_t rb = 2 * u2 + half;
+        const bool oka = ra < a.nrows;
+        const bool okb = u2 < sw.end && rb < a.nrows;
+        const int64_t oa = (ra * kBlock + p0) / 2;
+        const int64_t ob = (rb * kBlock + p0) / 2;
+        V2 wa, va, ua, wb, vb, ub;
+        if (oka)
+        {
+            wa = R2[oa];
+            va = Vj2[oa];
+            if (Vjm1) ua = Vm2[oa];
+        }
+        if (okb)
+        {
+            wb = R2[ob];
+            vb = Vj2[ob];
+            if (Vjm1) ub = Vm2[ob];
+        }
+        if (oka)
+        {
+            double r0 = (double) wa.x - al0 * (double) va.x;
+            double r1 = (double) wa.y - al1 * (double) va.y;
+            if (Vjm1)
+            {
+                r0 = r0 - be0 * (double) ua.x;
+                r1 = r1 - be1 * (double) ua.y;
+            }
+            V2 o;
+            o.x = (T) r0;
+            o.y = (T) r1;
+            R2[oa] = o;
+            red0 += (double) o.x * (double) o.x;
+            red1 += (double) o.y * (double) o.y;
+        }
+        if (okb)
+        {
+            double r0 = (double) wb.x - al0 * (double) vb.x;
+            double r1 = (double) wb.y - al1 * (double) vb.y;
+            if (Vjm1)
+            {
+                r0 = r0 - be0 * (double) ub.x;
+                r1 = r1 - be1 * (double) ub.y;
+            }
+            V2 o;
+            o.x = (T) r0;
+            o.y = (T) r1;
+            R2[ob] = o;
+            red0 += (double) o.x * (double) o.x;
+            red1 += (double) o.y * (double) o.y;
+        }
     }
     if (!a.do_norm) return;
-    wg_partial(red, a.partial);
+    // lanes l and l + 32 hold the same probes: move to lane-per-probe form
+    __shared__ double ex[kWavesPerWG][2][kWave];
+    ex[wave][0][lane] = red0;
+    ex[wave][1][lane] = red1;
+    __syncthreads();
+    // probe q = lane: pair slot (q % 2), owner lanes q/2 and q/2 + 32
+    const int q = lane;
+    const double mine = ex[wave][q & 1][q >> 1] + ex[wave][q & 1][(q >> 1) + 32];
     double total;
-    if (!last_block_total(a.partial, a.counter, total)) return;
+    if (!grid_total(mine, a.rb, total)) return;
     if (wave != 0) return;
     finish_norm(total, lane, a.j, a.tol_sqrt_n, a.beta_out, a.scale_out, a.sizes);
 }
@@ -440,19 +528,17 @@ __global__ __launch_bounds__(kThreads) void lanczos_update_kernel(UpdateArgs<T> 
 constexpr int kGsGroup = 8;     // basis vectors per register group
 
 // Optionally r -= sum_k h_k basis_k (h from a previous pass), then either the
-// per-probe dots h'_k = <basis_k, r> (partials [G][K][64]) or |r|^2.
-template <typename T>
+// per-probe dots h'_k = <basis_k, r> (partials [G][K][64], mode 0) or the
+// norm |r| with the breakdown test (mode 1).
 struct GsArgs {
     int64_t nrows;
-    T* R;
-    const T* basis;          // [nslots][nrows][64]
     int last;                // slot of the newest vector; vector k is slot last - k
     int K;
     const double* h_sub;     // [K][64] or nullptr (no subtraction)
-    int mode;                // 0: dots into partial, 1: norm finish
-    double* partial;         // [G][K][64] (mode 0) or [G][64] (mode 1)
-    unsigned int* counter;
-    double* beta_out;
+    int mode;
+    double* partial_k;       // [G][K][64] (mode 0)
+    RedBuf rb;               // mode 1
+    double* norm_out;
     double* scale_out;
     int32_t* sizes;
     int j;
@@ -460,23 +546,22 @@ struct GsArgs {
 };
 
 template <typename T>
-__global__ __launch_bounds__(kThreads) void gs_kernel(GsArgs<T> a)
+__global__ __launch_bounds__(kThreads) void gs_kernel(GsArgs a, T* __restrict__ R,
+                                                      const T* __restrict__ basis)
 {
     const int lane = threadIdx.x % kWave;
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
-    int64_t r0, r1;
-    row_run(a.nrows, r0, r1);
+    const int wave = wave_id();
+    const Sweep sw = sweep(a.nrows, wave);
     const int64_t stride = a.nrows * kBlock;
     if (a.h_sub)
     {
-        for (int64_t row = r0 + wave; row < r1; row += kWavesPerWG)
+        for (int64_t row = sw.first; row < sw.end; row += sw.stride)
         {
             const int64_t o = row * kBlock + lane;
-            double r = (double) a.R[o];
+            double r = (double) R[o];
             for (int k = 0; k < a.K; ++k)
-                r -= a.h_sub[k * kWave + lane] *
-                     (double) a.basis[(int64_t) (a.last - k) * stride + o];
-            a.R[o] = (T) r;
+                r -= a.h_sub[k * kWave + lane] * (double) basis[(int64_t) (a.last - k) * stride + o];
+            R[o] = (T) r;
         }
     }
     if (a.mode == 0)
@@ -488,14 +573,14 @@ __global__ __launch_bounds__(kThreads) void gs_kernel(GsArgs<T> a)
             double acc[kGsGroup];
 #pragma unroll
             for (int g = 0; g < kGsGroup; ++g) acc[g] = 0.0;
-            for (int64_t row = r0 + wave; row < r1; row += kWavesPerWG)
+            for (int64_t row = sw.first; row < sw.end; row += sw.stride)
             {
                 const int64_t o = row * kBlock + lane;
-                const double r = (double) a.R[o];
+                const double r = (double) R[o];
 #pragma unroll
                 for (int g = 0; g < kGsGroup; ++g)
                     if (g < kn)
-                        acc[g] += r * (double) a.basis[(int64_t) (a.last - k0 - g) * stride + o];
+                        acc[g] += r * (double) basis[(int64_t) (a.last - k0 - g) * stride + o];
             }
 #pragma unroll
             for (int g = 0; g < kGsGroup; ++g) red[wave][g][lane] = acc[g];
@@ -505,39 +590,50 @@ __global__ __launch_bounds__(kThreads) void gs_kernel(GsArgs<T> a)
                 double s = 0.0;
 #pragma unroll
                 for (int w = 0; w < kWavesPerWG; ++w) s += red[w][wave][lane];
-                a.partial[((int64_t) blockIdx.x * a.K + k0 + wave) * kWave + lane] = s;
+                a.partial_k[((int64_t) blockIdx.x * a.K + k0 + wave) * kWave + lane] = s;
             }
             __syncthreads();
         }
         return;
     }
     double red = 0.0;
-    for (int64_t row = r0 + wave; row < r1; row += kWavesPerWG)
+    for (int64_t row = sw.first; row < sw.end; row += sw.stride)
     {
-        const double r = (double) a.R[row * kBlock + lane];
+        const double r = (double) R[row * kBlock + lane];
         red += r * r;
     }
-    wg_partial(red, a.partial);
     double total;
-    if (!last_block_total(a.partial, a.counter, total)) return;
+    if (!grid_total(red, a.rb, total)) return;
     if (wave != 0) return;
-    finish_norm(total, lane, a.j, a.tol_sqrt_n, a.beta_out, a.scale_out, a.sizes);
+    finish_norm(total, lane, a.j, a.tol_sqrt_n, a.norm_out, a.scale_out, a.sizes);
 }
 
-// out[k][p] = sum_g partial[g][k][p], fixed order; one workgroup per k.
-__global__ __launch_bounds__(256) void reduce_partials_kernel(
+// out[k][p] = sum_g partial[g][k][p] in a fixed order; one workgroup per k,
+// 16 row groups of 64 lanes each then combined in order.
+__global__ __launch_bounds__(1024) void reduce_partials_kernel(
     const double* __restrict__ partial, int G, int K, double* __restrict__ out)
 {
-    __shared__ double red[4][kWave];
+    __shared__ double red[16][kWave];
     const int lane = threadIdx.x % kWave;
     const int grp = threadIdx.x / kWave;
     const int k = blockIdx.x;
-    double s = 0.0;
-    for (int g = grp; g < G; g += 4)
-        s += partial[((int64_t) g * K + k) * kWave + lane];
-    red[grp][lane] = s;
+    double s0 = 0.0, s1 = 0.0;
+    int g = grp;
+    for (; g + 16 < G; g += 32)
+    {
+        s0 += partial[((int64_t) g * K + k) * kWave + lane];
+        s1 += partial[((int64_t) (g + 16) * K + k) * kWave + lane];
+    }
+    if (g < G) s0 += partial[((int64_t) g * K + k) * kWave + lane];
+    red[grp][lane] = s0 + s1;
     __syncthreads();
-    if (grp == 0) out[k * kWave + lane] = red[0][lane] + red[1][lane] + red[2][lane] + red[3][lane];
+    if (grp == 0)
+    {
+        double t = 0.0;
+#pragma unroll
+        for (int i = 0; i < 16; ++i) t += red[i][lane];
+        out[k * kWave + lane] = t;
+    }
 }
 
 // Block set-up: scale[p] = s0 for probes p < count (0 otherwise), sizes = 0.
@@ -565,15 +661,64 @@ __global__ void convert_kernel(const S* __restrict__ in, D* __restrict__ out, in
     if (i < n) out[i] = (D) in[i];
 }
 
+// X[i][p] *= scale[p]; then scale[p] = 1 (second launch, after every reader).
+template <typename T>
+__global__ __launch_bounds__(256) void scale_rows_kernel(T* __restrict__ X, int64_t n,
+                                                         const double* __restrict__ scale)
+{
+    const int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n * kBlock) X[i] = (T) ((double) X[i] * scale[i % kBlock]);
+}
+
+__global__ void set_ones_kernel(double* s)
+{
+    if (threadIdx.x < kBlock) s[threadIdx.x] = 1.0;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void copy_block_kernel(const T* __restrict__ in,
+                                                         T* __restrict__ out, int64_t n)
+{
+    const int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = in[i];
+}
+
 // ---------------------------------------------------------------- host side
 
-int grid_for(int64_t rows)
+// Grid of a sweep over `units`: at most kMaxGrid workgroups (4 per CU at 8
+// wavefronts each, i.e. one resident wave of workgroups), at least one
+// wavefront-unit each, a multiple of 8 (one share per XCD).
+int max_grid()
 {
-    int64_t g = (rows + 63) / 64;            // aim at >= 64 rows per workgroup
-    if (g > kMaxGrid) g = kMaxGrid;
+    static int g = [] {
+        const char* e = std::getenv("SLQ_GRID");
+        int v = e ? std::atoi(e) : kGrid;
+        if (v < kXcds || v > kMaxGrid) v = kMaxGrid;
+        return (v / kXcds) * kXcds;
+    }();
+    return g;
+}
+
+int grid_for(int64_t units)
+{
+    int64_t g = (units + kWavesPerWG - 1) / kWavesPerWG;
+    if (g > max_grid()) g = max_grid();
     g = ((g + kXcds - 1) / kXcds) * kXcds;
     if (g < kXcds) g = kXcds;
     return (int) g;
+}
+
+// Grid of the streaming kernels (update, Gram-Schmidt): 256 workgroups, one
+// per CU, measured to reach the copy-kernel bandwidth where more workgroups
+// lose ~45 % to contention.
+int stream_grid(int64_t units)
+{
+    int g = grid_for(units);
+    const char* e = std::getenv("SLQ_STREAM_GRID");
+    int cap = e ? std::atoi(e) : 256;
+    if (cap < kXcds) cap = kXcds;
+    cap = (cap / kXcds) * kXcds;
+    return g > cap ? cap : g;
 }
 
 struct Bump {
@@ -590,6 +735,29 @@ struct Bump {
         return r;
     }
 };
+
+RedBuf take_redbuf(Bump& bump)
+{
+    RedBuf rb;
+    rb.partial = bump.take<double>((size_t) kMaxGrid * kBlock);
+    rb.gpartial = bump.take<double>((size_t) kMaxGroups * kBlock);
+    rb.counters = bump.take<unsigned int>(1 + kMaxGroups);
+    return rb;
+}
+
+bool redbuf_ok(const RedBuf& rb) { return rb.partial && rb.gpartial && rb.counters; }
+
+void zero_counters(hipStream_t s, const RedBuf& rb)
+{
+    hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, s, rb.counters, 1 + kMaxGroups);
+}
+
+size_t redbuf_bytes()
+{
+    auto r = [](size_t b) { return (b + 255) & ~(size_t) 255; };
+    return r(sizeof(double) * kMaxGrid * kBlock) + r(sizeof(double) * kMaxGroups * kBlock) +
+           r(sizeof(unsigned int) * (1 + kMaxGroups));
+}
 
 template <typename T>
 const T* values_of(MatrixStore& M, bool trans, bool want_f32)
@@ -637,8 +805,18 @@ const DevTiles* tiles_for(slq_op* op, bool trans)
     return &t;
 }
 
+// Buffers of one SpMM launch.
 template <typename T>
-int launch_spmm(slq_op* op, const SpmmArgs<T>& s, const DevTiles* tl = nullptr)
+struct SpmmIO {
+    const T* X = nullptr;
+    T* Y = nullptr;
+    T* Vout = nullptr;
+    const T* Sub = nullptr;
+};
+
+template <typename T>
+int launch_spmm(slq_op* op, const SpmmArgs<T>& s, const SpmmIO<T>& io,
+                const DevTiles* tl = nullptr)
 {
     hipEvent_t ev = nullptr;
     prof_begin(op, &ev);
@@ -646,53 +824,57 @@ int launch_spmm(slq_op* op, const SpmmArgs<T>& s, const DevTiles* tl = nullptr)
     {
         const T* tv = (sizeof(T) == 4 && op->A->csr.dtype == SLQ_F64)
                           ? (const T*) tl->vals_f32 : (const T*) tl->vals;
+        const int grid = grid_for(tl->ntiles);
         if (tl->R == 8)
-            hipLaunchKernelGGL((tiled_spmm_kernel<T, 8>), dim3(grid_for(tl->ntiles)),
-                               dim3(kThreads), 0, op->stream, s, tl->ntiles, tl->ptr,
-                               tl->cols, tv);
+            hipLaunchKernelGGL((tiled_spmm_kernel<T, 8>), dim3(grid),
+                               dim3(kThreads), 0, op->stream, s, io.X, io.Y, io.Vout, io.Sub,
+                               tl->ntiles, tl->ptr, tl->cols, tv);
         else
-            hipLaunchKernelGGL((tiled_spmm_kernel<T, 16>), dim3(grid_for(tl->ntiles)),
-                               dim3(kThreads), 0, op->stream, s, tl->ntiles, tl->ptr,
-                               tl->cols, tv);
+            hipLaunchKernelGGL((tiled_spmm_kernel<T, 16>), dim3(grid),
+                               dim3(kThreads), 0, op->stream, s, io.X, io.Y, io.Vout, io.Sub,
+                               tl->ntiles, tl->ptr, tl->cols, tv);
     }
     else
-        hipLaunchKernelGGL(spmm_kernel<T>, dim3(grid_for(s.nrows)), dim3(kThreads),
-                           0, op->stream, s);
+        hipLaunchKernelGGL(spmm_kernel<T>, dim3(grid_for(s.nrows)), dim3(kThreads), 0,
+                           op->stream, s, io.X, io.Y, io.Vout, io.Sub);
     SLQ_CHECK(hipGetLastError());
     prof_end(op, ev, 0);
     return SLQ_OK;
 }
 
 template <typename T>
-int launch_update(slq_op* op, const UpdateArgs<T>& u)
+int launch_update(slq_op* op, const UpdateArgs& u, T* R, const T* Vj, const T* Vjm1)
 {
     hipEvent_t ev = nullptr;
     prof_begin(op, &ev);
-    hipLaunchKernelGGL(lanczos_update_kernel<T>, dim3(grid_for(u.nrows)),
-                       dim3(kThreads), 0, op->stream, u);
+    hipLaunchKernelGGL(lanczos_update_kernel<T>, dim3(stream_grid((u.nrows + 3) / 4)),
+                       dim3(kThreads), 0, op->stream, u, R, Vj, Vjm1);
     SLQ_CHECK(hipGetLastError());
     prof_end(op, ev, 1);
     return SLQ_OK;
 }
+
+// Grid of the Gram-Schmidt kernels: fewer workgroups, since every one writes
+// K partial rows.
+int gs_grid(int64_t rows) { return stream_grid(rows); }
 
 // CGS2 of r against the K newest basis vectors (slots last, last-1, ...);
 // the final pass finishes the norm (beta_j, or alpha_j for Golub-Kahn u).
 template <typename T>
 int reorthogonalize(slq_op* op, int64_t nrows, T* R, const T* basis, int last,
                     int K, double* h1, double* h2, double* partial_k,
-                    double* partial, unsigned int* counter, double* norm_out,
-                    double* scale_out, int32_t* sizes, int j, double tol_sqrt_n)
+                    const RedBuf& rb, double* norm_out, double* scale_out,
+                    int32_t* sizes, int j, double tol_sqrt_n)
 {
-    const int G = grid_for(nrows);
-    GsArgs<T> g;
+    const int G = gs_grid(nrows);
+    GsArgs g;
     std::memset(&g, 0, sizeof(g));
     g.nrows = nrows;
-    g.R = R;
-    g.basis = basis;
     g.last = last;
     g.K = K;
-    g.counter = counter;
-    g.beta_out = norm_out;
+    g.partial_k = partial_k;
+    g.rb = rb;
+    g.norm_out = norm_out;
     g.scale_out = scale_out;
     g.sizes = sizes;
     g.j = j;
@@ -702,41 +884,25 @@ int reorthogonalize(slq_op* op, int64_t nrows, T* R, const T* basis, int last,
         // pass 1: h1 = V^T r
         g.h_sub = nullptr;
         g.mode = 0;
-        g.partial = partial_k;
-        hipLaunchKernelGGL(gs_kernel<T>, dim3(G), dim3(kThreads), 0, op->stream, g);
+        hipLaunchKernelGGL(gs_kernel<T>, dim3(G), dim3(kThreads), 0, op->stream, g, R, basis);
         SLQ_CHECK(hipGetLastError());
-        hipLaunchKernelGGL(reduce_partials_kernel, dim3(K), dim3(256), 0, op->stream,
+        hipLaunchKernelGGL(reduce_partials_kernel, dim3(K), dim3(1024), 0, op->stream,
                            partial_k, G, K, h1);
         SLQ_CHECK(hipGetLastError());
         // pass 2: r -= V h1; h2 = V^T r
         g.h_sub = h1;
-        hipLaunchKernelGGL(gs_kernel<T>, dim3(G), dim3(kThreads), 0, op->stream, g);
+        hipLaunchKernelGGL(gs_kernel<T>, dim3(G), dim3(kThreads), 0, op->stream, g, R, basis);
         SLQ_CHECK(hipGetLastError());
-        hipLaunchKernelGGL(reduce_partials_kernel, dim3(K), dim3(256), 0, op->stream,
+        hipLaunchKernelGGL(reduce_partials_kernel, dim3(K), dim3(1024), 0, op->stream,
                            partial_k, G, K, h2);
         SLQ_CHECK(hipGetLastError());
         g.h_sub = h2;
     }
     // pass 3: r -= V h2; norm
     g.mode = 1;
-    g.partial = partial;
-    hipLaunchKernelGGL(gs_kernel<T>, dim3(G), dim3(kThreads), 0, op->stream, g);
+    hipLaunchKernelGGL(gs_kernel<T>, dim3(G), dim3(kThreads), 0, op->stream, g, R, basis);
     SLQ_CHECK(hipGetLastError());
     return SLQ_OK;
-}
-
-// X[i][p] *= scale[p]; then scale[p] = 1 (second launch, after every reader).
-template <typename T>
-__global__ __launch_bounds__(256) void scale_rows_kernel(T* __restrict__ X, int64_t n,
-                                                         const double* __restrict__ scale)
-{
-    const int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n * kBlock) X[i] = (T) ((double) X[i] * scale[i % kBlock]);
-}
-
-__global__ void set_ones_kernel(double* s)
-{
-    if (threadIdx.x < kBlock) s[threadIdx.x] = 1.0;
 }
 
 template <typename T>
@@ -748,14 +914,6 @@ int normalize_in_place(slq_op* op, T* X, int64_t n, double* scale)
     hipLaunchKernelGGL(set_ones_kernel, dim3(1), dim3(kBlock), 0, op->stream, scale);
     SLQ_CHECK(hipGetLastError());
     return SLQ_OK;
-}
-
-template <typename T>
-__global__ __launch_bounds__(256) void copy_block_kernel(const T* __restrict__ in,
-                                                         T* __restrict__ out, int64_t n)
-{
-    const int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) out[i] = in[i];
 }
 
 template <typename T>
@@ -778,8 +936,7 @@ int lanczos(const RecurrenceArgs& ra, Bump& bump)
     T* Q = bump.take<T>(blk);
     T* V = bump.take<T>(blk * nslots);
     double* scale = bump.take<double>(kBlock);
-    double* partial = bump.take<double>((size_t) kMaxGrid * kBlock);
-    unsigned int* counter = bump.take<unsigned int>(64);
+    const RedBuf rb = take_redbuf(bump);
     double* h1 = nullptr;
     double* h2 = nullptr;
     double* partial_k = nullptr;
@@ -789,19 +946,19 @@ int lanczos(const RecurrenceArgs& ra, Bump& bump)
         h2 = bump.take<double>((size_t) (m + 1) * kBlock);
         partial_k = bump.take<double>((size_t) kMaxGrid * (m + 1) * kBlock);
     }
-    if (!P || !Q || !V || !scale || !partial || !counter ||
-        (reorth && (!h1 || !h2 || !partial_k)))
+    if (!P || !Q || !V || !scale || !redbuf_ok(rb) || (reorth && (!h1 || !h2 || !partial_k)))
     {
         set_error("internal: Lanczos workspace too small");
         return SLQ_E_OOM;
     }
-    hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, op->stream, counter, 64);
+    zero_counters(op->stream, rb);
     // initial_beta = |probe| = sqrt(n) exactly for a Rademacher probe
     const double s0 = 1.0 / std::sqrt((double) n);
     hipLaunchKernelGGL(init_block_kernel, dim3(1), dim3(kBlock), 0, op->stream,
                        scale, ra.sizes, kBlock, s0);
     SLQ_CHECK(hipGetLastError());
     const double tol_sqrt_n = ra.tol * std::sqrt((double) n);
+    const DevTiles* tl = tiles_for<T>(op, false);
 
     const T* src = (const T*) ra.X0;
     T* bufs[2] = {Q, P};
@@ -816,41 +973,36 @@ int lanczos(const RecurrenceArgs& ra, Bump& bump)
         s.shift_mode = op->kind == OP_AFFINE_I ? 1 : (op->kind == OP_AFFINE_B ? 2 : 0);
         s.shift_len = A.csr.nrows < A.csr.ncols ? A.csr.nrows : A.csr.ncols;
         s.t = ra.t;
-        s.X = src;
         s.xscale = scale;
-        s.Vout = Vj;
-        s.Y = W;
         s.red_mode = 1;
-        s.partial = partial;
-        s.counter = counter;
+        s.rb = rb;
         s.red_out = ra.alpha + (size_t) j * kBlock;
-        SLQ_TRY(launch_spmm<T>(op, s, tiles_for<T>(op, false)));
+        SpmmIO<T> io;
+        io.X = src;
+        io.Y = W;
+        io.Vout = Vj;
+        SLQ_TRY(launch_spmm<T>(op, s, io, tl));
 
-        UpdateArgs<T> u;
+        UpdateArgs u;
         std::memset(&u, 0, sizeof(u));
         u.nrows = n;
-        u.R = W;
-        u.Vj = Vj;
-        u.Vjm1 = Vjm1;
         u.alpha = ra.alpha + (size_t) j * kBlock;
         u.beta = j > 0 ? ra.beta + (size_t) (j - 1) * kBlock : nullptr;
         u.do_norm = reorth ? 0 : 1;
-        u.partial = partial;
-        u.counter = counter;
+        u.rb = rb;
         u.beta_out = ra.beta + (size_t) j * kBlock;
         u.scale_out = scale;
         u.sizes = ra.sizes;
         u.j = j;
         u.tol_sqrt_n = tol_sqrt_n;
-        SLQ_TRY(launch_update<T>(op, u));
+        SLQ_TRY(launch_update<T>(op, u, W, Vj, Vjm1));
 
         if (reorth)
         {
             const int num_ortho = j < buffer ? j + 1 : buffer;
-            SLQ_TRY(reorthogonalize<T>(op, n, W, V, j, num_ortho, h1, h2,
-                                       partial_k, partial, counter,
-                                       ra.beta + (size_t) j * kBlock, scale,
-                                       ra.sizes, j, tol_sqrt_n));
+            SLQ_TRY(reorthogonalize<T>(op, n, W, V, j, num_ortho, h1, h2, partial_k, rb,
+                                       ra.beta + (size_t) j * kBlock, scale, ra.sizes, j,
+                                       tol_sqrt_n));
         }
         src = W;
     }
@@ -883,8 +1035,7 @@ int golub_kahn(const RecurrenceArgs& ra, Bump& bump)
     T* V = bump.take<T>(vblk * nslots);
     double* su = bump.take<double>((size_t) (m + 1) * kBlock);   // 1/alpha_j
     double* sv = bump.take<double>((size_t) (m + 1) * kBlock);   // 1/beta_{j-1}
-    double* partial = bump.take<double>((size_t) kMaxGrid * kBlock);
-    unsigned int* counter = bump.take<unsigned int>(64);
+    const RedBuf rb = take_redbuf(bump);
     int32_t* usizes = bump.take<int32_t>(kBlock);
     double* h1 = nullptr;
     double* h2 = nullptr;
@@ -895,13 +1046,13 @@ int golub_kahn(const RecurrenceArgs& ra, Bump& bump)
         h2 = bump.take<double>((size_t) (m + 1) * kBlock);
         partial_k = bump.take<double>((size_t) kMaxGrid * (m + 1) * kBlock);
     }
-    if (!U || !V || !su || !sv || !partial || !counter || !usizes ||
+    if (!U || !V || !su || !sv || !redbuf_ok(rb) || !usizes ||
         (reorth && (!h1 || !h2 || !partial_k)))
     {
         set_error("internal: Golub-Kahn workspace too small");
         return SLQ_E_OOM;
     }
-    hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, op->stream, counter, 64);
+    zero_counters(op->stream, rb);
     // v_0 = probe / |probe|, |probe| = sqrt(ncols) exactly
     const double s0 = 1.0 / std::sqrt((double) nc);
     hipLaunchKernelGGL(init_block_kernel, dim3(1), dim3(kBlock), 0, op->stream,
@@ -921,6 +1072,8 @@ int golub_kahn(const RecurrenceArgs& ra, Bump& bump)
     const double tol_sqrt_n = ra.tol * std::sqrt((double) nc);
     const int64_t shift_len = nr < nc ? nr : nc;
     const int mode = op->kind == OP_AFFINE_I ? 1 : (op->kind == OP_AFFINE_B ? 2 : 0);
+    const DevTiles* tla = tiles_for<T>(op, false);
+    const DevTiles* tlt = tiles_for<T>(op, true);
 
     for (int j = 0; j < m; ++j)
     {
@@ -931,32 +1084,30 @@ int golub_kahn(const RecurrenceArgs& ra, Bump& bump)
         s.shift_mode = mode;
         s.shift_len = shift_len;
         s.t = ra.t;
-        s.X = vslot(j);
         s.xscale = sv + (size_t) j * kBlock;
-        s.Y = uslot(j);
+        SpmmIO<T> io;
+        io.X = vslot(j);
+        io.Y = uslot(j);
         if (j > 0)
         {
-            s.Sub = uslot(j - 1);
+            io.Sub = uslot(j - 1);
             s.sub_coef = ra.beta + (size_t) (j - 1) * kBlock;
             s.sub_scale = su + (size_t) (j - 1) * kBlock;
         }
         s.red_mode = reorth ? 0 : 2;
         s.finish_sqrt = 1;
-        s.partial = partial;
-        s.counter = counter;
+        s.rb = rb;
         s.red_out = ra.alpha + (size_t) j * kBlock;
         s.scale_out = su + (size_t) j * kBlock;
         s.sizes = usizes;            // a small alpha never ends the recurrence
         s.j = j;
         s.tol_sqrt_n = -1.0;
-        SLQ_TRY(launch_spmm<T>(op, s, tiles_for<T>(op, false)));
+        SLQ_TRY(launch_spmm<T>(op, s, io, tla));
         if (reorth)
         {
             const int num_ortho = j < buffer ? j : buffer - 1;
-            SLQ_TRY(reorthogonalize<T>(op, nr, uslot(j), U, j - 1,
-                                       j > 0 ? num_ortho : 0, h1, h2, partial_k,
-                                       partial, counter,
-                                       ra.alpha + (size_t) j * kBlock,
+            SLQ_TRY(reorthogonalize<T>(op, nr, uslot(j), U, j - 1, j > 0 ? num_ortho : 0,
+                                       h1, h2, partial_k, rb, ra.alpha + (size_t) j * kBlock,
                                        su + (size_t) j * kBlock, usizes, j, -1.0));
             SLQ_TRY(normalize_in_place<T>(op, uslot(j), nr, su + (size_t) j * kBlock));
         }
@@ -968,30 +1119,29 @@ int golub_kahn(const RecurrenceArgs& ra, Bump& bump)
         t.shift_mode = mode;
         t.shift_len = shift_len;
         t.t = ra.t;
-        t.X = uslot(j);
         t.xscale = su + (size_t) j * kBlock;
-        t.Y = vslot(j + 1);
-        t.Sub = vslot(j);
+        SpmmIO<T> io2;
+        io2.X = uslot(j);
+        io2.Y = vslot(j + 1);
+        io2.Sub = vslot(j);
         t.sub_coef = ra.alpha + (size_t) j * kBlock;
         t.sub_scale = sv + (size_t) j * kBlock;
         t.red_mode = reorth ? 0 : 2;
         t.finish_sqrt = 1;
-        t.partial = partial;
-        t.counter = counter;
+        t.rb = rb;
         t.red_out = ra.beta + (size_t) j * kBlock;
         t.scale_out = sv + (size_t) (j + 1) * kBlock;
         t.sizes = ra.sizes;
         t.j = j;
         t.tol_sqrt_n = tol_sqrt_n;
-        SLQ_TRY(launch_spmm<T>(op, t, tiles_for<T>(op, true)));
+        SLQ_TRY(launch_spmm<T>(op, t, io2, tlt));
         if (reorth)
         {
             const int num_ortho = j < buffer ? j : buffer - 1;
-            SLQ_TRY(reorthogonalize<T>(op, nc, vslot(j + 1), V, j, num_ortho, h1,
-                                       h2, partial_k, partial, counter,
-                                       ra.beta + (size_t) j * kBlock,
-                                       sv + (size_t) (j + 1) * kBlock, ra.sizes,
-                                       j, tol_sqrt_n));
+            SLQ_TRY(reorthogonalize<T>(op, nc, vslot(j + 1), V, j, num_ortho, h1, h2,
+                                       partial_k, rb, ra.beta + (size_t) j * kBlock,
+                                       sv + (size_t) (j + 1) * kBlock, ra.sizes, j,
+                                       tol_sqrt_n));
             SLQ_TRY(normalize_in_place<T>(op, vslot(j + 1), nc,
                                           sv + (size_t) (j + 1) * kBlock));
         }
@@ -1016,23 +1166,23 @@ size_t recurrence_scratch_bytes(const slq_op* op, int gram, int m, int ortho,
         add(blk);
         add(blk);
         add(blk * (reorth ? m : 2));
+        add(sizeof(double) * kBlock);
     }
     else
     {
         const int ns = reorth ? m + 1 : 3;
         add((size_t) nr * kBlock * value_bytes * ns);
         add((size_t) nc * kBlock * value_bytes * ns);
-        add(sizeof(double) * (m + 1) * kBlock * 2);
+        add(sizeof(double) * (m + 1) * kBlock);
+        add(sizeof(double) * (m + 1) * kBlock);
         add(sizeof(int32_t) * kBlock);
     }
-    add(sizeof(double) * kBlock);
-    add(sizeof(double) * kMaxGrid * kBlock);
-    add(sizeof(unsigned int) * 64);
+    b += redbuf_bytes();
     if (reorth)
     {
-        add(sizeof(double) * (m + 1) * kBlock * 2);
+        add(sizeof(double) * (m + 1) * kBlock);
+        add(sizeof(double) * (m + 1) * kBlock);
         add(sizeof(double) * (size_t) kMaxGrid * (m + 1) * kBlock);
-        add(sizeof(int32_t) * (m + 1));
     }
     return b + 4096;
 }
@@ -1056,28 +1206,29 @@ int run_spmm(slq_op* op, double t, int transpose, const void* X, void* Y,
 {
     MatrixStore& A = *op->A;
     MatrixStore* Bm = op->kind == OP_AFFINE_B ? op->B.get() : nullptr;
-    const bool f32 = value_dtype == SLQ_F32;
     const int64_t shift_len = A.csr.nrows < A.csr.ncols ? A.csr.nrows : A.csr.ncols;
     const int mode = op->kind == OP_AFFINE_I ? 1 : (op->kind == OP_AFFINE_B ? 2 : 0);
-    if (f32)
+    if (value_dtype == SLQ_F32)
     {
         SpmmArgs<float> s = spmm_defaults<float>();
         fill_matrix<float>(s, A, Bm, transpose != 0, A.csr.dtype == SLQ_F64);
         s.shift_mode = mode;
         s.shift_len = shift_len;
         s.t = t;
-        s.X = (const float*) X;
-        s.Y = (float*) Y;
-        return launch_spmm<float>(op, s, tiles_for<float>(op, transpose != 0));
+        SpmmIO<float> io;
+        io.X = (const float*) X;
+        io.Y = (float*) Y;
+        return launch_spmm<float>(op, s, io, tiles_for<float>(op, transpose != 0));
     }
     SpmmArgs<double> s = spmm_defaults<double>();
     fill_matrix<double>(s, A, Bm, transpose != 0, false);
     s.shift_mode = mode;
     s.shift_len = shift_len;
     s.t = t;
-    s.X = (const double*) X;
-    s.Y = (double*) Y;
-    return launch_spmm<double>(op, s, tiles_for<double>(op, transpose != 0));
+    SpmmIO<double> io;
+    io.X = (const double*) X;
+    io.Y = (double*) Y;
+    return launch_spmm<double>(op, s, io, tiles_for<double>(op, transpose != 0));
 }
 
 int tiles_f32(hipStream_t s, DevTiles& t, int dtype)

@@ -39,11 +39,15 @@ constexpr int kBlock = 64;
 constexpr int kWave = 64;
 
 // Persistent-style grids: workgroups of kThreads threads (kWavesPerWG
-// wavefronts), each taking a contiguous run of rows.
+// wavefronts) sweeping the rows in XCD windows. kGrid workgroups (2 per CU)
+// was measured best for the SpMM at 2^20 rows: more resident wavefronts
+// widen the window of gathered rows past an XCD's L2, fewer cannot hide the
+// gather latency. kMaxGrid bounds the reduction buffers.
 constexpr int kThreads = 512;
 constexpr int kWavesPerWG = kThreads / kWave;
 constexpr int kXcds = 8;           // MI355X: 8 XCDs, workgroups round-robin
-constexpr int kMaxGrid = 2048;     // upper bound of reduction partials
+constexpr int kGrid = 512;
+constexpr int kMaxGrid = 1024;
 
 // ---------------------------------------------------------------- matrices
 
