@@ -264,11 +264,12 @@ struct SpmmArgs {
     int32_t* sizes;
     int j;
     double tol_sqrt_n;
+    int nt_store;           // non-temporal stores of Y / Vout
 };
 
 // Epilogue of one output row: optional v = s X[row] (written to Vout), the
 // affine shift, the subtraction, the store and the reduction term.
-template <typename T>
+template <typename T, bool NT>
 __device__ __forceinline__ void spmm_epilogue(const SpmmArgs<T>& a, int64_t row, int lane,
                                               T y, T s, T tt, T sub_c, T sub_s,
                                               const T* __restrict__ X,
@@ -279,11 +280,20 @@ __device__ __forceinline__ void spmm_epilogue(const SpmmArgs<T>& a, int64_t row,
     const bool shift_row = a.shift_mode == 1 && row < a.shift_len;
     T v = T(0);
     if (Vout || a.red_mode == 1 || shift_row) v = s * X[o];
-    if (Vout) Vout[o] = v;
     // imate adds t x through subtract_scaled_vector(-t): y - (-t) x
     if (shift_row) y = y + tt * v;
     if (Sub) y = y - sub_c * (sub_s * Sub[o]);
-    Y[o] = y;
+    if constexpr (NT)
+    {
+        // streamed outputs: do not let them evict gathered operand rows
+        if (Vout) __builtin_nontemporal_store(v, &Vout[o]);
+        __builtin_nontemporal_store(y, &Y[o]);
+    }
+    else
+    {
+        if (Vout) Vout[o] = v;
+        Y[o] = y;
+    }
     if (a.red_mode == 1) red += (double) v * (double) y;
     else if (a.red_mode == 2) red += (double) y * (double) y;
 }
@@ -304,7 +314,7 @@ __device__ __forceinline__ void spmm_finish(const SpmmArgs<T>& a, double red, in
 }
 
 // CSR form: one wavefront per row.
-template <typename T>
+template <typename T, bool NT>
 __global__ __launch_bounds__(kThreads) void spmm_kernel(SpmmArgs<T> a,
                                                         const T* __restrict__ X,
                                                         T* __restrict__ Y,
@@ -325,7 +335,7 @@ __global__ __launch_bounds__(kThreads) void spmm_kernel(SpmmArgs<T> a,
         if (a.shift_mode == 2)
             y = y + tt * row_gather<T>(a.b_indptr[row], a.b_indptr[row + 1], a.b_indices,
                                        a.b_vals, X, s, lane);
-        spmm_epilogue<T>(a, row, lane, y, s, tt, sub_c, sub_s, X, Y, Vout, Sub, red);
+        spmm_epilogue<T, NT>(a, row, lane, y, s, tt, sub_c, sub_s, X, Y, Vout, Sub, red);
     }
     spmm_finish<T>(a, red, lane);
 }
@@ -334,7 +344,7 @@ __global__ __launch_bounds__(kThreads) void spmm_kernel(SpmmArgs<T> a,
 // Each gathered operand row X[u] feeds the R values of column u, read with
 // scalar loads (the slab address is wavefront-uniform, and a scalar operand
 // feeds the FMA directly).
-template <typename T, int R>
+template <typename T, int R, bool NT>
 __global__ __launch_bounds__(kThreads) void tiled_spmm_kernel(
     SpmmArgs<T> a, const T* __restrict__ X, T* __restrict__ Y, T* __restrict__ Vout,
     const T* __restrict__ Sub, int64_t ntiles, const int64_t* __restrict__ tptr,
@@ -396,8 +406,8 @@ __global__ __launch_bounds__(kThreads) void tiled_spmm_kernel(
         {
             const int64_t row = t * R + r;
             if (row < a.nrows)
-                spmm_epilogue<T>(a, row, lane, acc[r], s, tt, sub_c, sub_s, X, Y, Vout, Sub,
-                                 red);
+                spmm_epilogue<T, NT>(a, row, lane, acc[r], s, tt, sub_c, sub_s, X, Y, Vout,
+                                     Sub, red);
         }
     }
     spmm_finish<T>(a, red, lane);
@@ -785,11 +795,21 @@ void fill_matrix(SpmmArgs<T>& s, MatrixStore& A, MatrixStore* B, bool trans,
     }
 }
 
+int nt_default()
+{
+    static int v = [] {
+        const char* e = std::getenv("SLQ_NT_STORE");
+        return e ? std::atoi(e) : 1;
+    }();
+    return v;
+}
+
 template <typename T>
 SpmmArgs<T> spmm_defaults()
 {
     SpmmArgs<T> s;
     std::memset(&s, 0, sizeof(s));
+    s.nt_store = nt_default();
     return s;
 }
 
@@ -814,29 +834,36 @@ struct SpmmIO {
     const T* Sub = nullptr;
 };
 
-template <typename T>
-int launch_spmm(slq_op* op, const SpmmArgs<T>& s, const SpmmIO<T>& io,
-                const DevTiles* tl = nullptr)
+template <typename T, bool NT>
+void launch_spmm_nt(slq_op* op, const SpmmArgs<T>& s, const SpmmIO<T>& io, const DevTiles* tl)
 {
-    hipEvent_t ev = nullptr;
-    prof_begin(op, &ev);
     if (tl)
     {
         const T* tv = (sizeof(T) == 4 && op->A->csr.dtype == SLQ_F64)
                           ? (const T*) tl->vals_f32 : (const T*) tl->vals;
         const int grid = grid_for(tl->ntiles);
         if (tl->R == 8)
-            hipLaunchKernelGGL((tiled_spmm_kernel<T, 8>), dim3(grid),
-                               dim3(kThreads), 0, op->stream, s, io.X, io.Y, io.Vout, io.Sub,
-                               tl->ntiles, tl->ptr, tl->cols, tv);
+            hipLaunchKernelGGL((tiled_spmm_kernel<T, 8, NT>), dim3(grid), dim3(kThreads), 0,
+                               op->stream, s, io.X, io.Y, io.Vout, io.Sub, tl->ntiles,
+                               tl->ptr, tl->cols, tv);
         else
-            hipLaunchKernelGGL((tiled_spmm_kernel<T, 16>), dim3(grid),
-                               dim3(kThreads), 0, op->stream, s, io.X, io.Y, io.Vout, io.Sub,
-                               tl->ntiles, tl->ptr, tl->cols, tv);
+            hipLaunchKernelGGL((tiled_spmm_kernel<T, 16, NT>), dim3(grid), dim3(kThreads), 0,
+                               op->stream, s, io.X, io.Y, io.Vout, io.Sub, tl->ntiles,
+                               tl->ptr, tl->cols, tv);
     }
     else
-        hipLaunchKernelGGL(spmm_kernel<T>, dim3(grid_for(s.nrows)), dim3(kThreads), 0,
+        hipLaunchKernelGGL((spmm_kernel<T, NT>), dim3(grid_for(s.nrows)), dim3(kThreads), 0,
                            op->stream, s, io.X, io.Y, io.Vout, io.Sub);
+}
+
+template <typename T>
+int launch_spmm(slq_op* op, const SpmmArgs<T>& s, const SpmmIO<T>& io,
+                const DevTiles* tl = nullptr)
+{
+    hipEvent_t ev = nullptr;
+    prof_begin(op, &ev);
+    if (s.nt_store) launch_spmm_nt<T, true>(op, s, io, tl);
+    else launch_spmm_nt<T, false>(op, s, io, tl);
     SLQ_CHECK(hipGetLastError());
     prof_end(op, ev, 0);
     return SLQ_OK;
