@@ -50,6 +50,11 @@ def parse():
     p.add_argument('--cpu-threads', type=int, default=0)
     p.add_argument('--copy-calibration', action='store_true',
                    help='only run the plain copy kernel (rocprof calibration)')
+    p.add_argument('--form', default='plain',
+                   choices=['plain', 'precision', 'batched', 'surrogate'],
+                   help='plain: the headline workload (default); the others '
+                        'measure the north-star forms and print their own '
+                        'JSON lines')
     return p.parse_args()
 
 
@@ -89,6 +94,126 @@ def cpu_baseline(A, m, threads, seed):
     return steps / dt, dt, steps
 
 
+def precision_form(args):
+    """Precision form: n = 65536 correlation matrix with kernel scale 0.01
+    (~247 non-zeros per row, cond ~1e4), lanczos_degree 60, full
+    re-orthogonalisation, 64-probe blocks. FP32 SpMM / basis / Gram-Schmidt
+    with FP64 recurrences and eigensolve ('mixed') against the plain FP64 GPU
+    path; errors of logdet, traceinv and Schatten p=2 against exact values
+    (dense FP64 Cholesky, tests/golden/exact_precision_form.json)."""
+    import imate_amd
+    from imate_amd.sample_matrices import sparse_correlation_matrix
+    n, m, s = 65536, 60, args.samples
+    A = sparse_correlation_matrix(n, scale=0.01, seed=0)
+    exact = {}
+    path = os.path.join(HERE, 'tests', 'golden', 'exact_precision_form.json')
+    if os.path.exists(path):
+        with open(path) as f:
+            exact = json.load(f)
+    Aop = imate_amd.Matrix(A)
+    Aop.get_linear_operator(gpu=True)
+    rows = []
+    for prec in ('native', 'mixed'):
+        for ortho in (-1, 0):
+            kw = dict(method='slq', gpu=True, lanczos_degree=m,
+                      min_num_samples=s, max_num_samples=s, seed=args.seed,
+                      orthogonalize=ortho, precision=prec, return_info=True)
+            imate_amd.logdet(Aop, **kw)   # warm-up (workspace, FP32 copies)
+            res = {}
+            t0 = time.perf_counter()
+            ld, i1 = imate_amd.logdet(Aop, **kw)
+            ti, i2 = imate_amd.traceinv(Aop, **kw)
+            t2, i3 = imate_amd.trace(Aop, p=2, **kw)
+            el = time.perf_counter() - t0
+            sch = (t2 / n) ** 0.5
+            res = {'precision': prec, 'orthogonalize': ortho,
+                   'seconds_3_calls': el,
+                   'matvec_steps_per_s': 3 * s * m / el,
+                   'logdet': ld, 'logdet_mc_err': float(i1['error']['absolute_error']),
+                   'traceinv': ti, 'traceinv_mc_err': float(i2['error']['absolute_error']),
+                   'schatten2': sch}
+            if exact:
+                res['logdet_rel_err'] = abs(ld - exact['logdet']) / abs(exact['logdet'])
+                res['traceinv_rel_err'] = abs(ti - exact['traceinv']) / abs(exact['traceinv'])
+                res['schatten2_rel_err'] = abs(sch - exact['schatten2']) / exact['schatten2']
+            log('precision form: %s' % json.dumps(res))
+            rows.append(res)
+    print(json.dumps({'form': 'precision', 'n': n, 'nnz': int(A.nnz),
+                      'lanczos_degree': m, 'samples': s, 'exact': exact,
+                      'runs': rows}), flush=True)
+
+
+def batched_form(args):
+    """Batched form. (i) A + tI with 256 log-spaced t in [1e-4, 1e2] on an
+    n = 65536 correlation matrix (~50 non-zeros per row, FP64): one Lanczos
+    run per probe with Ritz shifts for every t, against a loop of one SLQ run
+    per t. (ii) 32 correlation matrices with sizes log-uniform in [1e3, 2e5],
+    ~30 non-zeros per row, FP32: one hipGraph launch over all problems (with
+    autotuned launch grids) against a per-problem launch loop."""
+    import imate_amd
+    from imate_amd.batch import slq_batch
+    from imate_amd.sample_matrices import sparse_correlation_matrix
+    m, s = 30, 64
+    out = {'form': 'batched', 'lanczos_degree': m, 'samples': s}
+    # (i) many t of A + tI
+    n = 65536
+    A = sparse_correlation_matrix(n, nnz_per_row=50, seed=0)
+    ts = numpy.logspace(-4, 2, 256)
+    Aop = imate_amd.AffineMatrixFunction(A)
+    kw = dict(method='slq', gpu=True, lanczos_degree=m, min_num_samples=s,
+              max_num_samples=s, seed=args.seed, return_info=True)
+    imate_amd.logdet(Aop, parameters=ts, **kw)
+    t0 = time.perf_counter()
+    vals, info = imate_amd.logdet(Aop, parameters=ts, **kw)
+    t_shift = time.perf_counter() - t0
+    loop_ts = ts[::16]
+    t0 = time.perf_counter()
+    direct = []
+    for t in loop_ts:
+        v, _ = imate_amd.logdet(Aop, parameters=float(t), **kw)
+        direct.append(v)
+    t_loop = (time.perf_counter() - t0) * (len(ts) / len(loop_ts))
+    rel = numpy.max(numpy.abs(numpy.array(direct) - vals[::16]) /
+                    numpy.abs(numpy.array(direct)))
+    out['affine_t256'] = {
+        'n': n, 'nnz': int(A.nnz), 'num_t': len(ts),
+        'shift_seconds': t_shift, 'per_t_loop_seconds_extrapolated': t_loop,
+        'speedup': t_loop / t_shift,
+        't_evaluations_per_s': len(ts) * s / t_shift,
+        'max_rel_diff_shift_vs_direct': float(rel)}
+    log('affine: %s' % json.dumps(out['affine_t256']))
+    # (ii) 32 uneven matrices
+    rng = numpy.random.default_rng(args.seed)
+    sizes = numpy.round(numpy.exp(rng.uniform(numpy.log(1e3), numpy.log(2e5),
+                                              32))).astype(int)
+    mats = [sparse_correlation_matrix(int(k), nnz_per_row=30, seed=i,
+                                      dtype='float32')
+            for i, k in enumerate(sizes)]
+    nnz = sum(M.nnz for M in mats)
+    tg, ig = slq_batch(mats, lanczos_degree=m, num_samples=s, seed=args.seed,
+                       use_graph=True)
+    tg, ig = slq_batch([ig['operators'][i] for i in range(32)],
+                       lanczos_degree=m, num_samples=s, seed=args.seed,
+                       use_graph=True)
+    tl, il = slq_batch(ig['operators'], lanczos_degree=m, num_samples=s,
+                       seed=args.seed, use_graph=False, autotune=False)
+    steps = 32 * s * m
+    out['matrices32'] = {
+        'sizes': [int(k) for k in sizes], 'total_rows': int(sizes.sum()),
+        'total_nnz': int(nnz), 'dtype': 'f32',
+        'graph_seconds': ig['run_time'], 'graph_setup_seconds': ig['setup_time'],
+        'loop_seconds': il['run_time'],
+        'speedup': il['run_time'] / ig['run_time'],
+        'graph_matvec_steps_per_s': steps / ig['run_time'],
+        'loop_matvec_steps_per_s': steps / il['run_time'],
+        'samples_identical_graph_vs_loop':
+            bool(numpy.array_equal(ig['samples'], il['samples'])),
+        'autotuned_grids': ig['grids']}
+    log('matrices: %s' % json.dumps({k: v for k, v in out['matrices32'].items()
+                                     if k not in ('sizes', 'autotuned_grids')}))
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -109,6 +234,11 @@ def main():
         print(json.dumps({'copy_bytes': nbytes, 'ms_per_launch': ms.value,
                           'GBps': 2 * nbytes / ms.value / 1e6}), flush=True)
         return
+
+    if args.form == 'precision':
+        return precision_form(args)
+    if args.form == 'batched':
+        return batched_form(args)
 
     from imate_amd.sample_matrices import sparse_correlation_matrix
     n, k, m, s = args.n, args.nnz_per_row, args.lanczos_degree, args.samples

@@ -60,6 +60,10 @@ SIGNATURES = {
     'slq_profile_enable': (ctypes.c_int, [_P, _i32]),
     'slq_profile_read': (ctypes.c_int, [_P, _P, _P, _P, _P]),
     'slq_copy_bandwidth': (ctypes.c_int, [_i64, _i32, _P]),
+    'slq_estimate_batch': (ctypes.c_int, [
+        _P, _i32, _i32, _i32, _f64, _i32, _i64, _i32, _f64, _i32, _f64, _f64,
+        _i32, _i32, _P, _P, _P, _P, _P]),
+    'slq_autotune': (ctypes.c_int, [_P, _i32, _P, _P]),
 }
 
 _lib = None

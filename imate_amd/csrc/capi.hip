@@ -23,6 +23,8 @@
 #include <cstring>
 #include <ctime>
 #include <limits>
+#include <map>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -697,6 +699,297 @@ int copy_bandwidth(int64_t bytes, int iters, double* ms)
 }  // namespace slq
 
 using namespace slq;
+
+// ---------------------------------------------------------------- autotuner
+
+namespace {
+
+// Runs one probe block of `steps` Lanczos steps with the op's current grid
+// caps and returns the device time (ms), or a negative value on error.
+double time_block(slq_op* op, int steps, int precision)
+{
+    const int vdt = value_dtype(op, precision);
+    Plan p = make_plan(op, 0, steps, 0, 1, vdt == SLQ_F64 ? 8 : 4);
+    if (op->ws.reserve(p.total) != SLQ_OK) return -1.0;
+    uint64_t base[4];
+    thread_state(1, 0, base);
+    if (make_probes(op, p, base, 0, kBlock, vdt) != SLQ_OK) return -1.0;
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess) return -1.0;
+    if (hipEventCreate(&e1) != hipSuccess)
+    {
+        (void) hipEventDestroy(e0);
+        return -1.0;
+    }
+    double best = -1.0;
+    for (int rep = 0; rep < 2; ++rep)   // first repetition warms up
+    {
+        (void) hipEventRecord(e0, op->stream);
+        if (recurrence_block(op, p, 0.0, 0, 1e-300, 0, precision) != SLQ_OK) break;
+        (void) hipEventRecord(e1, op->stream);
+        if (hipEventSynchronize(e1) != hipSuccess) break;
+        float ms = 0.f;
+        (void) hipEventElapsedTime(&ms, e0, e1);
+        best = ms;
+    }
+    (void) hipEventDestroy(e0);
+    (void) hipEventDestroy(e1);
+    return best;
+}
+
+struct TuneKey {
+    int log2_rows;
+    int nnz_per_row;
+    int vdtype;
+    int tile_rows;
+    bool operator<(const TuneKey& o) const
+    {
+        if (log2_rows != o.log2_rows) return log2_rows < o.log2_rows;
+        if (nnz_per_row != o.nnz_per_row) return nnz_per_row < o.nnz_per_row;
+        if (vdtype != o.vdtype) return vdtype < o.vdtype;
+        return tile_rows < o.tile_rows;
+    }
+};
+
+std::mutex g_tune_mu;
+std::map<TuneKey, std::pair<int, int>> g_tune_cache;
+
+}  // namespace
+
+extern "C" int slq_autotune(slq_op* op, int32_t precision, int32_t* spmm_grid,
+                            int32_t* stream_grid)
+{
+    if (!op || op->kind == OP_AFFINE_B ||
+        op->A->csr.nrows != op->A->csr.ncols)
+    {
+        set_error("slq_autotune: a square matrix or A + tI handle is required");
+        return SLQ_E_ARG;
+    }
+    DeviceGuard guard(op->device);
+    SLQ_TRY(prepare(op, 0, precision));
+    const DevCSR& c = op->A->csr;
+    TuneKey key;
+    key.log2_rows = (int) std::lround(std::log2((double) c.nrows));
+    key.nnz_per_row = (int) std::lround((double) c.nnz / (double) c.nrows);
+    key.vdtype = value_dtype(op, precision);
+    key.tile_rows = op->A->tiles.R;
+    {
+        std::lock_guard<std::mutex> lock(g_tune_mu);
+        auto it = g_tune_cache.find(key);
+        if (it != g_tune_cache.end())
+        {
+            op->spmm_grid = it->second.first;
+            op->stream_grid = it->second.second;
+            if (spmm_grid) *spmm_grid = op->spmm_grid;
+            if (stream_grid) *stream_grid = op->stream_grid;
+            return SLQ_OK;
+        }
+    }
+    // candidate caps (multiples of 8 up to the natural grid of the problem)
+    const int cands[] = {8, 16, 32, 64, 128, 256, 512, 1024};
+    const int steps = 4;
+    int best_s = 0, best_u = 0;
+    double best_t = 1e300;
+    op->stream_grid = 0;
+    for (int cap : cands)
+    {
+        if (cap > grid_for(c.nrows, kMaxGrid)) break;
+        op->spmm_grid = cap;
+        const double t = time_block(op, steps, precision);
+        if (t < 0) return SLQ_E_HIP;
+        if (t < best_t)
+        {
+            best_t = t;
+            best_s = cap;
+        }
+    }
+    op->spmm_grid = best_s;
+    best_t = 1e300;
+    for (int cap : cands)
+    {
+        if (cap > grid_for(c.nrows, kMaxGrid)) break;
+        op->stream_grid = cap;
+        const double t = time_block(op, steps, precision);
+        if (t < 0) return SLQ_E_HIP;
+        if (t < best_t)
+        {
+            best_t = t;
+            best_u = cap;
+        }
+    }
+    op->stream_grid = best_u;
+    {
+        std::lock_guard<std::mutex> lock(g_tune_mu);
+        g_tune_cache[key] = {best_s, best_u};
+    }
+    if (spmm_grid) *spmm_grid = best_s;
+    if (stream_grid) *stream_grid = best_u;
+    return SLQ_OK;
+}
+
+// ---------------------------------------------------------------- batch
+
+extern "C" int slq_estimate_batch(slq_op** ops, int32_t count, int32_t func,
+                                  int32_t gram, double exponent, int32_t orthogonalize,
+                                  int64_t seed, int32_t m, double tol,
+                                  int32_t num_samples, double confidence,
+                                  double outlier, int32_t precision, int32_t use_graph,
+                                  double* trace, double* error, double* samples,
+                                  double* run_time, double* setup_time)
+{
+    if (!ops || count < 1 || m < 1 || num_samples < 1 || !trace || !error ||
+        func < SLQ_FUNC_LOG || func > SLQ_FUNC_IDENT ||
+        (precision != SLQ_PREC_NATIVE && precision != SLQ_PREC_MIXED))
+    {
+        set_error("slq_estimate_batch: invalid argument");
+        return SLQ_E_ARG;
+    }
+    for (int i = 0; i < count; ++i)
+    {
+        if (!ops[i] || ops[i]->kind != OP_MATRIX ||
+            (!gram && ops[i]->A->csr.nrows != ops[i]->A->csr.ncols) ||
+            ops[i]->device != ops[0]->device)
+        {
+            set_error("slq_estimate_batch: operator %d must be a (square unless gram) "
+                      "matrix handle on the device of operator 0", i);
+            return SLQ_E_ARG;
+        }
+        for (int j = 0; j < i; ++j)
+            if (ops[j] == ops[i])
+            {
+                set_error("slq_estimate_batch: operator %d repeated (each problem needs "
+                          "its own handle)", i);
+                return SLQ_E_ARG;
+            }
+    }
+    DeviceGuard guard(ops[0]->device);
+    if (seed < 0) seed = clock_seed();
+    uint64_t base[4];
+    thread_state(seed, 0, base);
+    const auto c0 = std::chrono::steady_clock::now();
+    // one-time uploads (jump tables) must not happen inside a capture
+    SLQ_TRY(rng_words(ops[0]->stream, base, 1, 0, 0, nullptr));
+    std::vector<Plan> plans(count);
+    std::vector<size_t> sample_off(count);
+    for (int i = 0; i < count; ++i)
+    {
+        slq_op* op = ops[i];
+        op->prof.enabled = false;
+        SLQ_TRY(prepare(op, gram, precision));
+        const int vdt = value_dtype(op, precision);
+        plans[i] = make_plan(op, gram, m, orthogonalize, 1, vdt == SLQ_F64 ? 8 : 4);
+        sample_off[i] = plans[i].total;
+        SLQ_TRY(op->ws.reserve(plans[i].total + align256(sizeof(double) * num_samples)));
+    }
+    SLQ_CHECK(hipDeviceSynchronize());
+    // every block of problem i on its own stream
+    auto enqueue = [&](int i) -> int {
+        slq_op* op = ops[i];
+        const Plan& p = plans[i];
+        const int vdt = value_dtype(op, precision);
+        double* dsamp = at<double>(op, sample_off[i]);
+        for (int64_t s0 = 0; s0 < num_samples; s0 += kBlock)
+        {
+            const int cnt = (int) std::min<int64_t>(kBlock, num_samples - s0);
+            SLQ_TRY(make_probes(op, p, base, s0, cnt, vdt));
+            SLQ_TRY(recurrence_block(op, p, 0.0, gram, tol, orthogonalize, precision));
+            SLQ_TRY(run_quadrature(op->stream, at<double>(op, p.off_alpha),
+                                   at<double>(op, p.off_beta), at<int32_t>(op, p.off_sizes),
+                                   cnt, m, gram, func, exponent, (double) p.len, nullptr, 1, 0,
+                                   0, 1, dsamp + s0, nullptr, nullptr));
+        }
+        return SLQ_OK;
+    };
+    double t_run = 0.0;
+    if (use_graph)
+    {
+        hipStream_t main = nullptr;
+        SLQ_CHECK(hipStreamCreateWithFlags(&main, hipStreamNonBlocking));
+        std::vector<hipEvent_t> ev(count + 1, nullptr);
+        int r = SLQ_OK;
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        for (auto& e : ev)
+            if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess)
+                r = SLQ_E_HIP;
+        if (r == SLQ_OK &&
+            hipStreamBeginCapture(main, hipStreamCaptureModeThreadLocal) != hipSuccess)
+            r = SLQ_E_HIP;
+        if (r == SLQ_OK)
+        {
+            bool capturing = true;
+            if (hipEventRecord(ev[count], main) != hipSuccess) r = SLQ_E_HIP;
+            for (int i = 0; i < count && r == SLQ_OK; ++i)
+            {
+                if (hipStreamWaitEvent(ops[i]->stream, ev[count], 0) != hipSuccess)
+                {
+                    r = SLQ_E_HIP;
+                    break;
+                }
+                r = enqueue(i);
+                if (r == SLQ_OK && (hipEventRecord(ev[i], ops[i]->stream) != hipSuccess ||
+                                    hipStreamWaitEvent(main, ev[i], 0) != hipSuccess))
+                    r = SLQ_E_HIP;
+            }
+            hipError_t ee = hipStreamEndCapture(main, &graph);
+            capturing = false;
+            if (r == SLQ_OK && ee != hipSuccess) r = hip_status(ee, "hipStreamEndCapture", __FILE__, __LINE__);
+            (void) capturing;
+        }
+        if (r == SLQ_OK)
+        {
+            hipError_t ee = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+            if (ee != hipSuccess) r = hip_status(ee, "hipGraphInstantiate", __FILE__, __LINE__);
+        }
+        if (r == SLQ_OK)
+        {
+            if (setup_time)
+                *setup_time = std::chrono::duration<double>(std::chrono::steady_clock::now() - c0).count();
+            const auto t0 = std::chrono::steady_clock::now();
+            hipError_t ee = hipGraphLaunch(exec, main);
+            if (ee == hipSuccess) ee = hipStreamSynchronize(main);
+            t_run = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            if (ee != hipSuccess) r = hip_status(ee, "hipGraphLaunch", __FILE__, __LINE__);
+        }
+        if (exec) (void) hipGraphExecDestroy(exec);
+        if (graph) (void) hipGraphDestroy(graph);
+        for (auto& e : ev)
+            if (e) (void) hipEventDestroy(e);
+        (void) hipStreamDestroy(main);
+        if (r != SLQ_OK)
+        {
+            if (g_last_error.empty()) set_error("slq_estimate_batch: graph capture failed");
+            return r;
+        }
+    }
+    else
+    {
+        if (setup_time)
+            *setup_time = std::chrono::duration<double>(std::chrono::steady_clock::now() - c0).count();
+        const auto t0 = std::chrono::steady_clock::now();
+        for (int i = 0; i < count; ++i)
+        {
+            SLQ_TRY(enqueue(i));
+            SLQ_CHECK(hipStreamSynchronize(ops[i]->stream));
+        }
+        t_run = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
+    if (run_time) *run_time = t_run;
+    std::vector<double> hs(num_samples);
+    std::vector<int32_t> processed(num_samples);
+    for (int k = 0; k < num_samples; ++k) processed[k] = k;
+    for (int i = 0; i < count; ++i)
+    {
+        SLQ_CHECK(hipMemcpy(hs.data(), at<double>(ops[i], sample_off[i]),
+                            sizeof(double) * num_samples, hipMemcpyDeviceToHost));
+        if (samples) std::memcpy(samples + (size_t) i * num_samples, hs.data(),
+                                 sizeof(double) * num_samples);
+        int32_t used = num_samples, nout = 0;
+        average_estimates(hs.data(), 1, confidence, outlier, &used, processed.data(), &nout,
+                          trace + i, error + i);
+    }
+    return SLQ_OK;
+}
 
 // ================================================================= C ABI
 

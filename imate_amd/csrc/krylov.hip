@@ -709,26 +709,33 @@ int max_grid()
     return g;
 }
 
-int grid_for(int64_t units)
+}  // namespace
+
+int grid_for(int64_t units, int cap)
 {
+    // reduction buffers hold kMaxGrid partial rows: never exceed it
+    int c = cap > 0 ? ((cap < kXcds ? kXcds : cap) / kXcds) * kXcds : max_grid();
+    if (c > kMaxGrid) c = kMaxGrid;
     int64_t g = (units + kWavesPerWG - 1) / kWavesPerWG;
-    if (g > max_grid()) g = max_grid();
+    if (g > c) g = c;
     g = ((g + kXcds - 1) / kXcds) * kXcds;
     if (g < kXcds) g = kXcds;
     return (int) g;
 }
 
-// Grid of the streaming kernels (update, Gram-Schmidt): 256 workgroups, one
-// per CU, measured to reach the copy-kernel bandwidth where more workgroups
-// lose ~45 % to contention.
-int stream_grid(int64_t units)
+namespace {
+
+// Grid of the streaming kernels (update, Gram-Schmidt): by default 256
+// workgroups, one per CU, measured to reach the copy-kernel bandwidth where
+// more workgroups lose ~45 % to contention.
+int stream_grid(int64_t units, int cap = 0)
 {
-    int g = grid_for(units);
-    const char* e = std::getenv("SLQ_STREAM_GRID");
-    int cap = e ? std::atoi(e) : 256;
-    if (cap < kXcds) cap = kXcds;
-    cap = (cap / kXcds) * kXcds;
-    return g > cap ? cap : g;
+    if (cap <= 0)
+    {
+        const char* e = std::getenv("SLQ_STREAM_GRID");
+        cap = e ? std::atoi(e) : 256;
+    }
+    return grid_for(units, cap);
 }
 
 struct Bump {
@@ -841,7 +848,7 @@ void launch_spmm_nt(slq_op* op, const SpmmArgs<T>& s, const SpmmIO<T>& io, const
     {
         const T* tv = (sizeof(T) == 4 && op->A->csr.dtype == SLQ_F64)
                           ? (const T*) tl->vals_f32 : (const T*) tl->vals;
-        const int grid = grid_for(tl->ntiles);
+        const int grid = grid_for(tl->ntiles, op->spmm_grid);
         if (tl->R == 8)
             hipLaunchKernelGGL((tiled_spmm_kernel<T, 8, NT>), dim3(grid), dim3(kThreads), 0,
                                op->stream, s, io.X, io.Y, io.Vout, io.Sub, tl->ntiles,
@@ -852,7 +859,8 @@ void launch_spmm_nt(slq_op* op, const SpmmArgs<T>& s, const SpmmIO<T>& io, const
                                tl->ptr, tl->cols, tv);
     }
     else
-        hipLaunchKernelGGL((spmm_kernel<T, NT>), dim3(grid_for(s.nrows)), dim3(kThreads), 0,
+        hipLaunchKernelGGL((spmm_kernel<T, NT>), dim3(grid_for(s.nrows, op->spmm_grid)),
+                           dim3(kThreads), 0,
                            op->stream, s, io.X, io.Y, io.Vout, io.Sub);
 }
 
@@ -874,7 +882,8 @@ int launch_update(slq_op* op, const UpdateArgs& u, T* R, const T* Vj, const T* V
 {
     hipEvent_t ev = nullptr;
     prof_begin(op, &ev);
-    hipLaunchKernelGGL(lanczos_update_kernel<T>, dim3(stream_grid((u.nrows + 3) / 4)),
+    hipLaunchKernelGGL(lanczos_update_kernel<T>,
+                       dim3(stream_grid((u.nrows + 3) / 4, op->stream_grid)),
                        dim3(kThreads), 0, op->stream, u, R, Vj, Vjm1);
     SLQ_CHECK(hipGetLastError());
     prof_end(op, ev, 1);
@@ -883,7 +892,7 @@ int launch_update(slq_op* op, const UpdateArgs& u, T* R, const T* Vj, const T* V
 
 // Grid of the Gram-Schmidt kernels: fewer workgroups, since every one writes
 // K partial rows.
-int gs_grid(int64_t rows) { return stream_grid(rows); }
+int gs_grid(int64_t rows, int cap) { return stream_grid(rows, cap); }
 
 // CGS2 of r against the K newest basis vectors (slots last, last-1, ...);
 // the final pass finishes the norm (beta_j, or alpha_j for Golub-Kahn u).
@@ -893,7 +902,7 @@ int reorthogonalize(slq_op* op, int64_t nrows, T* R, const T* basis, int last,
                     const RedBuf& rb, double* norm_out, double* scale_out,
                     int32_t* sizes, int j, double tol_sqrt_n)
 {
-    const int G = gs_grid(nrows);
+    const int G = gs_grid(nrows, op->stream_grid);
     GsArgs g;
     std::memset(&g, 0, sizeof(g));
     g.nrows = nrows;

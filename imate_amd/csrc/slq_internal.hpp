@@ -133,6 +133,9 @@ struct slq_op {
     int device = 0;
     slq::Workspace ws;
     slq::ProfileState prof;
+    // launch-grid caps chosen by the autotuner (0 = library default)
+    int spmm_grid = 0;
+    int stream_grid = 0;
 };
 
 namespace slq {
@@ -187,5 +190,9 @@ void prof_end(slq_op* op, hipEvent_t ev0, int kind);
 void prof_collect(slq_op* op);
 
 int copy_bandwidth(int64_t bytes, int iters, double* ms);
+
+// Grid of a launch over `units` work units with an optional cap (0 = the
+// library default cap).
+int grid_for(int64_t units, int cap = 0);
 
 }  // namespace slq

@@ -114,6 +114,32 @@ int slq_estimate(slq_op* op, const double* params, int32_t num_inquiries,
                  int32_t* num_outliers, int32_t* converged,
                  int32_t* all_converged, double* alg_wall_time);
 
+/* Batched SLQ over `count` independent matrix handles (distinct handles, one
+ * device), each with a fixed number of samples (no early stop): per problem
+ * trace[i], error[i] (after the outlier pass) and samples[i][num_samples]
+ * (may be NULL). use_graph != 0 captures every kernel of every problem into
+ * one hipGraph (one branch per problem, so uneven problems run side by side)
+ * and launches it once; use_graph == 0 runs the same kernels problem after
+ * problem (a per-problem launch loop). *run_time is the launch-to-completion
+ * time (s), *setup_time the preparation and capture time before it.
+ * Replaces a Python loop of imate.logdet(..., method='slq', gpu=True) calls
+ * over separate matrices; imate has no batched entry point. */
+int slq_estimate_batch(slq_op** ops, int32_t count, int32_t func, int32_t gram,
+                       double exponent, int32_t orthogonalize, int64_t seed,
+                       int32_t lanczos_degree, double lanczos_tol,
+                       int32_t num_samples, double confidence_level,
+                       double outlier_significance_level, int32_t precision,
+                       int32_t use_graph, double* trace, double* error,
+                       double* samples, double* run_time, double* setup_time);
+
+/* Autotuner: times one 64-probe block of Lanczos steps over candidate launch
+ * grids for the SpMM and for the streaming kernels, keeps the fastest in the
+ * handle (used by every later call on it) and caches it per (rows, non-zeros
+ * per row, value type, tile rows). Outputs the chosen caps when non-NULL.
+ * No reference counterpart (imate has no tunable kernels). */
+int slq_autotune(slq_op* op, int32_t precision, int32_t* spmm_grid,
+                 int32_t* stream_grid);
+
 /* ---- component entry points (tests, calibration and benchmarks) ---- */
 
 /* `count` consecutive Rademacher probes of length n, starting at sample
