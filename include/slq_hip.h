@@ -140,6 +140,26 @@ int slq_estimate_batch(slq_op** ops, int32_t count, int32_t func, int32_t gram,
 int slq_autotune(slq_op* op, int32_t precision, int32_t* spmm_grid,
                  int32_t* stream_grid);
 
+/* ---- learned surrogate (no reference counterpart) ----
+ * A 2 -> 128 -> 128 -> 1 SiLU MLP used as a surrogate of SLQ estimates
+ * tr f(A(l) + tI) over (t, kernel scale l). Parameters are a flat FP32 array
+ * of slq_mlp_num_params() values: W1[128][2], b1[128], W2[128][128],
+ * b2[128], W3[128], b3. Inputs x[B][2], targets t[B]; loss = mean squared
+ * error. imate's own surrogates over t are its interpolators
+ * (imate/interpolator), which this build does not include. */
+int slq_mlp_num_params(void);
+/* Loss and gradient of all parameters at `params` (one forward + backward). */
+int slq_mlp_grad(const float* params, const float* x, const float* target,
+                 int32_t B, float* loss, float* grad);
+/* `steps` full-batch Adam steps (lr, beta1 0.9, beta2 0.999, eps 1e-8) on
+ * the device; params updated in place, losses[steps] when non-NULL. */
+int slq_mlp_train(float* params, const float* x, const float* target,
+                  int32_t B, int32_t steps, float lr, float* losses);
+/* y[Q] = MLP(x[Q][2]); bf16 != 0 runs the 128x128 layer on the matrix cores
+ * in BF16 with FP32 accumulation. *seconds: kernel time. */
+int slq_mlp_predict(const float* params, const float* x, int32_t Q,
+                    int32_t bf16, float* y, double* seconds);
+
 /* ---- component entry points (tests, calibration and benchmarks) ---- */
 
 /* `count` consecutive Rademacher probes of length n, starting at sample
