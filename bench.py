@@ -214,6 +214,128 @@ def batched_form(args):
     print(json.dumps(out), flush=True)
 
 
+def surrogate_form(args):
+    """Learned form. Family: n = 65536 correlation matrices K(l) over the
+    length scale l in [0.01, 0.1] with the support radius fixed at
+    0.01 ln(1/0.03) (one pattern, ~247 non-zeros per row), queries
+    logdet(K(l) + tI), t in [1e-4, 1e2], both log-uniform. The surrogate is
+    trained on 40 x 50 = 2000 GPU-SLQ estimates (lanczos_degree 30, 128
+    probes), calibrated on 10 x 40 validation estimates, judged on 10 x 20
+    held-out queries against SLQ (fallback where it leaves the 95% interval)
+    and against dense Cholesky values recorded for 12 of them, then swept
+    over 10,000 random queries (trust mode: SLQ only where the surrogate is
+    not trusted) and compared with plain FP64 GPU SLQ per query."""
+    from imate_amd.sample_matrices import CorrelationFamily
+    from imate_amd.surrogate import LogdetSurrogate, sample_queries
+    n = 65536
+    support = 0.01 * numpy.log(1.0 / 0.03)
+    t0 = time.perf_counter()
+    fam = CorrelationFamily(n, support, seed=0)
+    log('family n=%d nnz=%d built in %.1fs' % (n, fam.nnz,
+                                               time.perf_counter() - t0))
+    sur = LogdetSurrogate(fam, lanczos_degree=30, num_samples=128, seed=0)
+    out = {'form': 'surrogate', 'n': n, 'nnz': int(fam.nnz),
+           'scale_range': sur.scale_range, 't_range': sur.t_range,
+           'lanczos_degree': 30, 'samples': 128, 'network': '2-128-128-1 SiLU',
+           'train_dtype': 'f32', 'inference_dtype': 'bf16 (MFMA)'}
+    sur.solve([0.03], [1.0])                      # first-use set-up
+    train = sur.generate(40, 50, seed=101)
+    log('training data: 2000 SLQ estimates in %.1fs' % train['seconds'])
+    sec = sur.train(train)
+    out['train'] = {'points': 2000, 'slq_seconds': train['seconds'],
+                    'train_seconds': sec, 'adam_steps': int(len(sur.losses)),
+                    'loss_first': float(sur.losses[0]),
+                    'loss_last': float(sur.losses[-1])}
+    log('train: %s' % json.dumps(out['train']))
+    val = sur.generate(10, 40, seed=202)
+    rho = sur.calibrate(val)
+    out['validation'] = {'points': len(rho),
+                         'inside_ci': float(numpy.mean(numpy.abs(rho) <= 1)),
+                         'rms_rho': float(numpy.sqrt(numpy.mean(rho ** 2)))}
+    # held-out queries, judged by SLQ
+    ls, ts = sample_queries(10, 20, sur.scale_range, sur.t_range, seed=303)
+    hs, ht = numpy.repeat(ls, 20), ts.ravel()
+    y, info = sur.query(hs, ht, judge='slq', seed=303000)
+    slq, ci, pred = info['slq'], info['ci'], info['surrogate']
+    held = {'points': len(ht),
+            'surrogate_inside_ci': float(numpy.mean(numpy.abs(pred - slq) <= ci)),
+            'fallback_fraction': float(info['fallback'].mean()),
+            'median_rel_err_surrogate_vs_slq':
+                float(numpy.median(numpy.abs(pred - slq) / numpy.abs(slq))),
+            'max_rel_err_surrogate_vs_slq':
+                float(numpy.max(numpy.abs(pred - slq) / numpy.abs(slq))),
+            'median_ci_over_value': float(numpy.median(ci / numpy.abs(slq)))}
+    path = os.path.join(HERE, 'tests', 'golden', 'surrogate_cholesky.json')
+    if os.path.exists(path):
+        with open(path) as f:
+            ref = json.load(f)
+        rows = []
+        for e in ref['entries']:
+            i = e['scale_index'] * 20 + e['t_index']
+            exact = e['logdet']
+            rows.append({'scale': e['scale'], 't': e['t'], 'exact': exact,
+                         'rel_err_surrogate': abs(pred[i] - exact) / abs(exact),
+                         'rel_err_slq': abs(slq[i] - exact) / abs(exact),
+                         'rel_err_returned': abs(y[i] - exact) / abs(exact)})
+        held['vs_cholesky'] = {
+            'count': len(rows),
+            'max_rel_err_surrogate': max(r['rel_err_surrogate'] for r in rows),
+            'max_rel_err_slq': max(r['rel_err_slq'] for r in rows),
+            'max_rel_err_returned': max(r['rel_err_returned'] for r in rows),
+            'median_rel_err_surrogate':
+                float(numpy.median([r['rel_err_surrogate'] for r in rows])),
+            'median_rel_err_slq':
+                float(numpy.median([r['rel_err_slq'] for r in rows]))}
+    out['held_out'] = held
+    log('held-out: %s' % json.dumps(held))
+    # plain FP64 GPU SLQ per query: new l (values refreshed in place) + SLQ
+    rng = numpy.random.default_rng(404)
+    Q = 10000
+    qs = numpy.exp(rng.uniform(numpy.log(0.01), numpy.log(0.1), Q))
+    qt = numpy.exp(rng.uniform(numpy.log(1e-4), numpy.log(1e2), Q))
+    k = 16
+    _, _, plain_sec = sur.solve(qs[:k], qt[:k], seed=505)
+    plain_per_query = plain_sec / k
+    plain_split = dict(sur.last_solve)
+    # surrogate sweep: BF16 inference of all queries, trust check, SLQ for
+    # the untrusted ones (executed up to 64, the rest charged at the plain
+    # per-query time)
+    sur.predict(qs[:64], qt[:64])                 # warm-up
+    t0 = time.perf_counter()
+    pred_all, ksec = sur.predict(qs, qt, bf16=True)
+    t_pred = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    ok = sur.trusted(qs, qt)
+    t_trust = time.perf_counter() - t0
+    fb = numpy.nonzero(~ok)[0]
+    run = fb[:64]
+    fb_sec = 0.0
+    if len(run):
+        _, _, fb_sec = sur.solve(qs[run], qt[run], seed=606)
+    fb_total = fb_sec + (len(fb) - len(run)) * plain_per_query
+    p32, _ = sur.predict(qs, qt, bf16=False)
+    total = t_pred + t_trust + fb_total
+    out['sweep'] = {
+        'queries': Q, 'inference_kernel_seconds': ksec,
+        'inference_call_seconds': t_pred,
+        'inference_queries_per_s_kernel': Q / ksec,
+        'trust_seconds': t_trust, 'fallback_queries': int(len(fb)),
+        'fallback_fraction': float(len(fb) / Q),
+        'fallback_seconds_run': fb_sec, 'fallback_run': int(len(run)),
+        'fallback_seconds_total': fb_total,
+        'surrogate_queries_per_s': Q / total,
+        'plain_slq_seconds_per_query': plain_per_query,
+        'plain_slq_queries_per_s': 1.0 / plain_per_query,
+        'plain_slq_values_seconds_per_query':
+            plain_split['values_seconds'] / k,
+        'plain_slq_gpu_seconds_per_query': plain_split['slq_seconds'] / k,
+        'gain_vs_plain_slq': plain_per_query * Q / total,
+        'bf16_vs_f32_max_rel_diff':
+            float(numpy.max(numpy.abs(pred_all - p32) / numpy.abs(p32)))}
+    log('sweep: %s' % json.dumps(out['sweep']))
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -239,6 +361,8 @@ def main():
         return precision_form(args)
     if args.form == 'batched':
         return batched_form(args)
+    if args.form == 'surrogate':
+        return surrogate_form(args)
 
     from imate_amd.sample_matrices import sparse_correlation_matrix
     n, k, m, s = args.n, args.nnz_per_row, args.lanczos_degree, args.samples

@@ -45,6 +45,7 @@ SIGNATURES = {
                                          ctypes.c_int, _P]),
     'slq_op_create_affine': (ctypes.c_int, [_P, _P, _P]),
     'slq_op_destroy': (ctypes.c_int, [_P]),
+    'slq_op_set_values': (ctypes.c_int, [_P, _P]),
     'slq_estimate': (ctypes.c_int, [
         _P, _P, _i32, _i32, _i32, _f64, _i32, _i64, _i32, _f64, _i32, _i32,
         _f64, _f64, _f64, _f64, _i32, _i32, _P, _P, _P, _P, _P, _P, _P, _P,

@@ -65,6 +65,7 @@ static void free_tiles(DevTiles& t)
     if (t.cols) (void) hipFree(t.cols);
     if (t.vals) (void) hipFree(t.vals);
     if (t.vals_f32) (void) hipFree(t.vals_f32);
+    if (t.src) (void) hipFree(t.src);
     t = DevTiles();
 }
 
@@ -132,6 +133,8 @@ int build_tiles(int64_t nrows, int64_t ncols, const int64_t* indptr,
     if (density < min_density) return SLQ_OK;
     const size_t vb = dtype == SLQ_F64 ? 8 : 4;
     std::vector<char> vals((size_t) U * R * vb, 0);
+    bool keep_src = nnz <= (int64_t) INT32_MAX;
+    std::vector<int32_t> src(keep_src ? (size_t) U * R : 0, -1);
     for (int64_t t = 0; t < T; ++t)
     {
         const int32_t* cb = cols.data() + tptr[t];
@@ -149,6 +152,11 @@ int build_tiles(int64_t nrows, int64_t ncols, const int64_t* indptr,
                     ((double*) vals.data())[slot] += ((const double*) data)[k];
                 else
                     ((float*) vals.data())[slot] += ((const float*) data)[k];
+                if (keep_src)
+                {
+                    if (src[slot] >= 0) keep_src = false;   // duplicate entry
+                    else src[slot] = (int32_t) k;
+                }
             }
         }
     }
@@ -160,11 +168,14 @@ int build_tiles(int64_t nrows, int64_t ncols, const int64_t* indptr,
     if ((e = hipMalloc(&d.ptr, sizeof(int64_t) * (T + 1))) != hipSuccess ||
         (e = hipMalloc(&d.cols, sizeof(int32_t) * U)) != hipSuccess ||
         (e = hipMalloc(&d.vals, vals.size())) != hipSuccess ||
+        (keep_src && (e = hipMalloc(&d.src, sizeof(int32_t) * src.size())) != hipSuccess) ||
         (e = hipMemcpy(d.ptr, tptr.data(), sizeof(int64_t) * (T + 1),
                        hipMemcpyHostToDevice)) != hipSuccess ||
         (e = hipMemcpy(d.cols, cols.data(), sizeof(int32_t) * U,
                        hipMemcpyHostToDevice)) != hipSuccess ||
-        (e = hipMemcpy(d.vals, vals.data(), vals.size(), hipMemcpyHostToDevice)) != hipSuccess)
+        (e = hipMemcpy(d.vals, vals.data(), vals.size(), hipMemcpyHostToDevice)) != hipSuccess ||
+        (keep_src && (e = hipMemcpy(d.src, src.data(), sizeof(int32_t) * src.size(),
+                                    hipMemcpyHostToDevice)) != hipSuccess))
     {
         free_tiles(d);
         return hip_status(e, "tile upload", __FILE__, __LINE__);
@@ -1139,6 +1150,45 @@ int slq_op_create_affine(slq_op* A, slq_op* B, slq_op** out)
         return hip_status(e, "hipStreamCreate", __FILE__, __LINE__);
     }
     *out = op;
+    return SLQ_OK;
+}
+
+int slq_op_set_values(slq_op* op, const void* data)
+{
+    if (!op || op->kind != OP_MATRIX || (!data && op->A->csr.nnz > 0))
+    {
+        set_error("slq_op_set_values: a matrix handle and its new values are required");
+        return SLQ_E_ARG;
+    }
+    DeviceGuard guard(op->device);
+    MatrixStore& M = *op->A;
+    if (M.tiles.R > 0 && !M.tiles.src)
+    {
+        set_error("slq_op_set_values: the matrix has duplicate entries or too many "
+                  "non-zeros for an in-place value refresh; create a new operator");
+        return SLQ_E_ARG;
+    }
+    SLQ_CHECK(hipDeviceSynchronize());
+    const size_t vb = M.csr.dtype == SLQ_F64 ? 8 : 4;
+    if (M.csr.nnz > 0)
+        SLQ_CHECK(hipMemcpy(M.csr.data, data, vb * M.csr.nnz, hipMemcpyHostToDevice));
+    SLQ_TRY(tiles_refill(op->stream, M.tiles, M.csr.data, M.csr.dtype));
+    // derived forms are rebuilt from the new values on their next use
+    if (M.data_f32) (void) hipFree(M.data_f32);
+    M.data_f32 = nullptr;
+    if (M.trans_data_f32) (void) hipFree(M.trans_data_f32);
+    M.trans_data_f32 = nullptr;
+    if (M.has_trans)
+    {
+        (void) hipFree(M.trans.indptr);
+        (void) hipFree(M.trans.indices);
+        (void) hipFree(M.trans.data);
+        M.trans = DevCSR();
+        M.has_trans = false;
+    }
+    free_tiles(M.trans_tiles);
+    M.trans_tiles_tried = false;
+    SLQ_CHECK(hipStreamSynchronize(op->stream));
     return SLQ_OK;
 }
 

@@ -1267,6 +1267,46 @@ int run_spmm(slq_op* op, double t, int transpose, const void* X, void* Y,
     return launch_spmm<double>(op, s, io, tiles_for<double>(op, transpose != 0));
 }
 
+template <typename T>
+__global__ void tiles_refill_kernel(const int32_t* __restrict__ src,
+                                    const T* __restrict__ data, T* __restrict__ vals,
+                                    int64_t n)
+{
+    const int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+    {
+        const int32_t k = src[i];
+        vals[i] = k >= 0 ? data[k] : T(0);
+    }
+}
+
+int tiles_refill(hipStream_t s, DevTiles& t, const void* csr_data, int dtype)
+{
+    if (t.R == 0) return SLQ_OK;
+    if (!t.src)
+    {
+        set_error("the matrix has duplicate entries or too many non-zeros for an "
+                  "in-place value refresh; create a new operator instead");
+        return SLQ_E_ARG;
+    }
+    const int64_t n = t.nslabs * t.R;
+    const int64_t g = (n + 255) / 256;
+    if (dtype == SLQ_F64)
+        hipLaunchKernelGGL((tiles_refill_kernel<double>), dim3((unsigned) g), dim3(256), 0, s,
+                           t.src, (const double*) csr_data, (double*) t.vals, n);
+    else
+        hipLaunchKernelGGL((tiles_refill_kernel<float>), dim3((unsigned) g), dim3(256), 0, s,
+                           t.src, (const float*) csr_data, (float*) t.vals, n);
+    SLQ_CHECK(hipGetLastError());
+    if (t.vals_f32)
+    {
+        (void) hipStreamSynchronize(s);
+        (void) hipFree(t.vals_f32);
+        t.vals_f32 = nullptr;
+    }
+    return SLQ_OK;
+}
+
 int tiles_f32(hipStream_t s, DevTiles& t, int dtype)
 {
     if (dtype != SLQ_F64 || t.R == 0 || t.vals_f32) return SLQ_OK;

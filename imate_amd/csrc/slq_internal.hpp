@@ -78,6 +78,10 @@ struct DevTiles {
     int32_t* cols = nullptr;   // [nslabs]
     void* vals = nullptr;      // [nslabs * R], dtype of the matrix
     float* vals_f32 = nullptr; // FP32 copy (mixed precision)
+    // [nslabs * R] position in the CSR value array of each slot (-1: zero
+    // fill); null when the CSR has duplicate entries or more than 2^31-1
+    // non-zeros. Lets new values of the same pattern be scattered in place.
+    int32_t* src = nullptr;
 };
 
 // Device copy of a matrix plus derived forms built on demand: the transpose
@@ -101,6 +105,8 @@ int build_tiles(int64_t nrows, int64_t ncols, const int64_t* indptr,
                 const int32_t* indices, const void* data, int dtype, int R,
                 double min_density, DevTiles& out);
 int tiles_f32(hipStream_t s, DevTiles& t, int dtype);
+// Refill t.vals from new CSR values through t.src.
+int tiles_refill(hipStream_t s, DevTiles& t, const void* csr_data, int dtype);
 
 // Device buffers reused across calls on one handle.
 struct Workspace {

@@ -60,6 +60,11 @@ class DeviceOperator(object):
             ctypes.byref(h)), 'slq_op_create_affine')
         return DeviceOperator(h, parents=(A_dev, B_dev))
 
+    def set_values(self, data):
+        lib = _lib.load()
+        _lib.check(lib.slq_op_set_values(self.handle, _lib.ptr(data)),
+                   'slq_op_set_values')
+
     def get_device_properties(self):
         return _lib.device_properties()
 
@@ -153,6 +158,18 @@ class LinearOperator(object):
         return A.shape[0] * A.shape[1]
 
 
+def _new_values(A, data):
+    """Copy of the CSR matrix A with its stored values replaced by data."""
+    if not (scipy.sparse.isspmatrix_csr(A) and A.has_sorted_indices):
+        raise ValueError('set_values needs A as a CSR matrix with sorted '
+                         'indices.')
+    data = numpy.ascontiguousarray(data, dtype=A.dtype)
+    if data.shape != (A.nnz,):
+        raise ValueError('data should hold the %d stored values of A.' % A.nnz)
+    return scipy.sparse.csr_matrix((data, A.indices, A.indptr),
+                                   shape=A.shape), data
+
+
 class Matrix(LinearOperator):
     """A matrix as a linear operator (imate/linear_operator/matrix.py)."""
 
@@ -171,6 +188,15 @@ class Matrix(LinearOperator):
             self._initialize_cpu()
         self.gpu = gpu
         self.num_gpu_devices = num_gpu_devices
+
+    def set_values(self, data):
+        """Replace the stored values of A (same sparsity pattern, CSR order)
+        in place, also on the device, without re-uploading the pattern or
+        rebuilding the tiled layout: for families of matrices that share a
+        pattern. No counterpart in imate, which builds a new Matrix."""
+        self.A, data = _new_values(self.A, data)
+        if self.initialized_on_gpu:
+            self.gpu_Aop.set_values(data)
 
     def get_num_rows(self):
         return self.A.shape[0]
@@ -220,6 +246,13 @@ class AffineMatrixFunction(LinearOperator):
             self._initialize_cpu()
         self.gpu = gpu
         self.num_gpu_devices = num_gpu_devices
+
+    def set_values(self, data):
+        """Replace the stored values of A (not B) in place; see
+        ``Matrix.set_values``."""
+        self.A, data = _new_values(self.A, data)
+        if self.initialized_on_gpu:
+            self._A_dev.set_values(data)
 
     def get_num_rows(self):
         return self.A.shape[0]

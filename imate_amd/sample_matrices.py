@@ -21,6 +21,10 @@
   curve, so that neighbouring rows share columns — the property that keeps
   the SpMM's gathers in cache. The length scale for about k non-zeros per row
   is l = sqrt(k / (pi n)) / ln(1/tau).
+* ``CorrelationFamily(n, support, ...)`` — the matrices above over the
+  length scale l with the support radius R held fixed, so that every member
+  shares one sparsity pattern (K_ij = exp(-d_ij / l) w(d_ij / R), still
+  positive definite) and a new l only needs new values.
 * ``sparse_gaussian(nrows, ncols, density, seed)`` — rows with
   round(density * ncols) distinct uniform columns and N(0, 1) values (the
   rectangular workload of gram=True).
@@ -32,7 +36,8 @@ from scipy.sparse import coo_matrix
 
 __all__ = ['toeplitz', 'toeplitz_trace', 'toeplitz_traceinv',
            'toeplitz_logdet', 'toeplitz_schatten', 'correlation_scale',
-           'sparse_correlation_matrix', 'sparse_gaussian', 'morton_order']
+           'sparse_correlation_matrix', 'CorrelationFamily',
+           'sparse_gaussian', 'morton_order']
 
 
 # ---------------------------------------------------------------- Toeplitz
@@ -145,6 +150,61 @@ def sparse_correlation_matrix(n, scale=None, nnz_per_row=40, threshold=0.03,
     A.sum_duplicates()
     A.sort_indices()
     return A
+
+
+class CorrelationFamily(object):
+    """Correlation matrices over the length scale l with one fixed pattern.
+
+    Points as in ``sparse_correlation_matrix`` (seeded, Morton ordered);
+    pairs within the support radius ``support`` are the pattern, and
+
+        K_ij(l) = exp(-d_ij / l) * w(d_ij / support),  K_ii = 1 + jitter,
+
+    with the Wendland factor w, so every member is positive definite.
+    ``matrix(l)`` returns the CSR matrix (sorted indices, no duplicates);
+    ``values(l)`` only its value array, in the same order.
+    """
+
+    def __init__(self, n, support, jitter=1e-3, seed=0, dtype='float64'):
+        from scipy.spatial import cKDTree
+        rng = numpy.random.default_rng(seed)
+        pts = rng.random((n, 2))
+        pts = pts[morton_order(pts)]
+        pairs = cKDTree(pts).query_pairs(support, output_type='ndarray')
+        i, j = pairs[:, 0], pairs[:, 1]
+        d = numpy.sqrt(((pts[i] - pts[j])**2).sum(axis=1))
+        rows = numpy.concatenate([i, j, numpy.arange(n)])
+        cols = numpy.concatenate([j, i, numpy.arange(n)])
+        dist = numpy.concatenate([d, d, numpy.full(n, -1.0)])
+        order = numpy.lexsort((cols, rows))
+        self.n = n
+        self.support = float(support)
+        self.jitter = float(jitter)
+        self.dtype = numpy.dtype(dtype)
+        self.indices = cols[order].astype(numpy.int32)
+        self.indptr = numpy.zeros(n + 1, dtype=numpy.int64)
+        numpy.cumsum(numpy.bincount(rows, minlength=n), out=self.indptr[1:])
+        dist = dist[order]
+        self._diag = dist < 0
+        self._d = numpy.where(self._diag, 0.0, dist)
+        r = self._d / self.support
+        self._w = (1.0 - r)**4 * (4.0 * r + 1.0)
+        self.nnz = len(self.indices)
+
+    def values(self, scale):
+        v = numpy.exp(-self._d / scale) * self._w
+        v[self._diag] = 1.0 + self.jitter
+        return v.astype(self.dtype)
+
+    def matrix(self, scale):
+        return scipy.sparse.csr_matrix(
+            (self.values(scale), self.indices, self.indptr),
+            shape=(self.n, self.n))
+
+    def frobenius_squared(self, scale):
+        """tr(K(l)^2): the sum of the squared entries."""
+        v = self.values(scale).astype(numpy.float64)
+        return float(numpy.dot(v, v))
 
 
 def sparse_gaussian(nrows, ncols, density, seed=0, dtype='float64'):

@@ -79,6 +79,16 @@ int slq_op_create_csr(int64_t nrows, int64_t ncols, int64_t nnz,
  * cu_csr_affine_matrix_function.cu:29-211). */
 int slq_op_create_affine(slq_op* A, slq_op* B, slq_op** out);
 
+/* Replace the values of a matrix handle, keeping its sparsity pattern:
+ * data[nnz] in the handle's dtype and CSR order of slq_op_create_csr. The
+ * tiled layout is refilled on the device; the transpose and FP32 copies are
+ * rebuilt on next use. Affine handles made from this matrix see the new
+ * values. Fails with SLQ_E_ARG when the CSR had duplicate entries.
+ * No counterpart in the reference (imate builds a new pycuMatrix for new
+ * data, py_cu_matrix.pyx:118-200): it serves families of matrices with one
+ * pattern, such as a compactly supported kernel over its length scale. */
+int slq_op_set_values(slq_op* op, const void* data);
+
 /* Free a handle (pycuMatrix.__dealloc__, cu_csr_matrix.cu:94-154). */
 int slq_op_destroy(slq_op* op);
 

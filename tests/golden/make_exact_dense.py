@@ -24,6 +24,9 @@ import numpy
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
 from imate_amd.sample_matrices import sparse_correlation_matrix  # noqa: E402
+sys.path.insert(0, HERE)
+from dense_cholesky import (dense_from_csr, cholesky_in_place,  # noqa: E402
+                            logdet_from_factor)
 
 
 def main():
@@ -35,27 +38,12 @@ def main():
     print('built %.1fs nnz/row %.1f' % (time.time() - t0, A.nnz / n),
           file=sys.stderr, flush=True)
     dev = torch.device('cuda:0')
-    coo = A.tocoo()
-    D = torch.zeros((n, n), dtype=torch.float64, device=dev)
-    D[torch.from_numpy(coo.row).to(dev), torch.from_numpy(coo.col).to(dev)] = \
-        torch.from_numpy(coo.data).to(dev)
+    D = dense_from_csr(A, dev)
     t0 = time.time()
     nb = 8192
-    for k in range(0, n, nb):
-        e = min(n, k + nb)
-        Lkk = torch.linalg.cholesky(D[k:e, k:e])
-        D[k:e, k:e] = Lkk
-        if e < n:
-            # L21 = A21 Lkk^-T, then the trailing update A22 -= L21 L21^T
-            L21 = torch.linalg.solve_triangular(Lkk, D[e:, k:e].T.contiguous(),
-                                                upper=False).T.contiguous()
-            D[e:, k:e] = L21
-            D[e:, e:].addmm_(L21, L21.T.contiguous(), alpha=-1.0)
-            del L21
-        del Lkk
-    torch.cuda.synchronize()
+    cholesky_in_place(D, nb)
     print('cholesky %.1fs' % (time.time() - t0), file=sys.stderr, flush=True)
-    logdet = float(2.0 * torch.log(torch.diagonal(D)).sum())
+    logdet = logdet_from_factor(D)
     # tr(A^-1) = |L^-1|_F^2 by blocked forward substitution, one block of
     # identity columns at a time (rows above the block are zero)
     t0 = time.time()
