@@ -425,6 +425,11 @@ struct UpdateArgs {
     int64_t nrows;
     const double* alpha;   // [64]
     const double* beta;    // [64] beta_{j-1} (nullptr at j = 0)
+    // per-probe scales of Vj / Vjm1 (nullptr = 1): v = scale * V, rounded
+    // in T exactly as the SpMM epilogue rounds the basis vector it stores,
+    // so reading an unnormalised residual with its scale gives the same bits
+    const double* vscale;
+    const double* uscale;
     int do_norm;           // 1: reduce |r|^2 and finish beta_j
     RedBuf rb;
     double* beta_out;
@@ -459,6 +464,10 @@ __global__ __launch_bounds__(kThreads) void lanczos_update_kernel(
     const int p0 = 2 * (lane & 31);
     const double al0 = a.alpha[p0], al1 = a.alpha[p0 + 1];
     const double be0 = Vjm1 ? a.beta[p0] : 0.0, be1 = Vjm1 ? a.beta[p0 + 1] : 0.0;
+    const T sv0 = a.vscale ? (T) a.vscale[p0] : T(1);
+    const T sv1 = a.vscale ? (T) a.vscale[p0 + 1] : T(1);
+    const T su0 = (Vjm1 && a.uscale) ? (T) a.uscale[p0] : T(1);
+    const T su1 = (Vjm1 && a.uscale) ? (T) a.uscale[p0 + 1] : T(1);
     double red0 = 0.0, red1 = 0.0;
     V2* __restrict__ R2 = reinterpret_cast<V2*>(R);
     const V2* __restrict__ Vj2 = reinterpret_cast<const V2*>(Vj);
@@ -487,12 +496,12 @@ __global__ __launch_bounds__(kThreads) void lanczos_update_kernel(
         }
         if (oka)
         {
-            double r0 = (double) wa.x - al0 * (double) va.x;
-            double r1 = (double) wa.y - al1 * (double) va.y;
+            double r0 = (double) wa.x - al0 * (double) (sv0 * va.x);
+            double r1 = (double) wa.y - al1 * (double) (sv1 * va.y);
             if (Vjm1)
             {
-                r0 = r0 - be0 * (double) ua.x;
-                r1 = r1 - be1 * (double) ua.y;
+                r0 = r0 - be0 * (double) (su0 * ua.x);
+                r1 = r1 - be1 * (double) (su1 * ua.y);
             }
             V2 o;
             o.x = (T) r0;
@@ -503,12 +512,12 @@ __global__ __launch_bounds__(kThreads) void lanczos_update_kernel(
         }
         if (okb)
         {
-            double r0 = (double) wb.x - al0 * (double) vb.x;
-            double r1 = (double) wb.y - al1 * (double) vb.y;
+            double r0 = (double) wb.x - al0 * (double) (sv0 * vb.x);
+            double r1 = (double) wb.y - al1 * (double) (sv1 * vb.y);
             if (Vjm1)
             {
-                r0 = r0 - be0 * (double) ub.x;
-                r1 = r1 - be1 * (double) ub.y;
+                r0 = r0 - be0 * (double) (su0 * ub.x);
+                r1 = r1 - be1 * (double) (su1 * ub.y);
             }
             V2 o;
             o.x = (T) r0;
@@ -952,6 +961,77 @@ int normalize_in_place(slq_op* op, T* X, int64_t n, double* scale)
     return SLQ_OK;
 }
 
+// Lanczos without re-orthogonalisation: the basis is never stored. Step j
+// gathers X_j = r_{j-1} (the previous residual, unnormalised) with its
+// per-probe scale s_j = 1 / beta_{j-1}; the update reads v_j = s_j X_j and
+// v_{j-1} = s_{j-1} X_{j-1} from the two previous residual buffers of a ring
+// of three, rounding the products exactly as a stored basis would be
+// rounded. One block write per step less than storing v_j.
+template <typename T>
+int lanczos_ring(const RecurrenceArgs& ra, Bump& bump)
+{
+    slq_op* op = ra.op;
+    MatrixStore& A = *op->A;
+    const int64_t n = A.csr.nrows;
+    const int m = ra.m;
+    const bool f32 = (sizeof(T) == 4);
+    const size_t blk = (size_t) n * kBlock;
+    T* B[3] = {bump.take<T>(blk), bump.take<T>(blk), bump.take<T>(blk)};
+    double* S[3] = {bump.take<double>(kBlock), bump.take<double>(kBlock),
+                    bump.take<double>(kBlock)};
+    const RedBuf rb = take_redbuf(bump);
+    if (!B[0] || !B[1] || !B[2] || !S[0] || !S[1] || !S[2] || !redbuf_ok(rb))
+    {
+        set_error("internal: Lanczos workspace too small");
+        return SLQ_E_OOM;
+    }
+    zero_counters(op->stream, rb);
+    const double s0 = 1.0 / std::sqrt((double) n);
+    hipLaunchKernelGGL(init_block_kernel, dim3(1), dim3(kBlock), 0, op->stream,
+                       S[0], ra.sizes, kBlock, s0);
+    SLQ_CHECK(hipGetLastError());
+    const double tol_sqrt_n = ra.tol * std::sqrt((double) n);
+    const DevTiles* tl = tiles_for<T>(op, false);
+    const T* X0 = (const T*) ra.X0;
+    for (int j = 0; j < m; ++j)
+    {
+        T* W = B[j % 3];
+        const T* Xj = j == 0 ? X0 : B[(j + 2) % 3];
+        const T* Xjm1 = j == 0 ? nullptr : (j == 1 ? X0 : B[(j + 1) % 3]);
+
+        SpmmArgs<T> s = spmm_defaults<T>();
+        fill_matrix<T>(s, A, op->kind == OP_AFFINE_B ? op->B.get() : nullptr, false, f32);
+        s.shift_mode = op->kind == OP_AFFINE_I ? 1 : (op->kind == OP_AFFINE_B ? 2 : 0);
+        s.shift_len = A.csr.nrows < A.csr.ncols ? A.csr.nrows : A.csr.ncols;
+        s.t = ra.t;
+        s.xscale = S[j % 3];
+        s.red_mode = 1;
+        s.rb = rb;
+        s.red_out = ra.alpha + (size_t) j * kBlock;
+        SpmmIO<T> io;
+        io.X = Xj;
+        io.Y = W;
+        SLQ_TRY(launch_spmm<T>(op, s, io, tl));
+
+        UpdateArgs u;
+        std::memset(&u, 0, sizeof(u));
+        u.nrows = n;
+        u.alpha = ra.alpha + (size_t) j * kBlock;
+        u.beta = j > 0 ? ra.beta + (size_t) (j - 1) * kBlock : nullptr;
+        u.vscale = S[j % 3];
+        u.uscale = j > 0 ? S[(j + 2) % 3] : nullptr;
+        u.do_norm = 1;
+        u.rb = rb;
+        u.beta_out = ra.beta + (size_t) j * kBlock;
+        u.scale_out = S[(j + 1) % 3];
+        u.sizes = ra.sizes;
+        u.j = j;
+        u.tol_sqrt_n = tol_sqrt_n;
+        SLQ_TRY(launch_update<T>(op, u, W, Xj, Xjm1));
+    }
+    return SLQ_OK;
+}
+
 template <typename T>
 int lanczos(const RecurrenceArgs& ra, Bump& bump)
 {
@@ -968,6 +1048,7 @@ int lanczos(const RecurrenceArgs& ra, Bump& bump)
     const int nslots = reorth ? m : 2;
     const size_t blk = (size_t) n * kBlock;
 
+    if (!reorth) return lanczos_ring<T>(ra, bump);
     T* P = bump.take<T>(blk);
     T* Q = bump.take<T>(blk);
     T* V = bump.take<T>(blk * nslots);
@@ -1199,10 +1280,18 @@ size_t recurrence_scratch_bytes(const slq_op* op, int gram, int m, int ortho,
     if (!gram)
     {
         const size_t blk = (size_t) nr * kBlock * value_bytes;
-        add(blk);
-        add(blk);
-        add(blk * (reorth ? m : 2));
-        add(sizeof(double) * kBlock);
+        if (reorth)
+        {
+            add(blk);
+            add(blk);
+            add(blk * m);
+            add(sizeof(double) * kBlock);
+        }
+        else
+        {
+            for (int k = 0; k < 3; ++k) add(blk);
+            for (int k = 0; k < 3; ++k) add(sizeof(double) * kBlock);
+        }
     }
     else
     {

@@ -96,59 +96,89 @@ __global__ __launch_bounds__(H) void mlp_backward_kernel(
     d1[(size_t) b * H + j] = acc * dsilu(z1[(size_t) b * H + j]);
 }
 
-// Gradient of every parameter, one thread per parameter, batch sums in a
-// fixed order (deterministic); thread 0 also writes the loss.
+// Gradient of every parameter: one thread per (parameter, batch chunk),
+// each summing its chunk of samples in order into partial[chunk][p]; the
+// chunks are then added in order by mlp_reduce_kernel (deterministic, and
+// the batch loop is split over blockIdx.y so that a small network still
+// fills the GPU). The last parameter's thread of each chunk also writes the
+// chunk's sum of squared errors.
 __global__ __launch_bounds__(256) void mlp_grad_kernel(
-    const float* __restrict__ x, const float* __restrict__ target, int B,
+    const float* __restrict__ x, const float* __restrict__ target, int B, int chunk,
     const float* __restrict__ z1, const float* __restrict__ z2,
     const float* __restrict__ y, const float* __restrict__ d1,
     const float* __restrict__ d2, const float* __restrict__ d3,
-    float* __restrict__ grad, float* __restrict__ loss_out)
+    float* __restrict__ partial, float* __restrict__ loss_partial)
 {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= kMlpParams) return;
+    const int c = blockIdx.y;
+    const int b0 = c * chunk;
+    const int b1 = b0 + chunk < B ? b0 + chunk : B;
     float g = 0.0f;
     if (p < kB1)
     {
         const int j = p / 2, i = p % 2;
-        for (int b = 0; b < B; ++b) g += d1[(size_t) b * H + j] * x[2 * b + i];
+        for (int b = b0; b < b1; ++b) g += d1[(size_t) b * H + j] * x[2 * b + i];
     }
     else if (p < kW2)
     {
         const int j = p - kB1;
-        for (int b = 0; b < B; ++b) g += d1[(size_t) b * H + j];
+        for (int b = b0; b < b1; ++b) g += d1[(size_t) b * H + j];
     }
     else if (p < kB2)
     {
         const int j = (p - kW2) / H, k = (p - kW2) % H;
-        for (int b = 0; b < B; ++b)
+        for (int b = b0; b < b1; ++b)
             g += d2[(size_t) b * H + j] * silu(z1[(size_t) b * H + k]);
     }
     else if (p < kW3)
     {
         const int j = p - kB2;
-        for (int b = 0; b < B; ++b) g += d2[(size_t) b * H + j];
+        for (int b = b0; b < b1; ++b) g += d2[(size_t) b * H + j];
     }
     else if (p < kB3)
     {
         const int j = p - kW3;
-        for (int b = 0; b < B; ++b) g += d3[b] * silu(z2[(size_t) b * H + j]);
+        for (int b = b0; b < b1; ++b) g += d3[b] * silu(z2[(size_t) b * H + j]);
     }
     else
     {
-        for (int b = 0; b < B; ++b) g += d3[b];
-        if (loss_out)
+        for (int b = b0; b < b1; ++b) g += d3[b];
+        float l = 0.0f;
+        for (int b = b0; b < b1; ++b)
         {
-            float l = 0.0f;
-            for (int b = 0; b < B; ++b)
-            {
-                const float e = y[b] - target[b];
-                l += e * e;
-            }
-            *loss_out = l / (float) B;
+            const float e = y[b] - target[b];
+            l += e * e;
         }
+        loss_partial[c] = l;
     }
+    partial[(size_t) c * kMlpParams + p] = g;
+}
+
+__global__ __launch_bounds__(256) void mlp_reduce_kernel(const float* __restrict__ partial,
+                                                         const float* __restrict__ loss_partial,
+                                                         int chunks, int B,
+                                                         float* __restrict__ grad,
+                                                         float* __restrict__ loss_out)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= kMlpParams) return;
+    float g = 0.0f;
+    for (int c = 0; c < chunks; ++c) g += partial[(size_t) c * kMlpParams + p];
     grad[p] = g;
+    if (p == kMlpParams - 1 && loss_out)
+    {
+        float l = 0.0f;
+        for (int c = 0; c < chunks; ++c) l += loss_partial[c];
+        *loss_out = l / (float) B;
+    }
+}
+
+// Batch chunks of the gradient kernel: about 64 samples each, at most 32.
+inline int grad_chunks(int B)
+{
+    const int c = (B + 63) / 64;
+    return c < 32 ? c : 32;
 }
 
 // Adam (Kingma & Ba 2015) with bias correction; the step counter lives in
@@ -333,7 +363,11 @@ int slq_mlp_grad(const float* params, const float* x, const float* target, int32
         set_error("slq_mlp_grad: invalid argument");
         return SLQ_E_ARG;
     }
-    DevBuf dp, dx, dt, dz1, dz2, dy, dd1, dd2, dd3, dg, dl;
+    DevBuf dp, dx, dt, dz1, dz2, dy, dd1, dd2, dd3, dg, dl, dpart, dlpart;
+    const int C = grad_chunks(B);
+    const int chunk = (B + C - 1) / C;
+    SLQ_CHECK(hipMalloc(&dpart.p, sizeof(float) * kMlpParams * C));
+    SLQ_CHECK(hipMalloc(&dlpart.p, sizeof(float) * C));
     SLQ_CHECK(hipMalloc(&dp.p, sizeof(float) * kMlpParams));
     SLQ_CHECK(hipMalloc(&dx.p, sizeof(float) * 2 * B));
     SLQ_CHECK(hipMalloc(&dt.p, sizeof(float) * B));
@@ -353,10 +387,13 @@ int slq_mlp_grad(const float* params, const float* x, const float* target, int32
     hipLaunchKernelGGL(mlp_backward_kernel, dim3(B), dim3(H), 0, 0, (const float*) dp.p,
                        (const float*) dt.p, B, (const float*) dz1.p, (const float*) dz2.p,
                        (const float*) dy.p, (float*) dd1.p, (float*) dd2.p, (float*) dd3.p);
-    hipLaunchKernelGGL(mlp_grad_kernel, dim3((kMlpParams + 255) / 256), dim3(256), 0, 0,
-                       (const float*) dx.p, (const float*) dt.p, B, (const float*) dz1.p,
-                       (const float*) dz2.p, (const float*) dy.p, (const float*) dd1.p,
-                       (const float*) dd2.p, (const float*) dd3.p, (float*) dg.p,
+    hipLaunchKernelGGL(mlp_grad_kernel, dim3((kMlpParams + 255) / 256, C), dim3(256), 0, 0,
+                       (const float*) dx.p, (const float*) dt.p, B, chunk,
+                       (const float*) dz1.p, (const float*) dz2.p, (const float*) dy.p,
+                       (const float*) dd1.p, (const float*) dd2.p, (const float*) dd3.p,
+                       (float*) dpart.p, (float*) dlpart.p);
+    hipLaunchKernelGGL(mlp_reduce_kernel, dim3((kMlpParams + 255) / 256), dim3(256), 0, 0,
+                       (const float*) dpart.p, (const float*) dlpart.p, C, B, (float*) dg.p,
                        (float*) dl.p);
     SLQ_CHECK(hipGetLastError());
     SLQ_CHECK(hipDeviceSynchronize());
@@ -373,7 +410,12 @@ int slq_mlp_train(float* params, const float* x, const float* target, int32_t B,
         set_error("slq_mlp_train: invalid argument");
         return SLQ_E_ARG;
     }
-    DevBuf dp, dx, dt, dz1, dz2, dy, dd1, dd2, dd3, dg, dl, dm1, dm2, dstep, dlosses;
+    DevBuf dp, dx, dt, dz1, dz2, dy, dd1, dd2, dd3, dg, dl, dm1, dm2, dstep, dlosses, dpart,
+        dlpart;
+    const int C = grad_chunks(B);
+    const int chunk = (B + C - 1) / C;
+    SLQ_CHECK(hipMalloc(&dpart.p, sizeof(float) * kMlpParams * C));
+    SLQ_CHECK(hipMalloc(&dlpart.p, sizeof(float) * C));
     SLQ_CHECK(hipMalloc(&dp.p, sizeof(float) * kMlpParams));
     SLQ_CHECK(hipMalloc(&dx.p, sizeof(float) * 2 * B));
     SLQ_CHECK(hipMalloc(&dt.p, sizeof(float) * B));
@@ -412,11 +454,14 @@ int slq_mlp_train(float* params, const float* x, const float* target, int32_t B,
                            (const float*) dt.p, B, (const float*) dz1.p, (const float*) dz2.p,
                            (const float*) dy.p, (float*) dd1.p, (float*) dd2.p,
                            (float*) dd3.p);
-        hipLaunchKernelGGL(mlp_grad_kernel, dim3(pg), dim3(256), 0, s, (const float*) dx.p,
-                           (const float*) dt.p, B, (const float*) dz1.p,
+        hipLaunchKernelGGL(mlp_grad_kernel, dim3(pg, C), dim3(256), 0, s, (const float*) dx.p,
+                           (const float*) dt.p, B, chunk, (const float*) dz1.p,
                            (const float*) dz2.p, (const float*) dy.p, (const float*) dd1.p,
-                           (const float*) dd2.p, (const float*) dd3.p, (float*) dg.p,
-                           (float*) dl.p);
+                           (const float*) dd2.p, (const float*) dd3.p, (float*) dpart.p,
+                           (float*) dlpart.p);
+        hipLaunchKernelGGL(mlp_reduce_kernel, dim3(pg), dim3(256), 0, s,
+                           (const float*) dpart.p, (const float*) dlpart.p, C, B,
+                           (float*) dg.p, (float*) dl.p);
         hipLaunchKernelGGL(mlp_adam_kernel, dim3(pg), dim3(256), 0, s, (float*) dp.p,
                            (const float*) dg.p, (float*) dm1.p, (float*) dm2.p,
                            (const int*) dstep.p, lr, 0.9f, 0.999f, 1e-8f);

@@ -196,7 +196,7 @@ class LogdetSurrogate(object):
 
     # ------------------------------------------------------------ network
 
-    def train(self, data, steps=(6000, 6000, 6000, 6000),
+    def train(self, data, steps=(10000, 10000, 10000, 10000),
               lr=(3e-3, 1e-3, 3e-4, 1e-4), init_seed=0):
         """Full-batch Adam on the GPU, one stage per (steps, lr) pair (each
         stage one captured step replayed ``steps`` times). Returns seconds."""
