@@ -413,6 +413,83 @@ __global__ __launch_bounds__(kThreads) void tiled_spmm_kernel(
     spmm_finish<T>(a, red, lane);
 }
 
+// Row-tiled form with vector-loaded slabs: for each chunk of up to 64 slabs
+// lane l loads the R values of slab (chunk + l) with one coalesced vector
+// load (consecutive lanes, consecutive 8R-byte runs), and each slab's values
+// are broadcast to the wavefront with readlane. Vector loads return in
+// order, so the gathers of eight slabs are kept in flight together instead
+// of draining scalar loads between groups.
+template <typename T, int R, bool NT, int U>
+__global__ __launch_bounds__(kThreads) void tiled_spmm_v_kernel(
+    SpmmArgs<T> a, const T* __restrict__ X, T* __restrict__ Y, T* __restrict__ Vout,
+    const T* __restrict__ Sub, int64_t ntiles, const int64_t* __restrict__ tptr,
+    const int32_t* __restrict__ tcols, const T* __restrict__ tvals)
+{
+    // U: gathers in flight per wavefront
+    const int lane = threadIdx.x % kWave;
+    const int wave = wave_id();
+    const Sweep sw = sweep(ntiles, wave);
+    const T s = a.xscale ? (T) a.xscale[lane] : T(1);
+    const T tt = (T) a.t;
+    const T sub_c = Sub ? (T) a.sub_coef[lane] : T(0);
+    const T sub_s = Sub ? (T) a.sub_scale[lane] : T(0);
+    double red = 0.0;
+    for (int64_t t = sw.first; t < sw.end; t += sw.stride)
+    {
+        T acc[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) acc[r] = T(0);
+        const int64_t beg = tptr[t];
+        const int64_t end = tptr[t + 1];
+        for (int64_t base = beg; base < end; base += kWave)
+        {
+            const int64_t rem = end - base;
+            const int cnt = rem < kWave ? (int) rem : kWave;
+            int cl = 0;
+            T v[R];
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[r] = T(0);
+            if (lane < cnt)
+            {
+                cl = tcols[base + lane];
+                const T* __restrict__ src = tvals + (base + lane) * R;
+#pragma unroll
+                for (int r = 0; r < R; ++r) v[r] = src[r];
+            }
+            int k = 0;
+            for (; k + U <= cnt; k += U)
+            {
+                T x[U];
+#pragma unroll
+                for (int i = 0; i < U; ++i)
+                    x[i] = X[(int64_t) readlane(cl, k + i) * kBlock + lane];
+#pragma unroll
+                for (int i = 0; i < U; ++i)
+                {
+                    const T xi = s * x[i];
+#pragma unroll
+                    for (int r = 0; r < R; ++r) acc[r] = fma(readlane(v[r], k + i), xi, acc[r]);
+                }
+            }
+            for (; k < cnt; ++k)
+            {
+                const T xi = s * X[(int64_t) readlane(cl, k) * kBlock + lane];
+#pragma unroll
+                for (int r = 0; r < R; ++r) acc[r] = fma(readlane(v[r], k), xi, acc[r]);
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+        {
+            const int64_t row = t * R + r;
+            if (row < a.nrows)
+                spmm_epilogue<T, NT>(a, row, lane, acc[r], s, tt, sub_c, sub_s, X, Y, Vout,
+                                     Sub, red);
+        }
+    }
+    spmm_finish<T>(a, red, lane);
+}
+
 // ---------------------------------------------------------------- update
 
 // Lanczos three-term update r = (w - alpha v_j) - beta_{j-1} v_{j-1}, in
@@ -820,6 +897,17 @@ int nt_default()
     return v;
 }
 
+// Tiled SpMM variant: 1 = scalar-loaded slabs, 2 / 3 = vector-loaded slabs
+// broadcast by readlane with 8 / 16 gathers in flight (SLQ_SPMM overrides).
+int spmm_variant()
+{
+    static int v = [] {
+        const char* e = std::getenv("SLQ_SPMM");
+        return e ? std::atoi(e) : 1;
+    }();
+    return v;
+}
+
 template <typename T>
 SpmmArgs<T> spmm_defaults()
 {
@@ -858,7 +946,17 @@ void launch_spmm_nt(slq_op* op, const SpmmArgs<T>& s, const SpmmIO<T>& io, const
         const T* tv = (sizeof(T) == 4 && op->A->csr.dtype == SLQ_F64)
                           ? (const T*) tl->vals_f32 : (const T*) tl->vals;
         const int grid = grid_for(tl->ntiles, op->spmm_grid);
-        if (tl->R == 8)
+        if (spmm_variant() == 2 || spmm_variant() == 3)
+        {
+            const bool u16 = spmm_variant() == 3;
+            auto k = tl->R == 8 ? (u16 ? tiled_spmm_v_kernel<T, 8, NT, 16>
+                                       : tiled_spmm_v_kernel<T, 8, NT, 8>)
+                                : (u16 ? tiled_spmm_v_kernel<T, 16, NT, 16>
+                                       : tiled_spmm_v_kernel<T, 16, NT, 8>);
+            hipLaunchKernelGGL(k, dim3(grid), dim3(kThreads), 0, op->stream, s, io.X, io.Y,
+                               io.Vout, io.Sub, tl->ntiles, tl->ptr, tl->cols, tv);
+        }
+        else if (tl->R == 8)
             hipLaunchKernelGGL((tiled_spmm_kernel<T, 8, NT>), dim3(grid), dim3(kThreads), 0,
                                op->stream, s, io.X, io.Y, io.Vout, io.Sub, tl->ntiles,
                                tl->ptr, tl->cols, tv);
