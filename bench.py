@@ -222,8 +222,9 @@ def surrogate_form(args):
     length scale l in [0.01, 0.1] with the support radius fixed at
     0.01 ln(1/0.03) (one pattern, ~247 non-zeros per row), queries
     logdet(K(l) + tI), t in [1e-4, 1e2], both log-uniform. The surrogate is
-    trained on 40 x 50 = 2000 GPU-SLQ estimates (lanczos_degree 30, 128
-    probes), calibrated on 10 x 40 validation estimates, judged on 10 x 20
+    trained on 40 x 50 = 2000 GPU-SLQ estimates (lanczos_degree 60, 128
+    probes; scales and t stratified over the box), calibrated on 10 x 40
+    random validation estimates, judged on 10 x 20
     held-out queries against SLQ (fallback where it leaves the 95% interval)
     and against dense Cholesky values recorded for 12 of them, then swept
     over 10,000 random queries (trust mode: SLQ only where the surrogate is
@@ -236,10 +237,11 @@ def surrogate_form(args):
     fam = CorrelationFamily(n, support, seed=0)
     log('family n=%d nnz=%d built in %.1fs' % (n, fam.nnz,
                                                time.perf_counter() - t0))
-    sur = LogdetSurrogate(fam, lanczos_degree=30, num_samples=128, seed=0)
+    m = 60
+    sur = LogdetSurrogate(fam, lanczos_degree=m, num_samples=128, seed=0)
     out = {'form': 'surrogate', 'n': n, 'nnz': int(fam.nnz),
            'scale_range': sur.scale_range, 't_range': sur.t_range,
-           'lanczos_degree': 30, 'samples': 128, 'network': '2-128-128-1 SiLU',
+           'lanczos_degree': m, 'samples': 128, 'network': '2-128-128-1 SiLU',
            'train_dtype': 'f32', 'inference_dtype': 'bf16 (MFMA)'}
     sur.solve([0.03], [1.0])                      # first-use set-up
     train = sur.generate(40, 50, seed=101)
@@ -250,7 +252,7 @@ def surrogate_form(args):
                     'loss_first': float(sur.losses[0]),
                     'loss_last': float(sur.losses[-1])}
     log('train: %s' % json.dumps(out['train']))
-    val = sur.generate(10, 40, seed=202)
+    val = sur.generate(10, 40, seed=202, stratified=False)
     rho = sur.calibrate(val)
     out['validation'] = {'points': len(rho),
                          'inside_ci': float(numpy.mean(numpy.abs(rho) <= 1)),
@@ -272,6 +274,14 @@ def surrogate_form(args):
             'median_err_over_ci': float(numpy.median(numpy.abs(pred - slq) / ci)),
             'max_err_over_ci': float(numpy.max(numpy.abs(pred - slq) / ci)),
             'median_ci_per_n': float(numpy.median(ci) / n)}
+    # how the trust rule (used without SLQ in the sweep) fares on these
+    trusted = sur.trusted(hs, ht)
+    inside = numpy.abs(pred - slq) <= ci
+    held['trusted_fraction'] = float(trusted.mean())
+    held['inside_ci_among_trusted'] = float(inside[trusted].mean()) \
+        if trusted.any() else None
+    held['inside_ci_among_untrusted'] = float(inside[~trusted].mean()) \
+        if (~trusted).any() else None
     path = os.path.join(HERE, 'tests', 'golden', 'surrogate_cholesky.json')
     if os.path.exists(path):
         with open(path) as f:

@@ -148,7 +148,13 @@ struct RedBuf {
     double* partial;            // [G][64]
     double* gpartial;           // [ceil(G/32)][64]
     unsigned int* counters;     // [1 + ceil(G/32)], zero between launches
+    // in-order tile schedule of the tiled SpMM: per XCD a ticket counter
+    // (uint 64 * x) and a finished-wave counter (uint 64 * x + 32), zero
+    // between launches, and one partial row per tile ([ntiles][64])
+    unsigned int* tickets;
+    double* tile_partial;
 };
+constexpr int kTicketWords = kXcds * 64;
 
 // Per-lane sum of v over the whole grid (see the header). Returns true in
 // exactly one workgroup, whose wave 0 then holds the totals in `total`.
@@ -488,6 +494,185 @@ __global__ __launch_bounds__(kThreads) void tiled_spmm_v_kernel(
         }
     }
     spmm_finish<T>(a, red, lane);
+}
+
+// One chunk (up to 64 slabs) of a tile, scalar-loaded slab values: the
+// slab address is wavefront-uniform, so the R values come in with one scalar
+// load and feed the FMAs as scalar operands; gathers in groups of four.
+template <typename T, int R>
+__device__ __forceinline__ void slabs_scalar(const T* __restrict__ X,
+                                             const int32_t* __restrict__ tcols,
+                                             const T* __restrict__ tvals, int64_t base, int cnt,
+                                             T s, int lane, T (&acc)[R])
+{
+    const int cl = lane < cnt ? tcols[base + lane] : 0;
+    const T* __restrict__ vb = tvals + base * R;
+    int k = 0;
+    for (; k + 4 <= cnt; k += 4)
+    {
+        const int c0 = readlane(cl, k);
+        const int c1 = readlane(cl, k + 1);
+        const int c2 = readlane(cl, k + 2);
+        const int c3 = readlane(cl, k + 3);
+        const T x0 = s * X[(int64_t) c0 * kBlock + lane];
+        const T x1 = s * X[(int64_t) c1 * kBlock + lane];
+        const T x2 = s * X[(int64_t) c2 * kBlock + lane];
+        const T x3 = s * X[(int64_t) c3 * kBlock + lane];
+        const T* v = vb + (int64_t) k * R;
+#pragma unroll
+        for (int r = 0; r < R; ++r) acc[r] = fma(v[r], x0, acc[r]);
+#pragma unroll
+        for (int r = 0; r < R; ++r) acc[r] = fma(v[R + r], x1, acc[r]);
+#pragma unroll
+        for (int r = 0; r < R; ++r) acc[r] = fma(v[2 * R + r], x2, acc[r]);
+#pragma unroll
+        for (int r = 0; r < R; ++r) acc[r] = fma(v[3 * R + r], x3, acc[r]);
+    }
+    for (; k < cnt; ++k)
+    {
+        const int c = readlane(cl, k);
+        const T xv = s * X[(int64_t) c * kBlock + lane];
+        const T* v = vb + (int64_t) k * R;
+#pragma unroll
+        for (int r = 0; r < R; ++r) acc[r] = fma(v[r], xv, acc[r]);
+    }
+}
+
+// The same with vector-loaded slab values: lane l loads the R values of slab
+// base + l (streamed once: non-temporal), broadcast by readlane; U gathers
+// in flight per wavefront.
+template <typename T, int R, int U>
+__device__ __forceinline__ void slabs_vector(const T* __restrict__ X,
+                                             const int32_t* __restrict__ tcols,
+                                             const T* __restrict__ tvals, int64_t base, int cnt,
+                                             T s, int lane, T (&acc)[R])
+{
+    int cl = 0;
+    T v[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[r] = T(0);
+    if (lane < cnt)
+    {
+        cl = __builtin_nontemporal_load(tcols + base + lane);
+        const T* __restrict__ src = tvals + (base + lane) * R;
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[r] = __builtin_nontemporal_load(src + r);
+    }
+    int k = 0;
+    for (; k + U <= cnt; k += U)
+    {
+        T xs[U];
+#pragma unroll
+        for (int i = 0; i < U; ++i) xs[i] = X[(int64_t) readlane(cl, k + i) * kBlock + lane];
+#pragma unroll
+        for (int i = 0; i < U; ++i)
+        {
+            const T xi = s * xs[i];
+#pragma unroll
+            for (int r = 0; r < R; ++r) acc[r] = fma(readlane(v[r], k + i), xi, acc[r]);
+        }
+    }
+    for (; k < cnt; ++k)
+    {
+        const T xi = s * X[(int64_t) readlane(cl, k) * kBlock + lane];
+#pragma unroll
+        for (int r = 0; r < R; ++r) acc[r] = fma(readlane(v[r], k), xi, acc[r]);
+    }
+}
+
+// Row-tiled form with an in-order tile schedule. Tiles are handed out to
+// wavefronts in increasing order by a per-XCD ticket counter (XCD = workgroup
+// id mod 8, each owning a contiguous eighth of the tiles), so the tiles in
+// flight on an XCD at any moment are consecutive and their gathered operand
+// rows stay within that XCD's L2. (A static interleaved assignment lets fast
+// and slow wavefronts drift apart; the set of tiles in flight then spreads
+// over several sweeps and the operand rows are fetched from HBM several
+// times.) Which wavefront handles which tile varies from run to run, so the
+// reduction terms are written per tile (tile_partial[t][probe], rows of the
+// tile added in order) and summed in tile order by tile_reduce_kernel: the
+// result does not depend on the schedule.
+template <typename T, int R, bool NT, int U>
+__global__ __launch_bounds__(kThreads) void tiled_spmm_sched_kernel(
+    SpmmArgs<T> a, const T* __restrict__ X, T* __restrict__ Y, T* __restrict__ Vout,
+    const T* __restrict__ Sub, int64_t ntiles, const int64_t* __restrict__ tptr,
+    const int32_t* __restrict__ tcols, const T* __restrict__ tvals)
+{
+    const int lane = threadIdx.x % kWave;
+    // V groups of workgroups (id mod V); V = 8 = the XCDs unless the grid is
+    // smaller, so that every range has workgroups
+    const int V = gridDim.x < (unsigned int) kXcds ? (int) gridDim.x : kXcds;
+    const int x = blockIdx.x % V;
+    const int64_t span = (ntiles + V - 1) / V;
+    const int64_t b = x * span;
+    const int64_t e = b + span < ntiles ? b + span : ntiles;
+    // wavefronts of the workgroups of this group
+    const unsigned int waves = (unsigned int) ((gridDim.x - x + V - 1) / V) * kWavesPerWG;
+    unsigned int* ticket = a.rb.tickets + 64 * x;
+    unsigned int* done = ticket + 32;
+    const T s = a.xscale ? (T) a.xscale[lane] : T(1);
+    const T tt = (T) a.t;
+    const T sub_c = Sub ? (T) a.sub_coef[lane] : T(0);
+    const T sub_s = Sub ? (T) a.sub_scale[lane] : T(0);
+    for (;;)
+    {
+        unsigned int k = 0;
+        if (lane == 0) k = atomicAdd(ticket, 1u);
+        const int64_t t = b + (int64_t) __builtin_amdgcn_readfirstlane(k);
+        if (t >= e) break;
+        T acc[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) acc[r] = T(0);
+        const int64_t beg = tptr[t];
+        const int64_t end = tptr[t + 1];
+        for (int64_t base = beg; base < end; base += kWave)
+        {
+            const int64_t rem = end - base;
+            const int cnt = rem < kWave ? (int) rem : kWave;
+            if constexpr (U == 0)
+                slabs_scalar<T, R>(X, tcols, tvals, base, cnt, s, lane, acc);
+            else
+                slabs_vector<T, R, U>(X, tcols, tvals, base, cnt, s, lane, acc);
+        }
+        double tred = 0.0;
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+        {
+            const int64_t row = t * R + r;
+            if (row < a.nrows)
+                spmm_epilogue<T, NT>(a, row, lane, acc[r], s, tt, sub_c, sub_s, X, Y, Vout,
+                                     Sub, tred);
+        }
+        if (a.red_mode) a.rb.tile_partial[t * kWave + lane] = tred;
+    }
+    // The last wavefront of this XCD to finish resets its counters for the
+    // next launch: every wavefront has drawn its final ticket before it
+    // counts itself done, so no ticket is drawn after the reset.
+    unsigned int d = 0;
+    if (lane == 0) d = atomicAdd(done, 1u);
+    d = __builtin_amdgcn_readfirstlane(d);
+    if (d == waves - 1 && lane == 0)
+    {
+        atomicExch(ticket, 0u);
+        atomicExch(done, 0u);
+    }
+}
+
+// Sum of the per-tile partial rows in tile order (workgroup g adds a
+// contiguous range, its wavefronts interleaved in a fixed order, then the
+// deterministic grid total), finished like the SpMM's own reduction.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void tile_reduce_kernel(SpmmArgs<T> a, int64_t ntiles)
+{
+    const int lane = threadIdx.x % kWave;
+    const int wave = wave_id();
+    const int64_t G = gridDim.x;
+    const int64_t per = (ntiles + G - 1) / G;
+    const int64_t t0 = (int64_t) blockIdx.x * per;
+    const int64_t t1 = t0 + per < ntiles ? t0 + per : ntiles;
+    double v = 0.0;
+    for (int64_t t = t0 + wave; t < t1; t += kWavesPerWG)
+        v += a.rb.tile_partial[t * kWave + lane];
+    spmm_finish<T>(a, v, lane);
 }
 
 // ---------------------------------------------------------------- update
@@ -839,27 +1024,47 @@ struct Bump {
     }
 };
 
-RedBuf take_redbuf(Bump& bump)
+// Tiles of the largest tiled operand a recurrence on `op` may multiply by
+// (A and, for Golub-Kahn, A^T): the length of the per-tile partial rows.
+int64_t sched_tiles(const slq_op* op)
+{
+    const MatrixStore& A = *op->A;
+    if (A.tiles.R == 0) return 0;
+    const int64_t R = A.tiles.R;
+    const int64_t a = (A.csr.nrows + R - 1) / R, b = (A.csr.ncols + R - 1) / R;
+    return a > b ? a : b;
+}
+
+RedBuf take_redbuf(Bump& bump, int64_t tiles)
 {
     RedBuf rb;
     rb.partial = bump.take<double>((size_t) kMaxGrid * kBlock);
     rb.gpartial = bump.take<double>((size_t) kMaxGroups * kBlock);
     rb.counters = bump.take<unsigned int>(1 + kMaxGroups);
+    rb.tickets = bump.take<unsigned int>(kTicketWords);
+    rb.tile_partial = tiles > 0 ? bump.take<double>((size_t) tiles * kBlock) : nullptr;
+    if (tiles > 0 && !rb.tile_partial) rb.partial = nullptr;   // reported as too small
     return rb;
 }
 
-bool redbuf_ok(const RedBuf& rb) { return rb.partial && rb.gpartial && rb.counters; }
+bool redbuf_ok(const RedBuf& rb)
+{
+    return rb.partial && rb.gpartial && rb.counters && rb.tickets;
+}
 
 void zero_counters(hipStream_t s, const RedBuf& rb)
 {
     hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, s, rb.counters, 1 + kMaxGroups);
+    hipLaunchKernelGGL(zero_u32_kernel, dim3(kTicketWords / 64), dim3(64), 0, s, rb.tickets,
+                       kTicketWords);
 }
 
-size_t redbuf_bytes()
+size_t redbuf_bytes(int64_t tiles)
 {
     auto r = [](size_t b) { return (b + 255) & ~(size_t) 255; };
     return r(sizeof(double) * kMaxGrid * kBlock) + r(sizeof(double) * kMaxGroups * kBlock) +
-           r(sizeof(unsigned int) * (1 + kMaxGroups));
+           r(sizeof(unsigned int) * (1 + kMaxGroups)) + r(sizeof(unsigned int) * kTicketWords) +
+           (tiles > 0 ? r(sizeof(double) * (size_t) tiles * kBlock) : 0);
 }
 
 template <typename T>
@@ -897,8 +1102,11 @@ int nt_default()
     return v;
 }
 
-// Tiled SpMM variant: 1 = scalar-loaded slabs, 2 / 3 = vector-loaded slabs
-// broadcast by readlane with 8 / 16 gathers in flight (SLQ_SPMM overrides).
+// Tiled SpMM variant: 1 = scalar-loaded slabs with the in-order tile
+// schedule, 0 = the same with the static interleaved sweep, 2 / 3 =
+// vector-loaded slabs broadcast by readlane with 8 / 16 gathers in flight
+// (static sweep), 4 / 5 = the vector-loaded forms with the in-order
+// schedule. SLQ_SPMM overrides.
 int spmm_variant()
 {
     static int v = [] {
@@ -946,7 +1154,27 @@ void launch_spmm_nt(slq_op* op, const SpmmArgs<T>& s, const SpmmIO<T>& io, const
         const T* tv = (sizeof(T) == 4 && op->A->csr.dtype == SLQ_F64)
                           ? (const T*) tl->vals_f32 : (const T*) tl->vals;
         const int grid = grid_for(tl->ntiles, op->spmm_grid);
-        if (spmm_variant() == 2 || spmm_variant() == 3)
+        const int var = spmm_variant();
+        if ((var == 1 || var >= 4) && s.rb.tickets && (s.red_mode == 0 || s.rb.tile_partial))
+        {
+            auto k8 = var == 4 ? tiled_spmm_sched_kernel<T, 8, NT, 8>
+                               : (var == 5 ? tiled_spmm_sched_kernel<T, 8, NT, 16>
+                                           : tiled_spmm_sched_kernel<T, 8, NT, 0>);
+            auto k16 = var == 4 ? tiled_spmm_sched_kernel<T, 16, NT, 8>
+                                : (var == 5 ? tiled_spmm_sched_kernel<T, 16, NT, 16>
+                                            : tiled_spmm_sched_kernel<T, 16, NT, 0>);
+            hipLaunchKernelGGL(tl->R == 8 ? k8 : k16, dim3(grid), dim3(kThreads), 0,
+                               op->stream, s, io.X, io.Y, io.Vout, io.Sub, tl->ntiles,
+                               tl->ptr, tl->cols, tv);
+            if (s.red_mode)
+            {
+                int rg = (int) ((tl->ntiles + 63) / 64);
+                rg = rg < 8 ? 8 : (rg > 256 ? 256 : rg);
+                hipLaunchKernelGGL((tile_reduce_kernel<T>), dim3(rg), dim3(kThreads), 0,
+                                   op->stream, s, tl->ntiles);
+            }
+        }
+        else if (spmm_variant() == 2 || spmm_variant() == 3)
         {
             const bool u16 = spmm_variant() == 3;
             auto k = tl->R == 8 ? (u16 ? tiled_spmm_v_kernel<T, 8, NT, 16>
@@ -1077,7 +1305,7 @@ int lanczos_ring(const RecurrenceArgs& ra, Bump& bump)
     T* B[3] = {bump.take<T>(blk), bump.take<T>(blk), bump.take<T>(blk)};
     double* S[3] = {bump.take<double>(kBlock), bump.take<double>(kBlock),
                     bump.take<double>(kBlock)};
-    const RedBuf rb = take_redbuf(bump);
+    const RedBuf rb = take_redbuf(bump, sched_tiles(op));
     if (!B[0] || !B[1] || !B[2] || !S[0] || !S[1] || !S[2] || !redbuf_ok(rb))
     {
         set_error("internal: Lanczos workspace too small");
@@ -1151,7 +1379,7 @@ int lanczos(const RecurrenceArgs& ra, Bump& bump)
     T* Q = bump.take<T>(blk);
     T* V = bump.take<T>(blk * nslots);
     double* scale = bump.take<double>(kBlock);
-    const RedBuf rb = take_redbuf(bump);
+    const RedBuf rb = take_redbuf(bump, sched_tiles(op));
     double* h1 = nullptr;
     double* h2 = nullptr;
     double* partial_k = nullptr;
@@ -1250,7 +1478,7 @@ int golub_kahn(const RecurrenceArgs& ra, Bump& bump)
     T* V = bump.take<T>(vblk * nslots);
     double* su = bump.take<double>((size_t) (m + 1) * kBlock);   // 1/alpha_j
     double* sv = bump.take<double>((size_t) (m + 1) * kBlock);   // 1/beta_{j-1}
-    const RedBuf rb = take_redbuf(bump);
+    const RedBuf rb = take_redbuf(bump, sched_tiles(op));
     int32_t* usizes = bump.take<int32_t>(kBlock);
     double* h1 = nullptr;
     double* h2 = nullptr;
@@ -1400,7 +1628,7 @@ size_t recurrence_scratch_bytes(const slq_op* op, int gram, int m, int ortho,
         add(sizeof(double) * (m + 1) * kBlock);
         add(sizeof(int32_t) * kBlock);
     }
-    b += redbuf_bytes();
+    b += redbuf_bytes(sched_tiles(op));
     if (reorth)
     {
         add(sizeof(double) * (m + 1) * kBlock);

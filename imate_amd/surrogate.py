@@ -56,15 +56,27 @@ __all__ = ['LogdetSurrogate', 'sample_queries']
 HIDDEN = 128
 
 
-def sample_queries(num_scales, num_t, scale_range, t_range, seed):
+def sample_queries(num_scales, num_t, scale_range, t_range, seed,
+                   stratified=False):
     """Seeded queries: ``num_scales`` length scales log-uniform over
     ``scale_range``, each with ``num_t`` t values log-uniform over
-    ``t_range``. Returns (scales[num_scales], t[num_scales, num_t])."""
+    ``t_range``. Returns (scales[num_scales], t[num_scales, num_t]).
+
+    ``stratified=True`` draws one value per equal slice of the log range
+    (scales, and the t values of each scale), which covers the edges of the
+    box as densely as its middle."""
     rng = numpy.random.default_rng(seed)
-    ls = numpy.exp(rng.uniform(numpy.log(scale_range[0]),
-                               numpy.log(scale_range[1]), num_scales))
-    ts = numpy.exp(rng.uniform(numpy.log(t_range[0]), numpy.log(t_range[1]),
-                               (num_scales, num_t)))
+    a, b = numpy.log(scale_range[0]), numpy.log(scale_range[1])
+    c, d = numpy.log(t_range[0]), numpy.log(t_range[1])
+    if stratified:
+        us = (rng.permutation(num_scales) + rng.random(num_scales)) / num_scales
+        ut = (numpy.argsort(rng.random((num_scales, num_t)), axis=1) +
+              rng.random((num_scales, num_t))) / num_t
+    else:
+        us = rng.random(num_scales)
+        ut = rng.random((num_scales, num_t))
+    ls = numpy.exp(a + (b - a) * us)
+    ts = numpy.exp(c + (d - c) * ut)
     return ls, numpy.sort(ts, axis=1)
 
 
@@ -185,10 +197,11 @@ class LogdetSurrogate(object):
                            'lanczos_runs': len(uniq)}
         return y, ci, sec
 
-    def generate(self, num_scales, num_t, seed):
-        """Training or validation data: SLQ estimates at seeded queries."""
+    def generate(self, num_scales, num_t, seed, stratified=True):
+        """Training or validation data: SLQ estimates at seeded queries
+        (stratified over the box by default)."""
         ls, ts = sample_queries(num_scales, num_t, self.scale_range,
-                                self.t_range, seed)
+                                self.t_range, seed, stratified=stratified)
         scales = numpy.repeat(ls, num_t)
         t = ts.ravel()
         y, ci, sec = self.solve(scales, t, seed=seed * 1000 + 1)

@@ -56,3 +56,13 @@ def test_sample_queries_and_inputs():
     x = sur._inputs(numpy.array([0.01, 0.1]), numpy.array([1e-4, 1e2]))
     assert numpy.allclose(x, [[-1, -1], [1, 1]], atol=1e-6)
     assert init_params(0).shape == (17025,)
+
+
+def test_stratified_queries_cover_every_slice():
+    ls, ts = sample_queries(16, 10, (0.01, 0.1), (1e-4, 1e2), seed=3,
+                            stratified=True)
+    u = (numpy.log(ls) - numpy.log(0.01)) / (numpy.log(0.1) - numpy.log(0.01))
+    assert sorted(numpy.floor(u * 16).astype(int)) == list(range(16))
+    v = (numpy.log(ts) - numpy.log(1e-4)) / (numpy.log(1e2) - numpy.log(1e-4))
+    for row in v:
+        assert sorted(numpy.floor(row * 10).astype(int)) == list(range(10))
