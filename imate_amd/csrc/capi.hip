@@ -125,6 +125,11 @@ int build_tiles(int64_t nrows, int64_t ncols, const int64_t* indptr,
             buf.insert(buf.end(), indices + indptr[row], indices + indptr[row + 1]);
         std::sort(buf.begin(), buf.end());
         buf.erase(std::unique(buf.begin(), buf.end()), buf.end());
+        // 16-row tiles feed the matrix-core SpMM four slabs at a time: pad
+        // each tile's slab list to a multiple of four with zero slabs that
+        // repeat its last column (a valid row of the operand)
+        if (R == 16 && !buf.empty())
+            while (buf.size() % 4) buf.push_back(buf.back());
         cols.insert(cols.end(), buf.begin(), buf.end());
         tptr[t + 1] = (int64_t) cols.size();
     }

@@ -657,6 +657,277 @@ __global__ __launch_bounds__(kThreads) void tiled_spmm_sched_kernel(
     }
 }
 
+// ------------------------------------------------ matrix-core SpMM
+
+// v_mfma_{f64,f32}_16x16x4: D[16][16] += A[16][4] B[4][16], one A and one B
+// element per lane: A[l % 16][l / 16], B[l / 16][l % 16]. Lane l holds four
+// elements of column l % 16 of D: rows 4 (l / 16) + m for the f32 form and
+// rows l / 16 + 4 m for the f64 form, m = 0..3 (measured on gfx950).
+template <typename T>
+struct Mfma16;
+template <>
+struct Mfma16<double> {
+    static constexpr int kRowStride = 4;   // row = l / 16 + 4 m
+    typedef double V4 __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ V4 op(double a, double b, V4 c)
+    {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+};
+template <>
+struct Mfma16<float> {
+    static constexpr int kRowStride = 1;   // row = 4 (l / 16) + m
+    typedef float V4 __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ V4 op(float a, float b, V4 c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+};
+
+// Tiled SpMM on the matrix cores. A 16-row tile times the gathered operand
+// rows of its slabs is a dense product (16 x slabs) (slabs x 64 probes):
+// four slabs at a time, A = the 16 x 4 slab values (64 consecutive values of
+// the slab array, one coalesced load), B = the four gathered operand rows,
+// one 16-probe block per MFMA (four per group). Matrix values thus arrive
+// through ordinary vector loads and are broadcast by the matrix core, so
+// many slabs can be in flight per wavefront (G4 groups of four per
+// iteration). Tiles are handed out in order as in tiled_spmm_sched_kernel;
+// the per-probe reduction term of a tile adds its rows within each lane
+// (four rows), then across the four row groups (lanes l, l ^ 16, l ^ 32,
+// l ^ 48) in a fixed order.
+template <typename T, bool NT, int G4>
+__global__ __launch_bounds__(kThreads) void mfma_spmm_kernel(
+    SpmmArgs<T> a, const T* __restrict__ X, T* __restrict__ Y, T* __restrict__ Vout,
+    const T* __restrict__ Sub, int64_t ntiles, const int64_t* __restrict__ tptr,
+    const int32_t* __restrict__ tcols, const T* __restrict__ tvals)
+{
+    using V4 = typename Mfma16<T>::V4;
+    const int lane = threadIdx.x % kWave;
+    const int i0 = lane % 16;
+    const int kq = lane / 16;
+    const int V = gridDim.x < (unsigned int) kXcds ? (int) gridDim.x : kXcds;
+    const int x = blockIdx.x % V;
+    const int64_t span = (ntiles + V - 1) / V;
+    const int64_t b = x * span;
+    const int64_t e = b + span < ntiles ? b + span : ntiles;
+    const unsigned int waves = (unsigned int) ((gridDim.x - x + V - 1) / V) * kWavesPerWG;
+    unsigned int* ticket = a.rb.tickets + 64 * x;
+    unsigned int* done = ticket + 32;
+    T sq[4], sub_c[4], sub_s[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+    {
+        const int p = 16 * q + i0;
+        sq[q] = a.xscale ? (T) a.xscale[p] : T(1);
+        sub_c[q] = Sub ? (T) a.sub_coef[p] : T(0);
+        sub_s[q] = Sub ? (T) a.sub_scale[p] : T(0);
+    }
+    const T tt = (T) a.t;
+    for (;;)
+    {
+        unsigned int k = 0;
+        if (lane == 0) k = atomicAdd(ticket, 1u);
+        const int64_t t = b + (int64_t) __builtin_amdgcn_readfirstlane(k);
+        if (t >= e) break;
+        V4 acc[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) acc[q] = V4{T(0), T(0), T(0), T(0)};
+        const int64_t beg = tptr[t];
+        const int64_t end = tptr[t + 1];
+        for (int64_t u = beg; u < end; u += 4 * G4)
+        {
+            T av[G4];
+            T bv[G4][4];
+#pragma unroll
+            for (int g = 0; g < G4; ++g)
+            {
+                const int64_t uu = u + 4 * g;
+                if (uu < end)
+                {
+                    av[g] = tvals[uu * 16 + lane];
+                    const int64_t c = tcols[uu + kq];
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) bv[g][q] = X[c * kBlock + 16 * q + i0];
+                }
+            }
+#pragma unroll
+            for (int g = 0; g < G4; ++g)
+            {
+                if (u + 4 * g < end)
+                {
+#pragma unroll
+                    for (int q = 0; q < 4; ++q)
+                        acc[q] = Mfma16<T>::op(av[g], sq[q] * bv[g][q], acc[q]);
+                }
+            }
+        }
+        double red[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+        {
+#pragma unroll
+            for (int m = 0; m < 4; ++m)
+            {
+                const int64_t row = t * 16 +
+                                    (Mfma16<T>::kRowStride == 4 ? kq + 4 * m : 4 * kq + m);
+                if (row < a.nrows)
+                    spmm_epilogue<T, NT>(a, row, 16 * q + i0, acc[q][m], sq[q], tt, sub_c[q],
+                                         sub_s[q], X, Y, Vout, Sub, red[q]);
+            }
+        }
+        if (a.red_mode)
+        {
+            // four rows of the tile are in this lane; add the other groups
+            double mine = 0.0;
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+            {
+                const double r1 = red[q] + __shfl_xor(red[q], 16);
+                const double r2 = r1 + __shfl_xor(r1, 32);
+                if (q == kq) mine = r2;
+            }
+            a.rb.tile_partial[t * kWave + 16 * kq + i0] = mine;
+        }
+    }
+    unsigned int d = 0;
+    if (lane == 0) d = atomicAdd(done, 1u);
+    d = __builtin_amdgcn_readfirstlane(d);
+    if (d == waves - 1 && lane == 0)
+    {
+        atomicExch(ticket, 0u);
+        atomicExch(done, 0u);
+    }
+}
+
+// Matrix-core SpMM, pipelined: the column indices of up to 64 slabs are
+// loaded once per chunk (lane l holds slab chunk + l) and broadcast with
+// readlane, so a group's gathers need no dependent index load, and the
+// loads of the next group of four slabs are issued before the MFMAs of the
+// current one (two register sets).
+template <typename T, bool NT, bool LATE>
+__global__ __launch_bounds__(kThreads) void mfma_spmm_pipe_kernel(
+    SpmmArgs<T> a, const T* __restrict__ X, T* __restrict__ Y, T* __restrict__ Vout,
+    const T* __restrict__ Sub, int64_t ntiles, const int64_t* __restrict__ tptr,
+    const int32_t* __restrict__ tcols, const T* __restrict__ tvals)
+{
+    using V4 = typename Mfma16<T>::V4;
+    const int lane = threadIdx.x % kWave;
+    const int i0 = lane % 16;
+    const int kq = lane / 16;
+    const int V = gridDim.x < (unsigned int) kXcds ? (int) gridDim.x : kXcds;
+    const int x = blockIdx.x % V;
+    const int64_t span = (ntiles + V - 1) / V;
+    const int64_t b = x * span;
+    const int64_t e = b + span < ntiles ? b + span : ntiles;
+    const unsigned int waves = (unsigned int) ((gridDim.x - x + V - 1) / V) * kWavesPerWG;
+    unsigned int* ticket = a.rb.tickets + 64 * x;
+    unsigned int* done = ticket + 32;
+    // LATE: the per-probe scale of the operand is applied to the product in
+    // the epilogue, s (A x), instead of to every gathered value, A (s x)
+    T sq[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+        sq[q] = (!LATE && a.xscale) ? (T) a.xscale[16 * q + i0] : T(1);
+    const T tt = (T) a.t;
+    for (;;)
+    {
+        unsigned int k = 0;
+        if (lane == 0) k = atomicAdd(ticket, 1u);
+        const int64_t t = b + (int64_t) __builtin_amdgcn_readfirstlane(k);
+        if (t >= e) break;
+        V4 acc[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) acc[q] = V4{T(0), T(0), T(0), T(0)};
+        const int64_t beg = tptr[t];
+        const int64_t end = tptr[t + 1];
+        for (int64_t base = beg; base < end; base += kWave)
+        {
+            const int64_t rem = end - base;
+            const int cnt = rem < kWave ? (int) rem : kWave;   // multiple of 4
+            const int cl = lane < cnt ? tcols[base + lane] : 0;
+            auto col_of = [&](int g) {
+                const int c0 = readlane(cl, 4 * g);
+                const int c1 = readlane(cl, 4 * g + 1);
+                const int c2 = readlane(cl, 4 * g + 2);
+                const int c3 = readlane(cl, 4 * g + 3);
+                return (int64_t) (kq == 0 ? c0 : (kq == 1 ? c1 : (kq == 2 ? c2 : c3)));
+            };
+            const int ng = cnt / 4;
+            T a0, a1, b0[4], b1[4];
+            {
+                a0 = tvals[base * 16 + lane];
+                const int64_t c = col_of(0);
+#pragma unroll
+                for (int q = 0; q < 4; ++q) b0[q] = X[c * kBlock + 16 * q + i0];
+            }
+            int g = 0;
+            for (; g + 2 <= ng; g += 2)
+            {
+                {
+                    a1 = tvals[(base + 4 * (g + 1)) * 16 + lane];
+                    const int64_t c = col_of(g + 1);
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) b1[q] = X[c * kBlock + 16 * q + i0];
+                }
+#pragma unroll
+                for (int q = 0; q < 4; ++q) acc[q] = Mfma16<T>::op(a0, LATE ? b0[q] : sq[q] * b0[q], acc[q]);
+                if (g + 2 < ng)
+                {
+                    a0 = tvals[(base + 4 * (g + 2)) * 16 + lane];
+                    const int64_t c = col_of(g + 2);
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) b0[q] = X[c * kBlock + 16 * q + i0];
+                }
+#pragma unroll
+                for (int q = 0; q < 4; ++q) acc[q] = Mfma16<T>::op(a1, LATE ? b1[q] : sq[q] * b1[q], acc[q]);
+            }
+            if (g < ng)
+            {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) acc[q] = Mfma16<T>::op(a0, LATE ? b0[q] : sq[q] * b0[q], acc[q]);
+            }
+        }
+        double red[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+        {
+            const int p = 16 * q + i0;
+            const T sub_c = Sub ? (T) a.sub_coef[p] : T(0);
+            const T sub_s = Sub ? (T) a.sub_scale[p] : T(0);
+            const T sp = a.xscale ? (T) a.xscale[p] : T(1);
+#pragma unroll
+            for (int m = 0; m < 4; ++m)
+            {
+                const int64_t row = t * 16 +
+                                    (Mfma16<T>::kRowStride == 4 ? kq + 4 * m : 4 * kq + m);
+                if (row < a.nrows)
+                    spmm_epilogue<T, NT>(a, row, p, LATE ? sp * acc[q][m] : acc[q][m], sp, tt,
+                                         sub_c, sub_s, X, Y, Vout, Sub, red[q]);
+            }
+        }
+        if (a.red_mode)
+        {
+            double mine = 0.0;
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+            {
+                const double r1 = red[q] + __shfl_xor(red[q], 16);
+                const double r2 = r1 + __shfl_xor(r1, 32);
+                if (q == kq) mine = r2;
+            }
+            a.rb.tile_partial[t * kWave + 16 * kq + i0] = mine;
+        }
+    }
+    unsigned int d = 0;
+    if (lane == 0) d = atomicAdd(done, 1u);
+    d = __builtin_amdgcn_readfirstlane(d);
+    if (d == waves - 1 && lane == 0)
+    {
+        atomicExch(ticket, 0u);
+        atomicExch(done, 0u);
+    }
+}
+
 // Sum of the per-tile partial rows in tile order (workgroup g adds a
 // contiguous range, its wavefronts interleaved in a fixed order, then the
 // deterministic grid total), finished like the SpMM's own reduction.
@@ -1155,7 +1426,24 @@ void launch_spmm_nt(slq_op* op, const SpmmArgs<T>& s, const SpmmIO<T>& io, const
                           ? (const T*) tl->vals_f32 : (const T*) tl->vals;
         const int grid = grid_for(tl->ntiles, op->spmm_grid);
         const int var = spmm_variant();
-        if ((var == 1 || var >= 4) && s.rb.tickets && (s.red_mode == 0 || s.rb.tile_partial))
+        const bool sched = s.rb.tickets && (s.red_mode == 0 || s.rb.tile_partial);
+        if (tl->R == 16 && sched && (var == 1 || (var >= 6 && var <= 9)))
+        {
+            auto km = var == 7   ? mfma_spmm_kernel<T, NT, 2>
+                      : var == 6 ? mfma_spmm_kernel<T, NT, 4>
+                      : var == 9 ? mfma_spmm_pipe_kernel<T, NT, true>
+                                 : mfma_spmm_pipe_kernel<T, NT, false>;
+            hipLaunchKernelGGL(km, dim3(grid), dim3(kThreads), 0, op->stream, s, io.X, io.Y,
+                               io.Vout, io.Sub, tl->ntiles, tl->ptr, tl->cols, tv);
+            if (s.red_mode)
+            {
+                int rg = (int) ((tl->ntiles + 63) / 64);
+                rg = rg < 8 ? 8 : (rg > 256 ? 256 : rg);
+                hipLaunchKernelGGL((tile_reduce_kernel<T>), dim3(rg), dim3(kThreads), 0,
+                                   op->stream, s, tl->ntiles);
+            }
+        }
+        else if ((var == 1 || var >= 4) && sched)
         {
             auto k8 = var == 4 ? tiled_spmm_sched_kernel<T, 8, NT, 8>
                                : (var == 5 ? tiled_spmm_sched_kernel<T, 8, NT, 16>

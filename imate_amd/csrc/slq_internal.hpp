@@ -101,6 +101,9 @@ struct MatrixStore {
 
 // Host construction of the tiled form (R in {8, 16}); returns R = 0 in
 // `out` when fewer than `min_density` of the slab entries would be non-zero.
+// With R = 16 every tile's slab count is padded to a multiple of four (zero
+// slabs repeating the tile's last column), the layout of the matrix-core
+// SpMM.
 int build_tiles(int64_t nrows, int64_t ncols, const int64_t* indptr,
                 const int32_t* indices, const void* data, int dtype, int R,
                 double min_density, DevTiles& out);

@@ -90,3 +90,31 @@ def test_logdet_per_sample_parity_c1():
         rel = numpy.abs(got - want) / numpy.abs(want)
         assert rel.max() < 1e-10, rel.max()
         assert abs(ld - ref['trace'][0]) <= 1e-10 * abs(ref['trace'][0])
+
+
+@pytest.mark.parametrize('tile_rows', ['0', '8', '16'])
+@pytest.mark.parametrize('gram', [0, 1])
+@pytest.mark.parametrize('ortho', [0, -1])
+def test_recurrences_every_spmm_layout(gpu_lib, monkeypatch, tile_rows, gram,
+                                       ortho):
+    """CSR, 8-row tiles (scalar-loaded slabs) and 16-row tiles (matrix
+    cores) give the oracle's recurrences, for Lanczos and Golub-Kahn, with
+    and without re-orthogonalisation."""
+    monkeypatch.setenv('SLQ_TILE_ROWS', tile_rows)
+    A = _corr(3000, 40, 2)
+    dev = _op(A)
+    m, count, seed = 16, 64, 9
+    a = numpy.zeros((count, m))
+    b = numpy.zeros((count, m))
+    sz = numpy.zeros(count, dtype=numpy.int32)
+    _lib.check(gpu_lib.slq_recurrences(dev.handle, 0.0, gram, seed, 0, count,
+                                       m, numpy.finfo(float).eps, ortho, 0,
+                                       _lib.ptr(a), _lib.ptr(b), _lib.ptr(sz)),
+               'slq_recurrences')
+    op = oracle.Operator(A)
+    ra, rb, rs = oracle.recurrences(op, seed, count, m,
+                                    numpy.finfo(float).eps, ortho, gram=gram)
+    assert numpy.array_equal(sz, rs)
+    scale = max(1.0, numpy.abs(ra).max())
+    assert numpy.max(numpy.abs(a - ra)) < 1e-10 * scale
+    assert numpy.max(numpy.abs(b - rb)) < 1e-10 * scale
