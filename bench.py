@@ -62,10 +62,11 @@ def parse():
 
 
 def spmm_bytes(n, nnz, b=64, vbytes=8):
-    """Algorithmic HBM bytes of one fused Lanczos SpMM launch: the CSR arrays
-    once (values, int32 columns, int64 row pointers), the gathered operand
-    block once, the normalised basis block written and the product written."""
-    return nnz * (vbytes + 4) + (n + 1) * 8 + 3 * n * b * vbytes
+    """Algorithmic HBM bytes of one Lanczos SpMM launch: the CSR arrays once
+    (values, int32 columns, int64 row pointers), the gathered operand block
+    read once and the product block written once:
+    nnz (s_v + 4) + 8 (n + 1) + 2 n b s_v."""
+    return nnz * (vbytes + 4) + (n + 1) * 8 + 2 * n * b * vbytes
 
 
 def cholesky_reference(n, k, seed):
@@ -481,7 +482,8 @@ def main():
         'roofline': {'bound': 'hbm', 'achieved': achieved,
                      'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
                      'frac': achieved / HBM_PEAK_GBS, 'traffic': None,
-                     'kernel': 'spmm_kernel<double> (fused Lanczos SpMM)',
+                     'kernel': 'mfma_spmm_kernel<double> + tile sums (one '
+                               'Lanczos SpMM launch)',
                      'bytes_per_launch': bspmm,
                      'avg_launch_ms': avg_spmm},
         'update_kernel': {'avg_launch_ms': avg_upd,

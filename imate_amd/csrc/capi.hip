@@ -1112,8 +1112,10 @@ int slq_op_create_csr(int64_t nrows, int64_t ncols, int64_t nnz,
         (nnz > 0 && (e = hipMemcpy(c.data, data, vb * nnz, hipMemcpyHostToDevice)) != hipSuccess))
         return fail(hip_status(e, "hipMemcpy", __FILE__, __LINE__));
     {
+        // 16-row tiles (matrix-core SpMM) unless SLQ_TILE_ROWS asks for 8 or
+        // 0 (CSR only)
         const char* env = std::getenv("SLQ_TILE_ROWS");
-        const int R = env ? std::atoi(env) : 8;
+        const int R = env ? std::atoi(env) : 16;
         if (R == 8 || R == 16)
         {
             int r = build_tiles(nrows, ncols, indptr, indices, data, dtype, R, 0.25,

@@ -304,6 +304,19 @@ __device__ __forceinline__ void spmm_epilogue(const SpmmArgs<T>& a, int64_t row,
     else if (a.red_mode == 2) red += (double) y * (double) y;
 }
 
+// Stores the grid total of the SpMM's reduction term (alpha, or |u|^2 /
+// its root for Golub-Kahn).
+template <typename T>
+__device__ __forceinline__ void spmm_store_total(const SpmmArgs<T>& a, double total, int lane)
+{
+    if (a.red_mode == 1 || !a.finish_sqrt)
+    {
+        a.red_out[lane] = total;
+        return;
+    }
+    finish_norm(total, lane, a.j, a.tol_sqrt_n, a.red_out, a.scale_out, a.sizes);
+}
+
 template <typename T>
 __device__ __forceinline__ void spmm_finish(const SpmmArgs<T>& a, double red, int lane)
 {
@@ -311,12 +324,7 @@ __device__ __forceinline__ void spmm_finish(const SpmmArgs<T>& a, double red, in
     double total;
     if (!grid_total(red, a.rb, total)) return;
     if (wave_id() != 0) return;
-    if (a.red_mode == 1 || !a.finish_sqrt)
-    {
-        a.red_out[lane] = total;
-        return;
-    }
-    finish_norm(total, lane, a.j, a.tol_sqrt_n, a.red_out, a.scale_out, a.sizes);
+    spmm_store_total(a, total, lane);
 }
 
 // CSR form: one wavefront per row.
@@ -419,83 +427,6 @@ __global__ __launch_bounds__(kThreads) void tiled_spmm_kernel(
     spmm_finish<T>(a, red, lane);
 }
 
-// Row-tiled form with vector-loaded slabs: for each chunk of up to 64 slabs
-// lane l loads the R values of slab (chunk + l) with one coalesced vector
-// load (consecutive lanes, consecutive 8R-byte runs), and each slab's values
-// are broadcast to the wavefront with readlane. Vector loads return in
-// order, so the gathers of eight slabs are kept in flight together instead
-// of draining scalar loads between groups.
-template <typename T, int R, bool NT, int U>
-__global__ __launch_bounds__(kThreads) void tiled_spmm_v_kernel(
-    SpmmArgs<T> a, const T* __restrict__ X, T* __restrict__ Y, T* __restrict__ Vout,
-    const T* __restrict__ Sub, int64_t ntiles, const int64_t* __restrict__ tptr,
-    const int32_t* __restrict__ tcols, const T* __restrict__ tvals)
-{
-    // U: gathers in flight per wavefront
-    const int lane = threadIdx.x % kWave;
-    const int wave = wave_id();
-    const Sweep sw = sweep(ntiles, wave);
-    const T s = a.xscale ? (T) a.xscale[lane] : T(1);
-    const T tt = (T) a.t;
-    const T sub_c = Sub ? (T) a.sub_coef[lane] : T(0);
-    const T sub_s = Sub ? (T) a.sub_scale[lane] : T(0);
-    double red = 0.0;
-    for (int64_t t = sw.first; t < sw.end; t += sw.stride)
-    {
-        T acc[R];
-#pragma unroll
-        for (int r = 0; r < R; ++r) acc[r] = T(0);
-        const int64_t beg = tptr[t];
-        const int64_t end = tptr[t + 1];
-        for (int64_t base = beg; base < end; base += kWave)
-        {
-            const int64_t rem = end - base;
-            const int cnt = rem < kWave ? (int) rem : kWave;
-            int cl = 0;
-            T v[R];
-#pragma unroll
-            for (int r = 0; r < R; ++r) v[r] = T(0);
-            if (lane < cnt)
-            {
-                cl = tcols[base + lane];
-                const T* __restrict__ src = tvals + (base + lane) * R;
-#pragma unroll
-                for (int r = 0; r < R; ++r) v[r] = src[r];
-            }
-            int k = 0;
-            for (; k + U <= cnt; k += U)
-            {
-                T x[U];
-#pragma unroll
-                for (int i = 0; i < U; ++i)
-                    x[i] = X[(int64_t) readlane(cl, k + i) * kBlock + lane];
-#pragma unroll
-                for (int i = 0; i < U; ++i)
-                {
-                    const T xi = s * x[i];
-#pragma unroll
-                    for (int r = 0; r < R; ++r) acc[r] = fma(readlane(v[r], k + i), xi, acc[r]);
-                }
-            }
-            for (; k < cnt; ++k)
-            {
-                const T xi = s * X[(int64_t) readlane(cl, k) * kBlock + lane];
-#pragma unroll
-                for (int r = 0; r < R; ++r) acc[r] = fma(readlane(v[r], k), xi, acc[r]);
-            }
-        }
-#pragma unroll
-        for (int r = 0; r < R; ++r)
-        {
-            const int64_t row = t * R + r;
-            if (row < a.nrows)
-                spmm_epilogue<T, NT>(a, row, lane, acc[r], s, tt, sub_c, sub_s, X, Y, Vout,
-                                     Sub, red);
-        }
-    }
-    spmm_finish<T>(a, red, lane);
-}
-
 // One chunk (up to 64 slabs) of a tile, scalar-loaded slab values: the
 // slab address is wavefront-uniform, so the R values come in with one scalar
 // load and feed the FMAs as scalar operands; gathers in groups of four.
@@ -538,48 +469,6 @@ __device__ __forceinline__ void slabs_scalar(const T* __restrict__ X,
     }
 }
 
-// The same with vector-loaded slab values: lane l loads the R values of slab
-// base + l (streamed once: non-temporal), broadcast by readlane; U gathers
-// in flight per wavefront.
-template <typename T, int R, int U>
-__device__ __forceinline__ void slabs_vector(const T* __restrict__ X,
-                                             const int32_t* __restrict__ tcols,
-                                             const T* __restrict__ tvals, int64_t base, int cnt,
-                                             T s, int lane, T (&acc)[R])
-{
-    int cl = 0;
-    T v[R];
-#pragma unroll
-    for (int r = 0; r < R; ++r) v[r] = T(0);
-    if (lane < cnt)
-    {
-        cl = __builtin_nontemporal_load(tcols + base + lane);
-        const T* __restrict__ src = tvals + (base + lane) * R;
-#pragma unroll
-        for (int r = 0; r < R; ++r) v[r] = __builtin_nontemporal_load(src + r);
-    }
-    int k = 0;
-    for (; k + U <= cnt; k += U)
-    {
-        T xs[U];
-#pragma unroll
-        for (int i = 0; i < U; ++i) xs[i] = X[(int64_t) readlane(cl, k + i) * kBlock + lane];
-#pragma unroll
-        for (int i = 0; i < U; ++i)
-        {
-            const T xi = s * xs[i];
-#pragma unroll
-            for (int r = 0; r < R; ++r) acc[r] = fma(readlane(v[r], k + i), xi, acc[r]);
-        }
-    }
-    for (; k < cnt; ++k)
-    {
-        const T xi = s * X[(int64_t) readlane(cl, k) * kBlock + lane];
-#pragma unroll
-        for (int r = 0; r < R; ++r) acc[r] = fma(readlane(v[r], k), xi, acc[r]);
-    }
-}
-
 // Row-tiled form with an in-order tile schedule. Tiles are handed out to
 // wavefronts in increasing order by a per-XCD ticket counter (XCD = workgroup
 // id mod 8, each owning a contiguous eighth of the tiles), so the tiles in
@@ -589,9 +478,10 @@ __device__ __forceinline__ void slabs_vector(const T* __restrict__ X,
 // over several sweeps and the operand rows are fetched from HBM several
 // times.) Which wavefront handles which tile varies from run to run, so the
 // reduction terms are written per tile (tile_partial[t][probe], rows of the
-// tile added in order) and summed in tile order by tile_reduce_kernel: the
+// tile added in order) and summed in tile order by tile_sum_kernel and
+// tile_total_kernel: the
 // result does not depend on the schedule.
-template <typename T, int R, bool NT, int U>
+template <typename T, int R, bool NT>
 __global__ __launch_bounds__(kThreads) void tiled_spmm_sched_kernel(
     SpmmArgs<T> a, const T* __restrict__ X, T* __restrict__ Y, T* __restrict__ Vout,
     const T* __restrict__ Sub, int64_t ntiles, const int64_t* __restrict__ tptr,
@@ -628,10 +518,7 @@ __global__ __launch_bounds__(kThreads) void tiled_spmm_sched_kernel(
         {
             const int64_t rem = end - base;
             const int cnt = rem < kWave ? (int) rem : kWave;
-            if constexpr (U == 0)
-                slabs_scalar<T, R>(X, tcols, tvals, base, cnt, s, lane, acc);
-            else
-                slabs_vector<T, R, U>(X, tcols, tvals, base, cnt, s, lane, acc);
+            slabs_scalar<T, R>(X, tcols, tvals, base, cnt, s, lane, acc);
         }
         double tred = 0.0;
 #pragma unroll
@@ -688,14 +575,17 @@ struct Mfma16<float> {
 // rows of its slabs is a dense product (16 x slabs) (slabs x 64 probes):
 // four slabs at a time, A = the 16 x 4 slab values (64 consecutive values of
 // the slab array, one coalesced load), B = the four gathered operand rows,
-// one 16-probe block per MFMA (four per group). Matrix values thus arrive
-// through ordinary vector loads and are broadcast by the matrix core, so
-// many slabs can be in flight per wavefront (G4 groups of four per
-// iteration). Tiles are handed out in order as in tiled_spmm_sched_kernel;
-// the per-probe reduction term of a tile adds its rows within each lane
-// (four rows), then across the four row groups (lanes l, l ^ 16, l ^ 32,
-// l ^ 48) in a fixed order.
-template <typename T, bool NT, int G4>
+// one 16-probe block per MFMA (four per group). The matrix values arrive
+// through ordinary vector loads and are broadcast by the matrix core (no
+// scalar loads, no readlane per value), so loads stay in flight: the column
+// indices of up to 64 slabs are loaded once per chunk (lane l holds slab
+// chunk + l) and broadcast with readlane, so a group's gathers need no
+// dependent index load, and the loads of the next group of four slabs are
+// issued before the MFMAs of the current one (two register sets). Tiles are
+// handed out in order as in tiled_spmm_sched_kernel; the per-probe reduction
+// term of a tile adds the four rows held by a lane, then the four row groups
+// (lanes l, l ^ 16, l ^ 32, l ^ 48) in a fixed order.
+template <typename T, bool NT>
 __global__ __launch_bounds__(kThreads) void mfma_spmm_kernel(
     SpmmArgs<T> a, const T* __restrict__ X, T* __restrict__ Y, T* __restrict__ Vout,
     const T* __restrict__ Sub, int64_t ntiles, const int64_t* __restrict__ tptr,
@@ -713,121 +603,9 @@ __global__ __launch_bounds__(kThreads) void mfma_spmm_kernel(
     const unsigned int waves = (unsigned int) ((gridDim.x - x + V - 1) / V) * kWavesPerWG;
     unsigned int* ticket = a.rb.tickets + 64 * x;
     unsigned int* done = ticket + 32;
-    T sq[4], sub_c[4], sub_s[4];
-#pragma unroll
-    for (int q = 0; q < 4; ++q)
-    {
-        const int p = 16 * q + i0;
-        sq[q] = a.xscale ? (T) a.xscale[p] : T(1);
-        sub_c[q] = Sub ? (T) a.sub_coef[p] : T(0);
-        sub_s[q] = Sub ? (T) a.sub_scale[p] : T(0);
-    }
-    const T tt = (T) a.t;
-    for (;;)
-    {
-        unsigned int k = 0;
-        if (lane == 0) k = atomicAdd(ticket, 1u);
-        const int64_t t = b + (int64_t) __builtin_amdgcn_readfirstlane(k);
-        if (t >= e) break;
-        V4 acc[4];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) acc[q] = V4{T(0), T(0), T(0), T(0)};
-        const int64_t beg = tptr[t];
-        const int64_t end = tptr[t + 1];
-        for (int64_t u = beg; u < end; u += 4 * G4)
-        {
-            T av[G4];
-            T bv[G4][4];
-#pragma unroll
-            for (int g = 0; g < G4; ++g)
-            {
-                const int64_t uu = u + 4 * g;
-                if (uu < end)
-                {
-                    av[g] = tvals[uu * 16 + lane];
-                    const int64_t c = tcols[uu + kq];
-#pragma unroll
-                    for (int q = 0; q < 4; ++q) bv[g][q] = X[c * kBlock + 16 * q + i0];
-                }
-            }
-#pragma unroll
-            for (int g = 0; g < G4; ++g)
-            {
-                if (u + 4 * g < end)
-                {
-#pragma unroll
-                    for (int q = 0; q < 4; ++q)
-                        acc[q] = Mfma16<T>::op(av[g], sq[q] * bv[g][q], acc[q]);
-                }
-            }
-        }
-        double red[4] = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-        for (int q = 0; q < 4; ++q)
-        {
-#pragma unroll
-            for (int m = 0; m < 4; ++m)
-            {
-                const int64_t row = t * 16 +
-                                    (Mfma16<T>::kRowStride == 4 ? kq + 4 * m : 4 * kq + m);
-                if (row < a.nrows)
-                    spmm_epilogue<T, NT>(a, row, 16 * q + i0, acc[q][m], sq[q], tt, sub_c[q],
-                                         sub_s[q], X, Y, Vout, Sub, red[q]);
-            }
-        }
-        if (a.red_mode)
-        {
-            // four rows of the tile are in this lane; add the other groups
-            double mine = 0.0;
-#pragma unroll
-            for (int q = 0; q < 4; ++q)
-            {
-                const double r1 = red[q] + __shfl_xor(red[q], 16);
-                const double r2 = r1 + __shfl_xor(r1, 32);
-                if (q == kq) mine = r2;
-            }
-            a.rb.tile_partial[t * kWave + 16 * kq + i0] = mine;
-        }
-    }
-    unsigned int d = 0;
-    if (lane == 0) d = atomicAdd(done, 1u);
-    d = __builtin_amdgcn_readfirstlane(d);
-    if (d == waves - 1 && lane == 0)
-    {
-        atomicExch(ticket, 0u);
-        atomicExch(done, 0u);
-    }
-}
-
-// Matrix-core SpMM, pipelined: the column indices of up to 64 slabs are
-// loaded once per chunk (lane l holds slab chunk + l) and broadcast with
-// readlane, so a group's gathers need no dependent index load, and the
-// loads of the next group of four slabs are issued before the MFMAs of the
-// current one (two register sets).
-template <typename T, bool NT, bool LATE>
-__global__ __launch_bounds__(kThreads) void mfma_spmm_pipe_kernel(
-    SpmmArgs<T> a, const T* __restrict__ X, T* __restrict__ Y, T* __restrict__ Vout,
-    const T* __restrict__ Sub, int64_t ntiles, const int64_t* __restrict__ tptr,
-    const int32_t* __restrict__ tcols, const T* __restrict__ tvals)
-{
-    using V4 = typename Mfma16<T>::V4;
-    const int lane = threadIdx.x % kWave;
-    const int i0 = lane % 16;
-    const int kq = lane / 16;
-    const int V = gridDim.x < (unsigned int) kXcds ? (int) gridDim.x : kXcds;
-    const int x = blockIdx.x % V;
-    const int64_t span = (ntiles + V - 1) / V;
-    const int64_t b = x * span;
-    const int64_t e = b + span < ntiles ? b + span : ntiles;
-    const unsigned int waves = (unsigned int) ((gridDim.x - x + V - 1) / V) * kWavesPerWG;
-    unsigned int* ticket = a.rb.tickets + 64 * x;
-    unsigned int* done = ticket + 32;
-    // LATE: the per-probe scale of the operand is applied to the product in
-    // the epilogue, s (A x), instead of to every gathered value, A (s x)
     T sq[4];
 #pragma unroll
-    for (int q = 0; q < 4; ++q)
-        sq[q] = (!LATE && a.xscale) ? (T) a.xscale[16 * q + i0] : T(1);
+    for (int q = 0; q < 4; ++q) sq[q] = a.xscale ? (T) a.xscale[16 * q + i0] : T(1);
     const T tt = (T) a.t;
     for (;;)
     {
@@ -870,7 +648,7 @@ __global__ __launch_bounds__(kThreads) void mfma_spmm_pipe_kernel(
                     for (int q = 0; q < 4; ++q) b1[q] = X[c * kBlock + 16 * q + i0];
                 }
 #pragma unroll
-                for (int q = 0; q < 4; ++q) acc[q] = Mfma16<T>::op(a0, LATE ? b0[q] : sq[q] * b0[q], acc[q]);
+                for (int q = 0; q < 4; ++q) acc[q] = Mfma16<T>::op(a0, sq[q] * b0[q], acc[q]);
                 if (g + 2 < ng)
                 {
                     a0 = tvals[(base + 4 * (g + 2)) * 16 + lane];
@@ -879,12 +657,12 @@ __global__ __launch_bounds__(kThreads) void mfma_spmm_pipe_kernel(
                     for (int q = 0; q < 4; ++q) b0[q] = X[c * kBlock + 16 * q + i0];
                 }
 #pragma unroll
-                for (int q = 0; q < 4; ++q) acc[q] = Mfma16<T>::op(a1, LATE ? b1[q] : sq[q] * b1[q], acc[q]);
+                for (int q = 0; q < 4; ++q) acc[q] = Mfma16<T>::op(a1, sq[q] * b1[q], acc[q]);
             }
             if (g < ng)
             {
 #pragma unroll
-                for (int q = 0; q < 4; ++q) acc[q] = Mfma16<T>::op(a0, LATE ? b0[q] : sq[q] * b0[q], acc[q]);
+                for (int q = 0; q < 4; ++q) acc[q] = Mfma16<T>::op(a0, sq[q] * b0[q], acc[q]);
             }
         }
         double red[4] = {0.0, 0.0, 0.0, 0.0};
@@ -894,15 +672,14 @@ __global__ __launch_bounds__(kThreads) void mfma_spmm_pipe_kernel(
             const int p = 16 * q + i0;
             const T sub_c = Sub ? (T) a.sub_coef[p] : T(0);
             const T sub_s = Sub ? (T) a.sub_scale[p] : T(0);
-            const T sp = a.xscale ? (T) a.xscale[p] : T(1);
 #pragma unroll
             for (int m = 0; m < 4; ++m)
             {
                 const int64_t row = t * 16 +
                                     (Mfma16<T>::kRowStride == 4 ? kq + 4 * m : 4 * kq + m);
                 if (row < a.nrows)
-                    spmm_epilogue<T, NT>(a, row, p, LATE ? sp * acc[q][m] : acc[q][m], sp, tt,
-                                         sub_c, sub_s, X, Y, Vout, Sub, red[q]);
+                    spmm_epilogue<T, NT>(a, row, p, acc[q][m], sq[q], tt, sub_c, sub_s, X, Y,
+                                         Vout, Sub, red[q]);
             }
         }
         if (a.red_mode)
@@ -928,22 +705,63 @@ __global__ __launch_bounds__(kThreads) void mfma_spmm_pipe_kernel(
     }
 }
 
-// Sum of the per-tile partial rows in tile order (workgroup g adds a
-// contiguous range, its wavefronts interleaved in a fixed order, then the
-// deterministic grid total), finished like the SpMM's own reduction.
-template <typename T>
-__global__ __launch_bounds__(kThreads) void tile_reduce_kernel(SpmmArgs<T> a, int64_t ntiles)
+// Sum of the per-tile partial rows in tile order, in two launches (the
+// kernel boundary orders them; no fences): tile_sum_kernel gives workgroup g
+// a contiguous range of tiles, its wavefronts interleaved with four
+// accumulators each, combined in a fixed order into rb.partial[g];
+// tile_total_kernel (one workgroup) adds the rows of rb.partial in order and
+// stores the total like the SpMM's own reduction.
+constexpr int kTileSumGrid = 256;
+
+__global__ __launch_bounds__(kThreads) void tile_sum_kernel(const double* __restrict__ tp,
+                                                            int64_t ntiles,
+                                                            double* __restrict__ partial)
 {
+    __shared__ double red[kWavesPerWG][kWave];
     const int lane = threadIdx.x % kWave;
     const int wave = wave_id();
     const int64_t G = gridDim.x;
     const int64_t per = (ntiles + G - 1) / G;
     const int64_t t0 = (int64_t) blockIdx.x * per;
     const int64_t t1 = t0 + per < ntiles ? t0 + per : ntiles;
-    double v = 0.0;
-    for (int64_t t = t0 + wave; t < t1; t += kWavesPerWG)
-        v += a.rb.tile_partial[t * kWave + lane];
-    spmm_finish<T>(a, v, lane);
+    double v0 = 0.0, v1 = 0.0, v2 = 0.0, v3 = 0.0;
+    int64_t t = t0 + wave;
+    for (; t + 3 * kWavesPerWG < t1; t += 4 * kWavesPerWG)
+    {
+        v0 += tp[t * kWave + lane];
+        v1 += tp[(t + kWavesPerWG) * kWave + lane];
+        v2 += tp[(t + 2 * kWavesPerWG) * kWave + lane];
+        v3 += tp[(t + 3 * kWavesPerWG) * kWave + lane];
+    }
+    for (; t < t1; t += kWavesPerWG) v0 += tp[t * kWave + lane];
+    red[wave][lane] = (v0 + v1) + (v2 + v3);
+    __syncthreads();
+    if (wave == 0)
+    {
+        double s = 0.0;
+#pragma unroll
+        for (int w = 0; w < kWavesPerWG; ++w) s += red[w][lane];
+        partial[(int64_t) blockIdx.x * kWave + lane] = s;
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(1024) void tile_total_kernel(SpmmArgs<T> a, int G)
+{
+    __shared__ double red[16][kWave];
+    const int lane = threadIdx.x % kWave;
+    const int grp = threadIdx.x / kWave;
+    double s = 0.0;
+    for (int g = grp; g < G; g += 16) s += a.rb.partial[(int64_t) g * kWave + lane];
+    red[grp][lane] = s;
+    __syncthreads();
+    if (grp == 0)
+    {
+        double t = 0.0;
+#pragma unroll
+        for (int i = 0; i < 16; ++i) t += red[i][lane];
+        spmm_store_total(a, t, lane);
+    }
 }
 
 // ---------------------------------------------------------------- update
@@ -1373,11 +1191,11 @@ int nt_default()
     return v;
 }
 
-// Tiled SpMM variant: 1 = scalar-loaded slabs with the in-order tile
-// schedule, 0 = the same with the static interleaved sweep, 2 / 3 =
-// vector-loaded slabs broadcast by readlane with 8 / 16 gathers in flight
-// (static sweep), 4 / 5 = the vector-loaded forms with the in-order
-// schedule. SLQ_SPMM overrides.
+// Tiled SpMM schedule: 1 (default) = tiles handed out in order (16-row
+// tiles on the matrix cores, 8-row tiles with scalar-loaded slabs), 0 = the
+// static interleaved sweep (tiled_spmm_kernel). SLQ_SPMM overrides; the
+// static form is kept for comparison and for launches without ticket
+// counters.
 int spmm_variant()
 {
     static int v = [] {
@@ -1425,52 +1243,28 @@ void launch_spmm_nt(slq_op* op, const SpmmArgs<T>& s, const SpmmIO<T>& io, const
         const T* tv = (sizeof(T) == 4 && op->A->csr.dtype == SLQ_F64)
                           ? (const T*) tl->vals_f32 : (const T*) tl->vals;
         const int grid = grid_for(tl->ntiles, op->spmm_grid);
-        const int var = spmm_variant();
-        const bool sched = s.rb.tickets && (s.red_mode == 0 || s.rb.tile_partial);
-        if (tl->R == 16 && sched && (var == 1 || (var >= 6 && var <= 9)))
+        const bool sched = spmm_variant() != 0 && s.rb.tickets &&
+                           (s.red_mode == 0 || s.rb.tile_partial);
+        if (sched)
         {
-            auto km = var == 7   ? mfma_spmm_kernel<T, NT, 2>
-                      : var == 6 ? mfma_spmm_kernel<T, NT, 4>
-                      : var == 9 ? mfma_spmm_pipe_kernel<T, NT, true>
-                                 : mfma_spmm_pipe_kernel<T, NT, false>;
-            hipLaunchKernelGGL(km, dim3(grid), dim3(kThreads), 0, op->stream, s, io.X, io.Y,
-                               io.Vout, io.Sub, tl->ntiles, tl->ptr, tl->cols, tv);
+            if (tl->R == 16)
+                hipLaunchKernelGGL((mfma_spmm_kernel<T, NT>), dim3(grid), dim3(kThreads), 0,
+                                   op->stream, s, io.X, io.Y, io.Vout, io.Sub, tl->ntiles,
+                                   tl->ptr, tl->cols, tv);
+            else
+                hipLaunchKernelGGL((tiled_spmm_sched_kernel<T, 8, NT>), dim3(grid),
+                                   dim3(kThreads), 0, op->stream, s, io.X, io.Y, io.Vout,
+                                   io.Sub, tl->ntiles, tl->ptr, tl->cols, tv);
             if (s.red_mode)
             {
                 int rg = (int) ((tl->ntiles + 63) / 64);
-                rg = rg < 8 ? 8 : (rg > 256 ? 256 : rg);
-                hipLaunchKernelGGL((tile_reduce_kernel<T>), dim3(rg), dim3(kThreads), 0,
-                                   op->stream, s, tl->ntiles);
+                rg = rg < 1 ? 1 : (rg > kTileSumGrid ? kTileSumGrid : rg);
+                hipLaunchKernelGGL(tile_sum_kernel, dim3(rg), dim3(kThreads), 0, op->stream,
+                                   (const double*) s.rb.tile_partial, tl->ntiles,
+                                   s.rb.partial);
+                hipLaunchKernelGGL((tile_total_kernel<T>), dim3(1), dim3(1024), 0, op->stream,
+                                   s, rg);
             }
-        }
-        else if ((var == 1 || var >= 4) && sched)
-        {
-            auto k8 = var == 4 ? tiled_spmm_sched_kernel<T, 8, NT, 8>
-                               : (var == 5 ? tiled_spmm_sched_kernel<T, 8, NT, 16>
-                                           : tiled_spmm_sched_kernel<T, 8, NT, 0>);
-            auto k16 = var == 4 ? tiled_spmm_sched_kernel<T, 16, NT, 8>
-                                : (var == 5 ? tiled_spmm_sched_kernel<T, 16, NT, 16>
-                                            : tiled_spmm_sched_kernel<T, 16, NT, 0>);
-            hipLaunchKernelGGL(tl->R == 8 ? k8 : k16, dim3(grid), dim3(kThreads), 0,
-                               op->stream, s, io.X, io.Y, io.Vout, io.Sub, tl->ntiles,
-                               tl->ptr, tl->cols, tv);
-            if (s.red_mode)
-            {
-                int rg = (int) ((tl->ntiles + 63) / 64);
-                rg = rg < 8 ? 8 : (rg > 256 ? 256 : rg);
-                hipLaunchKernelGGL((tile_reduce_kernel<T>), dim3(rg), dim3(kThreads), 0,
-                                   op->stream, s, tl->ntiles);
-            }
-        }
-        else if (spmm_variant() == 2 || spmm_variant() == 3)
-        {
-            const bool u16 = spmm_variant() == 3;
-            auto k = tl->R == 8 ? (u16 ? tiled_spmm_v_kernel<T, 8, NT, 16>
-                                       : tiled_spmm_v_kernel<T, 8, NT, 8>)
-                                : (u16 ? tiled_spmm_v_kernel<T, 16, NT, 16>
-                                       : tiled_spmm_v_kernel<T, 16, NT, 8>);
-            hipLaunchKernelGGL(k, dim3(grid), dim3(kThreads), 0, op->stream, s, io.X, io.Y,
-                               io.Vout, io.Sub, tl->ntiles, tl->ptr, tl->cols, tv);
         }
         else if (tl->R == 8)
             hipLaunchKernelGGL((tiled_spmm_kernel<T, 8, NT>), dim3(grid), dim3(kThreads), 0,
@@ -1947,6 +1741,12 @@ int run_spmm(slq_op* op, double t, int transpose, const void* X, void* Y,
     MatrixStore* Bm = op->kind == OP_AFFINE_B ? op->B.get() : nullptr;
     const int64_t shift_len = A.csr.nrows < A.csr.ncols ? A.csr.nrows : A.csr.ncols;
     const int mode = op->kind == OP_AFFINE_I ? 1 : (op->kind == OP_AFFINE_B ? 2 : 0);
+    // ticket counters of the in-order tile schedule (no reduction here)
+    unsigned int* tickets = nullptr;
+    SLQ_CHECK(hipMalloc(&tickets, sizeof(unsigned int) * kTicketWords));
+    hipLaunchKernelGGL(zero_u32_kernel, dim3(kTicketWords / 64), dim3(64), 0, op->stream,
+                       tickets, kTicketWords);
+    int r;
     if (value_dtype == SLQ_F32)
     {
         SpmmArgs<float> s = spmm_defaults<float>();
@@ -1954,20 +1754,30 @@ int run_spmm(slq_op* op, double t, int transpose, const void* X, void* Y,
         s.shift_mode = mode;
         s.shift_len = shift_len;
         s.t = t;
+        s.rb.tickets = tickets;
         SpmmIO<float> io;
         io.X = (const float*) X;
         io.Y = (float*) Y;
-        return launch_spmm<float>(op, s, io, tiles_for<float>(op, transpose != 0));
+        r = launch_spmm<float>(op, s, io, tiles_for<float>(op, transpose != 0));
     }
-    SpmmArgs<double> s = spmm_defaults<double>();
-    fill_matrix<double>(s, A, Bm, transpose != 0, false);
-    s.shift_mode = mode;
-    s.shift_len = shift_len;
-    s.t = t;
-    SpmmIO<double> io;
-    io.X = (const double*) X;
-    io.Y = (double*) Y;
-    return launch_spmm<double>(op, s, io, tiles_for<double>(op, transpose != 0));
+    else
+    {
+        SpmmArgs<double> s = spmm_defaults<double>();
+        fill_matrix<double>(s, A, Bm, transpose != 0, false);
+        s.shift_mode = mode;
+        s.shift_len = shift_len;
+        s.t = t;
+        s.rb.tickets = tickets;
+        SpmmIO<double> io;
+        io.X = (const double*) X;
+        io.Y = (double*) Y;
+        r = launch_spmm<double>(op, s, io, tiles_for<double>(op, transpose != 0));
+    }
+    const hipError_t e = hipStreamSynchronize(op->stream);
+    (void) hipFree(tickets);
+    if (r != SLQ_OK) return r;
+    SLQ_CHECK(e);
+    return SLQ_OK;
 }
 
 template <typename T>

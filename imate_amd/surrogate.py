@@ -33,10 +33,11 @@ normalised input plane give the root mean square of
 
     rho_i = (surrogate_i - slq_i) / ci_i,
 
-which is about 1 for a surrogate that matches the truth (the SLQ noise
-alone) and grows with the surrogate's own error; the query is trusted when
-it is at most ``trust_rms``. Queries outside the trained box are never
-trusted.
+which is about 1/z = 0.51 (z = 1.96 at 95 %) for a surrogate that matches
+the truth (the SLQ noise alone) and grows with the surrogate's own error;
+the query is trusted when it is at most ``trust_rms`` (default 1.0: the
+surrogate's local error is at most about the SLQ interval). Queries outside
+the trained box are never trusted.
 
 No counterpart in imate, whose surrogates over t are the interpolators of
 imate/interpolator (not part of this package).
@@ -107,7 +108,7 @@ class LogdetSurrogate(object):
 
     def __init__(self, family, scale_range=(0.01, 0.1), t_range=(1e-4, 1e2),
                  lanczos_degree=30, num_samples=128, seed=0,
-                 confidence_level=0.95, trust_neighbors=8, trust_rms=1.5):
+                 confidence_level=0.95, trust_neighbors=8, trust_rms=1.0):
         self.family = family
         self.n = family.n
         self.scale_range = tuple(float(x) for x in scale_range)
