@@ -69,6 +69,21 @@ def spmm_bytes(n, nnz, b=64, vbytes=8):
     return nnz * (vbytes + 4) + (n + 1) * 8 + 2 * n * b * vbytes
 
 
+def measured_traffic(n, nnz):
+    """HBM bytes per SpMM launch measured by rocprofv3 --pmc for this exact
+    workload (profiles/spmm_traffic.json: FETCH_SIZE and WRITE_SIZE of the
+    SpMM kernels, each scaled by the factor the same counter reads for a
+    plain copy kernel of known size), or None when none is recorded."""
+    path = os.path.join(HERE, 'profiles', 'spmm_traffic.json')
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        rec = json.load(f)
+    if rec.get('n') != n or rec.get('nnz') != nnz:
+        return None
+    return rec['bytes_per_launch']
+
+
 def cholesky_reference(n, k, seed):
     path = os.path.join(HERE, 'tests', 'golden', 'cholesky_logdet.json')
     if not os.path.exists(path):
@@ -481,7 +496,8 @@ def main():
                    'samples': s, 'block': 64, 'parallelism': 'single-gpu'},
         'roofline': {'bound': 'hbm', 'achieved': achieved,
                      'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
-                     'frac': achieved / HBM_PEAK_GBS, 'traffic': None,
+                     'frac': achieved / HBM_PEAK_GBS,
+                     'traffic': measured_traffic(n, A.nnz),
                      'kernel': 'mfma_spmm_kernel<double> + tile sums (one '
                                'Lanczos SpMM launch)',
                      'bytes_per_launch': bspmm,
