@@ -28,6 +28,10 @@ SLQ_F32 = 1
 FUNC_LOG = 0
 FUNC_INV = 1
 FUNC_IDENT = 2
+FUNC_EXP = 3
+FUNC_LINFRAC = 4
+FUNC_INDICATOR = 5
+FUNC_GAUSSIAN = 6
 
 PREC_NATIVE = 0
 PREC_MIXED = 1
@@ -47,7 +51,7 @@ SIGNATURES = {
     'slq_op_destroy': (ctypes.c_int, [_P]),
     'slq_op_set_values': (ctypes.c_int, [_P, _P]),
     'slq_estimate': (ctypes.c_int, [
-        _P, _P, _i32, _i32, _i32, _f64, _i32, _i64, _i32, _f64, _i32, _i32,
+        _P, _P, _i32, _i32, _P, _i32, _f64, _i32, _i64, _i32, _f64, _i32, _i32,
         _f64, _f64, _f64, _f64, _i32, _i32, _P, _P, _P, _P, _P, _P, _P, _P,
         _P]),
     'slq_rademacher': (ctypes.c_int, [_i64, _i64, _i64, _i32, ctypes.c_int,
@@ -56,13 +60,13 @@ SIGNATURES = {
     'slq_recurrences': (ctypes.c_int, [_P, _f64, _i32, _i64, _i64, _i32, _i32,
                                        _f64, _i32, _i32, _P, _P, _P]),
     'slq_quadrature': (ctypes.c_int, [_P, _P, _P, _i32, _i32, _i32, _i32,
-                                      _f64, _f64, _P, _i32, _i32, _P, _P,
+                                      _P, _f64, _f64, _P, _i32, _i32, _P, _P,
                                       _P]),
     'slq_profile_enable': (ctypes.c_int, [_P, _i32]),
     'slq_profile_read': (ctypes.c_int, [_P, _P, _P, _P, _P]),
     'slq_copy_bandwidth': (ctypes.c_int, [_i64, _i32, _P]),
     'slq_estimate_batch': (ctypes.c_int, [
-        _P, _i32, _i32, _i32, _f64, _i32, _i64, _i32, _f64, _i32, _f64, _f64,
+        _P, _i32, _i32, _P, _i32, _f64, _i32, _i64, _i32, _f64, _i32, _f64, _f64,
         _i32, _i32, _P, _P, _P, _P, _P]),
     'slq_autotune': (ctypes.c_int, [_P, _i32, _P, _P]),
     'slq_mlp_num_params': (ctypes.c_int, []),

@@ -21,7 +21,26 @@ from .linear_operator import LinearOperator, Matrix
 __all__ = ['trace_estimator']
 
 FUNCTIONS = {'log': _lib.FUNC_LOG, 'inv': _lib.FUNC_INV,
-             'identity': _lib.FUNC_IDENT}
+             'identity': _lib.FUNC_IDENT, 'exp': _lib.FUNC_EXP,
+             'linfrac': _lib.FUNC_LINFRAC, 'indicator': _lib.FUNC_INDICATOR,
+             'gaussian': _lib.FUNC_GAUSSIAN}
+
+
+def function_spec(func):
+    """(code, coefficients) of a function given as a name ('log', 'inv',
+    'identity') or as (name, coefficients) for 'exp' [c], 'linfrac'
+    [a, b, c, d], 'indicator' [a, b] and 'gaussian' [mu, sigma]."""
+    if isinstance(func, str):
+        name, coeff = func, None
+    else:
+        name, coeff = func
+    if name not in FUNCTIONS:
+        raise ValueError('unknown function "%s".' % name)
+    fp = numpy.zeros(4, dtype=numpy.float64)
+    if coeff is not None:
+        c = numpy.atleast_1d(numpy.asarray(coeff, dtype=numpy.float64))
+        fp[:len(c)] = c
+    return FUNCTIONS[name], fp
 
 PRECISIONS = {None: _lib.PREC_NATIVE, 'native': _lib.PREC_NATIVE,
               'mixed': _lib.PREC_MIXED, 'fp32': _lib.PREC_MIXED}
@@ -268,7 +287,8 @@ def trace_estimator(A, parameters, func, gram, exponent, min_num_samples,
                     verbose, plot, gpu, precision=None, block_size=0):
     """SLQ estimate of tr f((A + tB)^p) (trace_estimator.pyx:38-390).
 
-    ``func`` is 'log', 'inv' or 'identity'. ``precision`` ('native' or
+    ``func`` is 'log', 'inv', 'identity' or a (name, coefficients) pair
+    (see ``function_spec``). ``precision`` ('native' or
     'mixed') and ``block_size`` (probes per device block, <= 64) are options
     of this build; their defaults reproduce the reference behaviour.
     """
@@ -344,8 +364,10 @@ def trace_estimator(A, parameters, func, gram, exponent, min_num_samples,
     else:
         params = None
 
+    fcode, fparams = function_spec(func)
     _lib.check(lib.slq_estimate(
-        gpu_Aop.handle, _lib.ptr(params), q, FUNCTIONS[func], int(gram),
+        gpu_Aop.handle, _lib.ptr(params), q, fcode, _lib.ptr(fparams),
+        int(gram),
         float(exponent), int(orthogonalize), int(seed), int(lanczos_degree),
         float(lanczos_tol), int(min_num_samples), int(max_num_samples),
         float(error_atol), float(error_rtol), float(confidence_level),

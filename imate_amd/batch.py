@@ -76,7 +76,8 @@ def slq_batch(matrices, func='log', gram=False, p=1.0, lanczos_degree=20,
     run_time = ctypes.c_double(0)
     setup_time = ctypes.c_double(0)
     _lib.check(lib.slq_estimate_batch(
-        ctypes.cast(arr, ctypes.c_void_p), count, _FUNCS[func], int(gram),
+        ctypes.cast(arr, ctypes.c_void_p), count, _FUNCS[func], None,
+        int(gram),
         float(p), int(orthogonalize), int(seed), int(lanczos_degree),
         float(lanczos_tol), int(num_samples), float(confidence_level),
         float(outlier_significance_level), prec, int(bool(use_graph)),

@@ -845,8 +845,19 @@ extern "C" int slq_autotune(slq_op* op, int32_t precision, int32_t* spmm_grid,
 
 // ---------------------------------------------------------------- batch
 
+// A valid matrix function: a known code, with coefficients where the code
+// takes them (a positive sigma for the Gaussian).
+bool func_ok(int func, const double* fp)
+{
+    if (func < SLQ_FUNC_LOG || func > SLQ_FUNC_GAUSSIAN) return false;
+    if (func >= SLQ_FUNC_EXP && !fp) return false;
+    if (func == SLQ_FUNC_GAUSSIAN && !(fp[1] > 0.0)) return false;
+    return true;
+}
+
 extern "C" int slq_estimate_batch(slq_op** ops, int32_t count, int32_t func,
-                                  int32_t gram, double exponent, int32_t orthogonalize,
+                                  const double* func_params, int32_t gram,
+                                  double exponent, int32_t orthogonalize,
                                   int64_t seed, int32_t m, double tol,
                                   int32_t num_samples, double confidence,
                                   double outlier, int32_t precision, int32_t use_graph,
@@ -854,7 +865,7 @@ extern "C" int slq_estimate_batch(slq_op** ops, int32_t count, int32_t func,
                                   double* run_time, double* setup_time)
 {
     if (!ops || count < 1 || m < 1 || num_samples < 1 || !trace || !error ||
-        func < SLQ_FUNC_LOG || func > SLQ_FUNC_IDENT ||
+        !func_ok(func, func_params) ||
         (precision != SLQ_PREC_NATIVE && precision != SLQ_PREC_MIXED))
     {
         set_error("slq_estimate_batch: invalid argument");
@@ -911,8 +922,8 @@ extern "C" int slq_estimate_batch(slq_op** ops, int32_t count, int32_t func,
             SLQ_TRY(recurrence_block(op, p, 0.0, gram, tol, orthogonalize, precision));
             SLQ_TRY(run_quadrature(op->stream, at<double>(op, p.off_alpha),
                                    at<double>(op, p.off_beta), at<int32_t>(op, p.off_sizes),
-                                   cnt, m, gram, func, exponent, (double) p.len, nullptr, 1, 0,
-                                   0, 1, dsamp + s0, nullptr, nullptr));
+                                   cnt, m, gram, func, func_params, exponent, (double) p.len,
+                                   nullptr, 1, 0, 0, 1, dsamp + s0, nullptr, nullptr));
         }
         return SLQ_OK;
     };
@@ -1211,7 +1222,8 @@ int slq_op_destroy(slq_op* op)
 }
 
 int slq_estimate(slq_op* op, const double* params, int32_t q, int32_t func,
-                 int32_t gram, double exponent, int32_t orthogonalize,
+                 const double* func_params, int32_t gram, double exponent,
+                 int32_t orthogonalize,
                  int64_t seed, int32_t m, double tol, int32_t min_s,
                  int32_t max_s, double atol, double rtol, double confidence,
                  double outlier, int32_t block_size, int32_t precision,
@@ -1222,7 +1234,7 @@ int slq_estimate(slq_op* op, const double* params, int32_t q, int32_t func,
 {
     if (!op || q < 1 || m < 1 || min_s < 1 || max_s < min_s || !trace || !error ||
         !samples || !processed || !num_used || !num_outliers || !converged ||
-        func < SLQ_FUNC_LOG || func > SLQ_FUNC_IDENT ||
+        !func_ok(func, func_params) ||
         (precision != SLQ_PREC_NATIVE && precision != SLQ_PREC_MIXED))
     {
         set_error("slq_estimate: invalid argument");
@@ -1286,7 +1298,7 @@ int slq_estimate(slq_op* op, const double* params, int32_t q, int32_t func,
             SLQ_TRY(recurrence_block(op, p, t, gram, tol, orthogonalize, precision));
             SLQ_TRY(run_quadrature(op->stream, at<double>(op, p.off_alpha),
                                    at<double>(op, p.off_beta), at<int32_t>(op, p.off_sizes),
-                                   cnt, m, gram, func, exponent, nq,
+                                   cnt, m, gram, func, func_params, exponent, nq,
                                    relation ? params_dev : nullptr, q, relation ? 1 : 0, j,
                                    q, at<double>(op, p.off_samples), nullptr, nullptr));
         }
@@ -1445,11 +1457,12 @@ int slq_recurrences(slq_op* op, double t, int32_t gram, int64_t seed,
 
 int slq_quadrature(const double* alpha, const double* beta, const int32_t* sizes,
                    int32_t count, int32_t m, int32_t gram, int32_t func,
-                   double exponent, double n, const double* params, int32_t q,
-                   int32_t shift, double* samples, double* theta, double* tau)
+                   const double* func_params, double exponent, double n,
+                   const double* params, int32_t q, int32_t shift, double* samples,
+                   double* theta, double* tau)
 {
     if (!alpha || !beta || !sizes || count < 1 || count > kBlock || m < 1 || q < 1 ||
-        !samples || (shift && !params))
+        !samples || (shift && !params) || !func_ok(func, func_params))
     {
         set_error("slq_quadrature: invalid argument (count must be 1..64)");
         return SLQ_E_ARG;
@@ -1480,7 +1493,7 @@ int slq_quadrature(const double* alpha, const double* beta, const int32_t* sizes
     SLQ_CHECK(hipMemcpy(db, hb.data(), sizeof(double) * hb.size(), hipMemcpyHostToDevice));
     SLQ_CHECK(hipMemcpy(ds, hs.data(), sizeof(int32_t) * kBlock, hipMemcpyHostToDevice));
     if (params) SLQ_CHECK(hipMemcpy(dp, params, sizeof(double) * q, hipMemcpyHostToDevice));
-    int r = run_quadrature(0, da, db, ds, count, m, gram, func, exponent, n,
+    int r = run_quadrature(0, da, db, ds, count, m, gram, func, func_params, exponent, n,
                            params ? dp : nullptr, q, shift, 0, qs, dsmp, dth, dta);
     if (r == SLQ_OK)
     {

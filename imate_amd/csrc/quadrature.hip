@@ -42,6 +42,7 @@ struct QuadArgs {
     int m;
     int gram;
     int func;
+    double fp[4];             // coefficients of f
     double exponent;
     double n;
     const double* params;     // [q] (device) or nullptr
@@ -59,11 +60,26 @@ __device__ __forceinline__ double fpow(double x, double p)
     return p == 1.0 ? x : pow(x, p);
 }
 
-__device__ __forceinline__ double ffunc(int func, double x)
+// f of the matrix functions (imate/functions/*.cpp): logarithm, inverse,
+// identity, exponential exp(c x), homographic (a x + b) / (c x + d),
+// indicator of [a, b], Gaussian density with mean mu and deviation sigma.
+__device__ __forceinline__ double ffunc(int func, const double* fp, double x)
 {
-    if (func == SLQ_FUNC_LOG) return log(x);
-    if (func == SLQ_FUNC_INV) return 1.0 / x;
-    return x;
+    switch (func)
+    {
+        case SLQ_FUNC_LOG: return log(x);
+        case SLQ_FUNC_INV: return 1.0 / x;
+        case SLQ_FUNC_EXP: return exp(x * fp[0]);
+        case SLQ_FUNC_LINFRAC: return (fp[0] * x + fp[1]) / (fp[2] * x + fp[3]);
+        case SLQ_FUNC_INDICATOR: return (x < fp[0] || x > fp[1]) ? 0.0 : 1.0;
+        case SLQ_FUNC_GAUSSIAN:
+        {
+            const double z = (x - fp[0]) / fp[1];
+            // 1 / (sigma sqrt(2 pi)) written as imate writes it
+            return (0.5 * M_SQRT1_2 * M_2_SQRTPI / fp[1]) * exp(-0.5 * z * z);
+        }
+        default: return x;
+    }
 }
 
 // Implicit QL on (d, e) of size K (e[i] couples i and i+1, e[K-1] = 0),
@@ -209,7 +225,7 @@ __global__ __launch_bounds__(kQuadThreads) void quadrature_kernel(QuadArgs a)
             double th = d[i * S];
             if (a.shift) th = (th - t0) + tj;
             const double tau = z[i * S];
-            sum += tau * tau * ffunc(a.func, fpow(th, a.exponent));
+            sum += tau * tau * ffunc(a.func, a.fp, fpow(th, a.exponent));
         }
         const int col = a.shift ? j : a.j_out;
         a.samples[(size_t) p * a.q_stride + col] = info == 0 ? a.n * sum : NAN;
@@ -220,7 +236,8 @@ __global__ __launch_bounds__(kQuadThreads) void quadrature_kernel(QuadArgs a)
 
 int run_quadrature(hipStream_t s, const double* alpha, const double* beta,
                    const int32_t* sizes, int count, int m, int gram, int func,
-                   double exponent, double n, const double* params_dev, int q,
+                   const double* func_params, double exponent, double n,
+                   const double* params_dev, int q,
                    int shift, int j_out, int q_stride, double* samples_dev,
                    double* theta_dev, double* tau_dev)
 {
@@ -232,6 +249,7 @@ int run_quadrature(hipStream_t s, const double* alpha, const double* beta,
     a.m = m;
     a.gram = gram;
     a.func = func;
+    for (int i = 0; i < 4; ++i) a.fp[i] = func_params ? func_params[i] : 0.0;
     a.exponent = exponent;
     a.n = n;
     a.params = params_dev;

@@ -181,6 +181,7 @@ size_t recurrence_scratch_bytes(const slq_op* op, int gram, int m, int ortho,
 // Device eigensolve + quadrature for `count` probes.
 int run_quadrature(hipStream_t s, const double* alpha, const double* beta,
                    const int32_t* sizes, int count, int m, int gram, int func,
+                   const double* func_params,
                    double exponent, double n, const double* params_dev, int q,
                    int shift, int j_out, int q_stride, double* samples_dev,
                    double* theta_dev, double* tau_dev);

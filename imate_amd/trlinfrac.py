@@ -1,0 +1,37 @@
+"""``trlinfrac(A, gram, p, coeff, return_info, method, **options)``
+(imate/trlinfrac/trlinfrac.py:22-383).
+
+tr of the linear fractional transformation (a A^p + b I)(c A^p + d I)^-1 by
+stochastic Lanczos quadrature with f(x) = (a x + b) / (c x + d)
+(imate/functions/homographic.cpp; trlinfrac/_slq_method.pyx:25-880). Only
+``method='slq'`` is part of this build.
+"""
+
+import numpy
+
+from ._slq import slq_method
+
+
+def _check_arguments(coeff, return_info):
+    """trlinfrac/_slq_method.pyx:864-880."""
+    if not isinstance(coeff, (tuple, list, numpy.ndarray)):
+        raise TypeError('"coeff" should be a tuple, list, or numpy array.')
+    elif len(coeff) != 4:
+        raise ValueError('"coeff" should be an array or list of length 4.')
+    if coeff[0] * coeff[3] == coeff[1] * coeff[2]:
+        raise ValueError('"coeff" of the linear fractional transformation ' +
+                         'are degenerate.')
+    if not isinstance(return_info, bool):
+        raise TypeError('"return_info" should be boolean.')
+
+
+def trlinfrac(A, gram=False, p=1.0, coeff=[1.0, 0.0, 1.0, 1.0],
+              return_info=False, method='eigenvalue', **options):
+    if method == 'slq':
+        _check_arguments(coeff, return_info)
+        return slq_method(('linfrac', [float(c) for c in coeff]), A,
+                          gram=gram, p=p, return_info=return_info, **options)
+    if method == 'eigenvalue':
+        raise NotImplementedError(
+            'Method "%s" is not part of imate_amd; use method="slq".' % method)
+    raise ValueError('Method "%s" is invalid.' % (method))

@@ -43,6 +43,14 @@ extern "C" {
 #define SLQ_FUNC_LOG 0      /* functions/logarithm.cpp:42-45 (logdet) */
 #define SLQ_FUNC_INV 1      /* functions/inverse.cpp:41-44   (traceinv) */
 #define SLQ_FUNC_IDENT 2    /* functions/identity.cpp:41-44  (trace) */
+/* The functions below take coefficients in func_params[]:              */
+#define SLQ_FUNC_EXP 3      /* exp(c x), c = [0]; exponential.cpp (trexp)      */
+#define SLQ_FUNC_LINFRAC 4  /* (a x + b) / (c x + d), [0..3]; homographic.cpp
+                               (trlinfrac)                                    */
+#define SLQ_FUNC_INDICATOR 5 /* 1 on [a, b], else 0, [0..1]; indicator.cpp
+                               (eigencount)                                   */
+#define SLQ_FUNC_GAUSSIAN 6 /* normal density, mu = [0], sigma = [1];
+                               gaussian.cpp (density)                         */
 
 /* Precision modes of the recurrences. */
 #define SLQ_PREC_NATIVE 0   /* vectors and SpMM in the matrix dtype, FP64 recurrences */
@@ -100,6 +108,8 @@ int slq_op_destroy(slq_op* op);
  * imate's trace_estimator (imate/_trace_estimator/trace_estimator.pyx:38-390):
  *   params[num_inquiries]  t values of an affine operator (ignored otherwise)
  *   func, exponent         f and p of f(theta^p)
+ *   func_params            coefficients of f (SLQ_FUNC_EXP .. GAUSSIAN;
+ *                          may be NULL for log, inverse and identity)
  *   orthogonalize          0 none, <0 or >= lanczos_degree full, else window
  *   seed                   >= 0 reproducible; < 0 uses the clock
  * Probe vectors are the Rademacher vectors of imate's thread-0 xoshiro256**
@@ -113,7 +123,8 @@ int slq_op_destroy(slq_op* op);
  *   num_samples_used[q], num_outliers[q], converged[q], *all_converged,
  *   *alg_wall_time (seconds, the sampling loop only). */
 int slq_estimate(slq_op* op, const double* params, int32_t num_inquiries,
-                 int32_t func, int32_t gram, double exponent,
+                 int32_t func, const double* func_params, int32_t gram,
+                 double exponent,
                  int32_t orthogonalize, int64_t seed, int32_t lanczos_degree,
                  double lanczos_tol, int32_t min_num_samples,
                  int32_t max_num_samples, double error_atol, double error_rtol,
@@ -134,8 +145,9 @@ int slq_estimate(slq_op* op, const double* params, int32_t num_inquiries,
  * time (s), *setup_time the preparation and capture time before it.
  * Replaces a Python loop of imate.logdet(..., method='slq', gpu=True) calls
  * over separate matrices; imate has no batched entry point. */
-int slq_estimate_batch(slq_op** ops, int32_t count, int32_t func, int32_t gram,
-                       double exponent, int32_t orthogonalize, int64_t seed,
+int slq_estimate_batch(slq_op** ops, int32_t count, int32_t func,
+                       const double* func_params, int32_t gram, double exponent,
+                       int32_t orthogonalize, int64_t seed,
                        int32_t lanczos_degree, double lanczos_tol,
                        int32_t num_samples, double confidence_level,
                        double outlier_significance_level, int32_t precision,
@@ -213,8 +225,9 @@ int slq_recurrences(slq_op* op, double t, int32_t gram, int64_t seed,
  * quadrature of cu_trace_estimator.cu:~555-630. */
 int slq_quadrature(const double* alpha, const double* beta,
                    const int32_t* sizes, int32_t count, int32_t m,
-                   int32_t gram, int32_t func, double exponent, double n,
-                   const double* params, int32_t q, int32_t shift,
+                   int32_t gram, int32_t func, const double* func_params,
+                   double exponent, double n, const double* params, int32_t q,
+                   int32_t shift,
                    double* samples, double* theta, double* tau);
 
 /* Per-kernel timing with HIP events on the handle's stream.

@@ -323,12 +323,25 @@ def svd_bidiagonal(alpha, beta, k):
 
 # ---------------------------------------------------------------- quadrature
 
-FUNCTIONS = ('log', 'inv', 'identity')
+FUNCTIONS = ('log', 'inv', 'identity', 'exp', 'linfrac', 'indicator',
+             'gaussian')
+
+# M_SQRT1_2 and M_2_SQRTPI of <math.h>, as imate's gaussian.cpp uses them
+_SQRT1_2 = 0.70710678118654752440
+_2_SQRTPI = 1.12837916709551257390
 
 
-def _fn64(name, x):
-    """functions/{logarithm,inverse,identity}.cpp on a double, with the C
-    library's behaviour at the poles."""
+def _spec(func):
+    """(name, coefficients) of ``func``: a name, or (name, coefficients)."""
+    if isinstance(func, str):
+        return func, None
+    return func[0], [float(c) for c in func[1]]
+
+
+def _fn64(name, x, fp=None):
+    """functions/{logarithm,inverse,identity,exponential,homographic,
+    indicator,gaussian}.cpp on a double, with the C library's behaviour at
+    the poles."""
     if name == 'log':
         if x > 0.0:
             return math.log(x)
@@ -337,6 +350,48 @@ def _fn64(name, x):
         if x == 0.0:
             return math.copysign(math.inf, x)
         return 1.0 / x
+    if name == 'exp':
+        # exponential.cpp:44-47: exp(lambda * coeff)
+        try:
+            return math.exp(x * fp[0])
+        except OverflowError:
+            return math.inf
+    if name == 'linfrac':
+        # homographic.cpp:53-61: (a lambda + b) / (c lambda + d)
+        num = fp[0] * x + fp[1]
+        den = fp[2] * x + fp[3]
+        if den == 0.0:
+            return math.nan if num == 0.0 else math.copysign(math.inf, num)
+        return num / den
+    if name == 'indicator':
+        # indicator.cpp:51-61: 0 outside [a, b]
+        return 0.0 if (x < fp[0] or x > fp[1]) else 1.0
+    if name == 'gaussian':
+        # gaussian.cpp:59-63
+        z = (x - fp[0]) / fp[1]
+        return (0.5 * _SQRT1_2 * _2_SQRTPI / fp[1]) * math.exp(-0.5 * z * z)
+    return x
+
+
+def _fn32(name, x, fp=None):
+    """The same in single precision (the float overloads)."""
+    f32 = numpy.float32
+    if name == 'log':
+        return numpy.log(x)
+    if name == 'inv':
+        return f32(1.0) / x
+    if name == 'exp':
+        return numpy.exp(x * f32(fp[0]))
+    if name == 'linfrac':
+        a, b, c, d = (f32(v) for v in fp)
+        return (a * x + b) / (c * x + d)
+    if name == 'indicator':
+        return f32(0.0) if (x < fp[0] or x > fp[1]) else f32(1.0)
+    if name == 'gaussian':
+        mu, sigma = f32(fp[0]), f32(fp[1])
+        z = (x - mu) / sigma
+        return f32((0.5 * _SQRT1_2 * _2_SQRTPI / sigma) *
+                   numpy.exp(f32(-0.5) * z * z))
     return x
 
 
@@ -351,24 +406,21 @@ def _pow64(x, p):
 
 def quadrature(theta, tau, k, func, exponent, n, dtype=numpy.float64):
     """``trace_estimate = n * sum_{i<k} tau_i^2 f(theta_i^p)``,
-    c_trace_estimator.cpp:597-621 (sequential sum in the matrix dtype)."""
+    c_trace_estimator.cpp:597-621 (sequential sum in the matrix dtype).
+    ``func`` is a name or (name, coefficients)."""
+    name, fp = _spec(func)
     if numpy.dtype(dtype) == numpy.float64:
         s = 0.0
         for i in range(k):
             s += float(tau[i]) * float(tau[i]) * _fn64(
-                func, _pow64(float(theta[i]), float(exponent)))
+                name, _pow64(float(theta[i]), float(exponent)), fp)
         return n * s
     f32 = numpy.float32
     with numpy.errstate(all='ignore'):
         s = f32(0.0)
         for i in range(k):
             x = numpy.power(f32(theta[i]), f32(exponent))
-            if func == 'log':
-                fx = numpy.log(x)
-            elif func == 'inv':
-                fx = f32(1.0) / x
-            else:
-                fx = x
+            fx = _fn32(name, x, fp)
             s = f32(s + f32(f32(tau[i]) * f32(tau[i])) * fx)
         return f32(n) * s
 
