@@ -50,6 +50,10 @@ SIGNATURES = {
     'slq_op_create_affine': (ctypes.c_int, [_P, _P, _P]),
     'slq_op_destroy': (ctypes.c_int, [_P]),
     'slq_op_set_values': (ctypes.c_int, [_P, _P]),
+    'slq_op_create_sparse_gaussian': (ctypes.c_int, [_i64, _i64, _i32, _i64,
+                                                     ctypes.c_int, _P]),
+    'slq_op_get_csr': (ctypes.c_int, [_P, _P, _P, _P]),
+    'slq_op_shape': (ctypes.c_int, [_P, _P, _P, _P, _P]),
     'slq_estimate': (ctypes.c_int, [
         _P, _P, _i32, _i32, _P, _i32, _f64, _i32, _i64, _i32, _f64, _i32, _i32,
         _f64, _f64, _f64, _f64, _i32, _i32, _P, _P, _P, _P, _P, _P, _P, _P,

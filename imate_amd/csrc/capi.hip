@@ -314,18 +314,233 @@ __global__ void set_neg_kernel(int64_t* p, int64_t n)
     if (i < n) p[i] = -1;
 }
 
+__global__ void fill_u32_kernel(unsigned int* p, int64_t n)
+{
+    const int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+// Chunked transpose (2^31 non-zeros and more): column counts, then per range
+// of columns the entries of each row inside the range (a contiguous run of
+// the sorted row), stably sorted by column.
+__global__ void count_cols_kernel(const int32_t* __restrict__ cols, int64_t nnz,
+                                  unsigned int* __restrict__ counts)
+{
+    for (int64_t k = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; k < nnz;
+         k += (int64_t) gridDim.x * blockDim.x)
+        atomicAdd(&counts[cols[k]], 1u);
+}
+
+__device__ __forceinline__ int64_t lower_bound_dev(const int32_t* a, int64_t lo, int64_t hi,
+                                                   int32_t v)
+{
+    while (lo < hi)
+    {
+        const int64_t mid = lo + (hi - lo) / 2;
+        if (a[mid] < v) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__global__ void row_range_kernel(const int64_t* __restrict__ indptr,
+                                 const int32_t* __restrict__ cols, int64_t nrows,
+                                 int32_t c0, int32_t c1, int64_t* __restrict__ lo,
+                                 int64_t* __restrict__ cnt)
+{
+    const int64_t r = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrows) return;
+    const int64_t a = lower_bound_dev(cols, indptr[r], indptr[r + 1], c0);
+    const int64_t b = lower_bound_dev(cols, a, indptr[r + 1], c1);
+    lo[r] = a;
+    cnt[r] = b - a;
+}
+
+__global__ void chunk_scatter_kernel(const int32_t* __restrict__ cols,
+                                     const int64_t* __restrict__ lo,
+                                     const int64_t* __restrict__ cnt,
+                                     const int64_t* __restrict__ off, int64_t nrows, int32_t c0,
+                                     int32_t* __restrict__ keys, int64_t* __restrict__ pos)
+{
+    const int64_t r = (int64_t) blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+    const int lane = threadIdx.x % 64;
+    if (r >= nrows) return;
+    const int64_t a = lo[r], o = off[r];
+    for (int64_t i = lane; i < cnt[r]; i += 64)
+    {
+        keys[o + i] = cols[a + i] - c0;
+        pos[o + i] = a + i;
+    }
+}
+
+template <typename T>
+__global__ void chunk_gather_kernel(const int64_t* __restrict__ pos, int64_t m,
+                                    const int64_t* __restrict__ indptr, int64_t nrows,
+                                    const T* __restrict__ data, int32_t* __restrict__ tind,
+                                    T* __restrict__ tdata)
+{
+    for (int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < m;
+         i += (int64_t) gridDim.x * blockDim.x)
+    {
+        const int64_t p = pos[i];
+        // row of entry p: the last row whose pointer is <= p
+        int64_t lo = 0, hi = nrows;
+        while (hi - lo > 1)
+        {
+            const int64_t mid = lo + (hi - lo) / 2;
+            if (indptr[mid] <= p) lo = mid;
+            else hi = mid;
+        }
+        tind[i] = (int32_t) lo;
+        tdata[i] = data[p];
+    }
+}
+
 }  // namespace
 
 // CSR of A^T by a stable radix sort of the column keys: within a column the
 // rows stay ascending, so the transpose is deterministic.
+// Transpose of a matrix with 2^31 non-zeros or more, in ranges of columns
+// holding at most 2^30 entries each (the radix sort counts in int).
+static int build_transpose_chunked(hipStream_t s, const DevCSR& A, DevCSR& T)
+{
+    const int64_t nnz = A.nnz;
+    const size_t vb = A.dtype == SLQ_F64 ? 8 : 4;
+    T.nrows = A.ncols;
+    T.ncols = A.nrows;
+    T.nnz = nnz;
+    T.dtype = A.dtype;
+    SLQ_CHECK(hipMalloc(&T.indptr, sizeof(int64_t) * (T.nrows + 1)));
+    SLQ_CHECK(hipMalloc(&T.indices, sizeof(int32_t) * nnz));
+    SLQ_CHECK(hipMalloc(&T.data, vb * nnz));
+    // column counts -> pointers (host scan)
+    unsigned int* dcnt = nullptr;
+    SLQ_CHECK(hipMalloc(&dcnt, sizeof(unsigned int) * A.ncols));
+    hipLaunchKernelGGL(fill_u32_kernel, dim3((unsigned) ((A.ncols + 255) / 256)), dim3(256),
+                       0, s, dcnt, A.ncols);
+    hipLaunchKernelGGL(count_cols_kernel, dim3(4096), dim3(256), 0, s, A.indices, nnz, dcnt);
+    std::vector<unsigned int> cnt(A.ncols);
+    SLQ_CHECK(hipMemcpyAsync(cnt.data(), dcnt, sizeof(unsigned int) * A.ncols,
+                             hipMemcpyDeviceToHost, s));
+    SLQ_CHECK(hipStreamSynchronize(s));
+    (void) hipFree(dcnt);
+    std::vector<int64_t> ptr(A.ncols + 1, 0);
+    for (int64_t c = 0; c < A.ncols; ++c) ptr[c + 1] = ptr[c] + cnt[c];
+    SLQ_CHECK(hipMemcpy(T.indptr, ptr.data(), sizeof(int64_t) * (A.ncols + 1),
+                        hipMemcpyHostToDevice));
+    // per-row scratch and the largest chunk's sort buffers; SLQ_TRANSPOSE_CHUNK
+    // sets a smaller chunk (entries) to exercise the chunked path
+    int64_t kChunk = (int64_t) 1 << 30;
+    if (const char* env = std::getenv("SLQ_TRANSPOSE_CHUNK"))
+        kChunk = std::max<int64_t>(1, std::min<int64_t>(kChunk, std::atoll(env)));
+    int64_t *lo = nullptr, *rc = nullptr, *off = nullptr;
+    SLQ_CHECK(hipMalloc(&lo, sizeof(int64_t) * A.nrows));
+    SLQ_CHECK(hipMalloc(&rc, sizeof(int64_t) * A.nrows));
+    SLQ_CHECK(hipMalloc(&off, sizeof(int64_t) * A.nrows));
+    int32_t *keys = nullptr, *keys2 = nullptr;
+    int64_t *pos = nullptr, *pos2 = nullptr;
+    void* tmp = nullptr;
+    size_t tmp_bytes = 0, scan_bytes = 0;
+    int r = SLQ_OK;
+    int64_t c0 = 0;
+    while (c0 < A.ncols && r == SLQ_OK)
+    {
+        int64_t c1 = c0 + 1;
+        while (c1 < A.ncols && ptr[c1 + 1] - ptr[c0] <= kChunk) ++c1;
+        const int64_t m = ptr[c1] - ptr[c0];
+        if (m > kChunk)
+        {
+            set_error("transpose: one column holds more than 2^30 entries");
+            r = SLQ_E_ARG;
+            break;
+        }
+        if (m > 0)
+        {
+            if (!keys)
+            {
+                // sized for the largest chunk, found by a dry pass of the
+                // same greedy split
+                int64_t big = 0, a = 0;
+                while (a < A.ncols)
+                {
+                    int64_t b = a + 1;
+                    while (b < A.ncols && ptr[b + 1] - ptr[a] <= kChunk) ++b;
+                    big = std::max<int64_t>(big, ptr[b] - ptr[a]);
+                    a = b;
+                }
+                hipError_t e;
+                if ((e = hipMalloc(&keys, sizeof(int32_t) * big)) != hipSuccess ||
+                    (e = hipMalloc(&keys2, sizeof(int32_t) * big)) != hipSuccess ||
+                    (e = hipMalloc(&pos, sizeof(int64_t) * big)) != hipSuccess ||
+                    (e = hipMalloc(&pos2, sizeof(int64_t) * big)) != hipSuccess)
+                {
+                    r = hip_status(e, "transpose buffers", __FILE__, __LINE__);
+                    break;
+                }
+                size_t sb = 0;
+                (void) hipcub::DeviceRadixSort::SortPairs(nullptr, sb, keys, keys2, pos, pos2,
+                                                          (int) big, 0, 32, s);
+                (void) hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, rc, off,
+                                                        (int) A.nrows, s);
+                tmp_bytes = std::max(sb, scan_bytes);
+                if ((e = hipMalloc(&tmp, tmp_bytes > 0 ? tmp_bytes : 1)) != hipSuccess)
+                {
+                    r = hip_status(e, "transpose buffers", __FILE__, __LINE__);
+                    break;
+                }
+            }
+            const unsigned gr = (unsigned) ((A.nrows + 255) / 256);
+            hipLaunchKernelGGL(row_range_kernel, dim3(gr), dim3(256), 0, s, A.indptr,
+                               A.indices, A.nrows, (int32_t) c0, (int32_t) c1, lo, rc);
+            size_t sb = tmp_bytes;
+            hipError_t e = hipcub::DeviceScan::ExclusiveSum(tmp, sb, rc, off, (int) A.nrows, s);
+            if (e != hipSuccess)
+            {
+                r = hip_status(e, "transpose scan", __FILE__, __LINE__);
+                break;
+            }
+            hipLaunchKernelGGL(chunk_scatter_kernel, dim3((unsigned) ((A.nrows + 3) / 4)),
+                               dim3(256), 0, s, A.indices, lo, rc, off, A.nrows,
+                               (int32_t) c0, keys, pos);
+            int end_bit = 1;
+            while (end_bit < 31 && ((int64_t) 1 << end_bit) < c1 - c0) ++end_bit;
+            sb = tmp_bytes;
+            e = hipcub::DeviceRadixSort::SortPairs(tmp, sb, keys, keys2, pos, pos2, (int) m, 0,
+                                                   end_bit, s);
+            if (e != hipSuccess)
+            {
+                r = hip_status(e, "transpose sort", __FILE__, __LINE__);
+                break;
+            }
+            if (A.dtype == SLQ_F64)
+                hipLaunchKernelGGL(chunk_gather_kernel<double>, dim3(8192), dim3(256), 0, s,
+                                   pos2, m, A.indptr, A.nrows, (const double*) A.data,
+                                   T.indices + ptr[c0], (double*) T.data + ptr[c0]);
+            else
+                hipLaunchKernelGGL(chunk_gather_kernel<float>, dim3(8192), dim3(256), 0, s,
+                                   pos2, m, A.indptr, A.nrows, (const float*) A.data,
+                                   T.indices + ptr[c0], (float*) T.data + ptr[c0]);
+            e = hipStreamSynchronize(s);
+            if (e != hipSuccess) r = hip_status(e, "transpose", __FILE__, __LINE__);
+        }
+        c0 = c1;
+    }
+    (void) hipFree(lo);
+    (void) hipFree(rc);
+    (void) hipFree(off);
+    (void) hipFree(keys);
+    (void) hipFree(keys2);
+    (void) hipFree(pos);
+    (void) hipFree(pos2);
+    (void) hipFree(tmp);
+    return r;
+}
+
 int build_transpose(hipStream_t s, const DevCSR& A, DevCSR& T)
 {
     const int64_t nnz = A.nnz;
-    if (nnz >= (int64_t) 1 << 31)
-    {
-        set_error("transpose of a matrix with >= 2^31 non-zeros is not supported");
-        return SLQ_E_ARG;
-    }
+    if (nnz >= (int64_t) 1 << 31 || std::getenv("SLQ_TRANSPOSE_CHUNK"))
+        return build_transpose_chunked(s, A, T);
     const size_t vb = A.dtype == SLQ_F64 ? 8 : 4;
     T.nrows = A.ncols;
     T.ncols = A.nrows;
@@ -1135,6 +1350,146 @@ int slq_op_create_csr(int64_t nrows, int64_t ncols, int64_t nnz,
         }
     }
     *out = op;
+    return SLQ_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Counter-based hash: the SplitMix64 output function of a combination of the
+// seed, the row and the stratum (Steele, Lea and Flood 2014).
+__device__ __forceinline__ uint64_t mix64(uint64_t z)
+{
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+
+// Row r of the sparse Gaussian matrix has one entry in each of k column
+// strata [floor(s n / k), floor((s + 1) n / k)): a hashed column of the
+// stratum (so the k columns are distinct and ascending) and an N(0, 1)
+// value by the Box-Muller transform of two hashed uniforms.
+template <typename T>
+__global__ void sparse_gaussian_kernel(int64_t nrows, int64_t ncols, int32_t k, uint64_t seed,
+                                       int32_t* __restrict__ indices, T* __restrict__ data)
+{
+    const int64_t total = nrows * k;
+    for (int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < total;
+         i += (int64_t) gridDim.x * blockDim.x)
+    {
+        const int64_t r = i / k;
+        const int64_t st = i - r * k;
+        const int64_t c0 = st * ncols / k;
+        const int64_t c1 = (st + 1) * ncols / k;
+        const uint64_t key = seed * 0x9e3779b97f4a7c15ull + (uint64_t) r * 0xd1b54a32d192ed03ull +
+                             (uint64_t) st;
+        const uint64_t h0 = mix64(key);
+        const uint64_t h1 = mix64(key ^ 0x5851f42d4c957f2dull);
+        const uint64_t h2 = mix64(key ^ 0x14057b7ef767814full);
+        indices[i] = (int32_t) (c0 + (int64_t) (h0 % (uint64_t) (c1 - c0)));
+        const double u1 = (double) ((h1 >> 11) + 1) * 0x1.0p-53;   // (0, 1]
+        const double u2 = (double) (h2 >> 11) * 0x1.0p-53;         // [0, 1)
+        data[i] = (T) (sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2));
+    }
+}
+
+__global__ void row_pointers_kernel(int64_t* __restrict__ indptr, int64_t nrows, int32_t k)
+{
+    const int64_t r = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (r <= nrows) indptr[r] = r * k;
+}
+
+}  // namespace
+
+extern "C" {
+
+int slq_op_create_sparse_gaussian(int64_t nrows, int64_t ncols, int32_t per_row, int64_t seed,
+                                  int dtype, slq_op** out)
+{
+    if (!out || nrows < 1 || ncols < 1 || ncols > INT32_MAX || per_row < 1 ||
+        per_row > ncols || (dtype != SLQ_F64 && dtype != SLQ_F32))
+    {
+        set_error("slq_op_create_sparse_gaussian: invalid argument");
+        return SLQ_E_ARG;
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+    {
+        set_error("no HIP device found");
+        return SLQ_E_NODEVICE;
+    }
+    slq_op* op = new (std::nothrow) slq_op();
+    if (!op)
+    {
+        set_error("host allocation failed");
+        return SLQ_E_OOM;
+    }
+    (void) hipGetDevice(&op->device);
+    op->A = std::make_shared<MatrixStore>();
+    DevCSR& c = op->A->csr;
+    c.nrows = nrows;
+    c.ncols = ncols;
+    c.nnz = nrows * per_row;
+    c.dtype = dtype;
+    const size_t vb = dtype == SLQ_F64 ? 8 : 4;
+    auto fail = [&](int code) {
+        delete op;
+        return code;
+    };
+    hipError_t e;
+    if ((e = hipStreamCreateWithFlags(&op->stream, hipStreamNonBlocking)) != hipSuccess ||
+        (e = hipMalloc(&c.indptr, sizeof(int64_t) * (nrows + 1))) != hipSuccess ||
+        (e = hipMalloc(&c.indices, sizeof(int32_t) * c.nnz)) != hipSuccess ||
+        (e = hipMalloc(&c.data, vb * c.nnz)) != hipSuccess)
+        return fail(hip_status(e, "hipMalloc", __FILE__, __LINE__));
+    hipLaunchKernelGGL(row_pointers_kernel, dim3((unsigned) ((nrows + 256) / 256)), dim3(256),
+                       0, op->stream, c.indptr, nrows, per_row);
+    if (dtype == SLQ_F64)
+        hipLaunchKernelGGL(sparse_gaussian_kernel<double>, dim3(8192), dim3(256), 0, op->stream,
+                           nrows, ncols, per_row, (uint64_t) seed, c.indices, (double*) c.data);
+    else
+        hipLaunchKernelGGL(sparse_gaussian_kernel<float>, dim3(8192), dim3(256), 0, op->stream,
+                           nrows, ncols, per_row, (uint64_t) seed, c.indices, (float*) c.data);
+    if ((e = hipGetLastError()) != hipSuccess || (e = hipStreamSynchronize(op->stream)) != hipSuccess)
+        return fail(hip_status(e, "sparse Gaussian generation", __FILE__, __LINE__));
+    // no tiled form: random columns leave 16-row slabs about 1/16 full
+    *out = op;
+    return SLQ_OK;
+}
+
+int slq_op_get_csr(slq_op* op, int64_t* indptr, int32_t* indices, void* data)
+{
+    if (!op || op->kind != OP_MATRIX || !indptr)
+    {
+        set_error("slq_op_get_csr: a matrix handle and an indptr buffer are required");
+        return SLQ_E_ARG;
+    }
+    DeviceGuard guard(op->device);
+    const DevCSR& c = op->A->csr;
+    const size_t vb = c.dtype == SLQ_F64 ? 8 : 4;
+    SLQ_CHECK(hipStreamSynchronize(op->stream));
+    SLQ_CHECK(hipMemcpy(indptr, c.indptr, sizeof(int64_t) * (c.nrows + 1),
+                        hipMemcpyDeviceToHost));
+    if (c.nnz > 0 && indices)
+        SLQ_CHECK(hipMemcpy(indices, c.indices, sizeof(int32_t) * c.nnz, hipMemcpyDeviceToHost));
+    if (c.nnz > 0 && data)
+        SLQ_CHECK(hipMemcpy(data, c.data, vb * c.nnz, hipMemcpyDeviceToHost));
+    return SLQ_OK;
+}
+
+int slq_op_shape(slq_op* op, int64_t* nrows, int64_t* ncols, int64_t* nnz, int32_t* dtype)
+{
+    if (!op)
+    {
+        set_error("slq_op_shape: null handle");
+        return SLQ_E_ARG;
+    }
+    const DevCSR& c = op->A->csr;
+    if (nrows) *nrows = c.nrows;
+    if (ncols) *ncols = c.ncols;
+    if (nnz) *nnz = c.nnz;
+    if (dtype) *dtype = c.dtype;
     return SLQ_OK;
 }
 

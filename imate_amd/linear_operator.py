@@ -18,7 +18,7 @@ import scipy.sparse
 
 from . import _lib
 
-__all__ = ['LinearOperator', 'Matrix', 'AffineMatrixFunction']
+__all__ = ['LinearOperator', 'Matrix', 'AffineMatrixFunction', 'DeviceMatrix']
 
 
 class DeviceOperator(object):
@@ -271,3 +271,61 @@ class AffineMatrixFunction(LinearOperator):
             return self.get_nnz() / \
                 (self.get_num_rows() * self.get_num_columns())
         return 1.0
+
+
+class DeviceMatrix(LinearOperator):
+    """A matrix that exists only in device memory (for instance generated on
+    the device, ``sample_matrices.sparse_gaussian_device``): the ``Matrix``
+    interface without a host copy. ``to_csr()`` copies it back."""
+
+    def __init__(self, device_operator):
+        super(DeviceMatrix, self).__init__()
+        lib = _lib.load()
+        nr = ctypes.c_int64(0)
+        nc = ctypes.c_int64(0)
+        nnz = ctypes.c_int64(0)
+        dt = ctypes.c_int32(0)
+        _lib.check(lib.slq_op_shape(device_operator.handle, ctypes.byref(nr),
+                                    ctypes.byref(nc), ctypes.byref(nnz),
+                                    ctypes.byref(dt)), 'slq_op_shape')
+        self.shape = (nr.value, nc.value)
+        self.nnz = nnz.value
+        self.dtype = numpy.dtype(numpy.float64 if dt.value == _lib.SLQ_F64
+                                 else numpy.float32)
+        self.data_type_name = self.dtype.name.encode()
+        self.num_parameters = 0
+        self.gpu_Aop = device_operator
+        self.initialized_on_gpu = True
+
+    def initialize(self, gpu, num_gpu_devices):
+        if not gpu:
+            self._initialize_cpu()
+        self.gpu = gpu
+        self.num_gpu_devices = num_gpu_devices
+
+    def get_num_rows(self):
+        return self.shape[0]
+
+    def get_num_columns(self):
+        return self.shape[1]
+
+    def is_sparse(self):
+        return True
+
+    def get_nnz(self):
+        return self.nnz
+
+    def get_density(self):
+        return self.nnz / (self.shape[0] * self.shape[1])
+
+    def to_csr(self):
+        """The matrix as a scipy CSR matrix (a copy from the device)."""
+        lib = _lib.load()
+        indptr = numpy.empty(self.shape[0] + 1, dtype=numpy.int64)
+        indices = numpy.empty(self.nnz, dtype=numpy.int32)
+        data = numpy.empty(self.nnz, dtype=self.dtype)
+        _lib.check(lib.slq_op_get_csr(self.gpu_Aop.handle, _lib.ptr(indptr),
+                                      _lib.ptr(indices), _lib.ptr(data)),
+                   'slq_op_get_csr')
+        return scipy.sparse.csr_matrix((data, indices, indptr),
+                                       shape=self.shape)

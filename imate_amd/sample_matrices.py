@@ -27,7 +27,11 @@
   positive definite) and a new l only needs new values.
 * ``sparse_gaussian(nrows, ncols, density, seed)`` — rows with
   round(density * ncols) distinct uniform columns and N(0, 1) values (the
-  rectangular workload of gram=True).
+  rectangular workload of gram=True), built on the host.
+* ``sparse_gaussian_device(nrows, ncols, density, seed)`` — the large
+  rectangular workload built on the device (a ``DeviceMatrix``): row r has
+  one entry in each of k = round(density * ncols) column strata, at a hashed
+  column of the stratum, with a hashed N(0, 1) value.
 """
 
 import numpy
@@ -37,7 +41,7 @@ from scipy.sparse import coo_matrix
 __all__ = ['toeplitz', 'toeplitz_trace', 'toeplitz_traceinv',
            'toeplitz_logdet', 'toeplitz_schatten', 'correlation_scale',
            'sparse_correlation_matrix', 'CorrelationFamily',
-           'sparse_gaussian', 'morton_order']
+           'sparse_gaussian', 'sparse_gaussian_device', 'morton_order']
 
 
 # ---------------------------------------------------------------- Toeplitz
@@ -218,3 +222,20 @@ def sparse_gaussian(nrows, ncols, density, seed=0, dtype='float64'):
     indptr = numpy.arange(0, nrows * k + 1, k, dtype=numpy.int64)
     return scipy.sparse.csr_matrix((vals.ravel(), cols.ravel(), indptr),
                                    shape=(nrows, ncols))
+
+
+def sparse_gaussian_device(nrows, ncols, density, seed=0, dtype='float64'):
+    """Seeded sparse Gaussian matrix generated in device memory
+    (``slq_op_create_sparse_gaussian``): k = round(density * ncols) entries
+    per row, one in each column stratum [floor(s ncols / k),
+    floor((s + 1) ncols / k)), N(0, 1) values. Returns a ``DeviceMatrix``."""
+    import ctypes
+    from . import _lib
+    from .linear_operator import DeviceOperator, DeviceMatrix
+    k = max(1, int(round(density * ncols)))
+    lib = _lib.load()
+    h = ctypes.c_void_p()
+    _lib.check(lib.slq_op_create_sparse_gaussian(
+        int(nrows), int(ncols), k, int(seed), _lib.dtype_code(dtype),
+        ctypes.byref(h)), 'slq_op_create_sparse_gaussian')
+    return DeviceMatrix(DeviceOperator(h))

@@ -97,6 +97,28 @@ int slq_op_create_affine(slq_op* A, slq_op* B, slq_op** out);
  * pattern, such as a compactly supported kernel over its length scale. */
 int slq_op_set_values(slq_op* op, const void* data);
 
+/* A seeded sparse Gaussian matrix generated on the device (the rectangular
+ * Golub-Kahn workload): row r has one entry in each of per_row column strata
+ * [floor(s ncols / per_row), floor((s + 1) ncols / per_row)), at a hashed
+ * column of the stratum (distinct, ascending), with an N(0, 1) value
+ * (Box-Muller of hashed uniforms); row pointers r * per_row. No counterpart
+ * in the reference (imate's sample matrices are host-built,
+ * imate/sample_matrices/toeplitz.py, correlation_matrix.py): a 2^20 x 2^18
+ * matrix at 1 % density has 2.75e9 entries, too many to build on the host. */
+int slq_op_create_sparse_gaussian(int64_t nrows, int64_t ncols, int32_t per_row,
+                                  int64_t seed, int dtype, slq_op** out);
+
+/* Copy a matrix handle's CSR back to the host: indptr[nrows + 1], and, when
+ * non-NULL, indices[nnz] and data[nnz] (dtype of the handle). The reverse of
+ * slq_op_create_csr (no counterpart in the reference, which keeps the host
+ * copy: py_cu_matrix.pyx:118-200). */
+int slq_op_get_csr(slq_op* op, int64_t* indptr, int32_t* indices, void* data);
+
+/* Shape, non-zeros and dtype of a handle (pycuLinearOperator get_num_rows /
+ * get_num_columns, py_cu_linear_operator.pyx:95-125). */
+int slq_op_shape(slq_op* op, int64_t* nrows, int64_t* ncols, int64_t* nnz,
+                 int32_t* dtype);
+
 /* Free a handle (pycuMatrix.__dealloc__, cu_csr_matrix.cu:94-154). */
 int slq_op_destroy(slq_op* op);
 

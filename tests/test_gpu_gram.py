@@ -1,0 +1,82 @@
+"""The device-generated sparse Gaussian matrix and the chunked transpose
+(the 2^20 x 2^18 Golub-Kahn workload), at sizes the oracle finishes."""
+
+import numpy
+import pytest
+import scipy.sparse
+
+import imate_amd
+from imate_amd.sample_matrices import sparse_gaussian_device
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+
+M64 = (1 << 64) - 1
+
+
+def _mix(z):
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9 & M64
+    z = (z ^ (z >> 27)) * 0x94d049bb133111eb & M64
+    return z ^ (z >> 31)
+
+
+def _columns(nrows, ncols, k, seed):
+    """Host restatement of the generator's column hash (exact integers)."""
+    cols = numpy.empty((nrows, k), dtype=numpy.int64)
+    for r in range(nrows):
+        for s in range(k):
+            c0, c1 = s * ncols // k, (s + 1) * ncols // k
+            key = (seed * 0x9e3779b97f4a7c15 + r * 0xd1b54a32d192ed03 + s) & M64
+            cols[r, s] = c0 + _mix(key) % (c1 - c0)
+    return cols
+
+
+def test_generator_pattern_and_values(gpu_lib):
+    nrows, ncols, density, seed = 400, 300, 0.1, 11
+    D = sparse_gaussian_device(nrows, ncols, density, seed=seed)
+    k = int(round(density * ncols))
+    A = D.to_csr()
+    assert A.shape == (nrows, ncols) and A.nnz == nrows * k
+    assert numpy.array_equal(A.indptr, numpy.arange(nrows + 1) * k)
+    assert numpy.array_equal(A.indices.reshape(nrows, k), _columns(nrows, ncols, k, seed))
+    assert numpy.all(numpy.diff(A.indices.reshape(nrows, k), axis=1) > 0)
+    big = sparse_gaussian_device(2000, 1000, 0.1, seed=1).to_csr().data
+    assert abs(big.mean()) < 0.02 and abs(big.var() - 1.0) < 0.03
+    again = sparse_gaussian_device(nrows, ncols, density, seed=seed).to_csr()
+    assert (abs(again - A)).max() == 0
+    other = sparse_gaussian_device(nrows, ncols, density, seed=seed + 1).to_csr()
+    assert not numpy.array_equal(other.indices, A.indices)
+
+
+@pytest.mark.parametrize('chunk', [None, '5000'])
+def test_gram_on_device_matrix_matches_oracle(gpu_lib, monkeypatch, chunk):
+    """Golub-Kahn on the device matrix (transpose built in one sort, or in
+    column chunks of 5000 entries) against the oracle on its CSR copy."""
+    if chunk:
+        monkeypatch.setenv('SLQ_TRANSPOSE_CHUNK', chunk)
+    D = sparse_gaussian_device(1500, 600, 0.05, seed=3)
+    A = D.to_csr()
+    val, info = imate_amd.logdet(D, gram=True, method='slq', gpu=True,
+                                 lanczos_degree=20, min_num_samples=64,
+                                 max_num_samples=64, seed=4, return_info=True)
+    ref = oracle.slq(A, func='log', gram=True, lanczos_degree=20,
+                     min_num_samples=64, max_num_samples=64, seed=4)
+    got = info['convergence']['samples']
+    want = ref['samples'][:, 0]
+    assert numpy.max(numpy.abs(got - want) / numpy.abs(want)) < 1e-10
+    # the chunked and the one-sort transposes are the same matrix
+    assert abs(val - ref['trace'][0]) <= 1e-10 * abs(ref['trace'][0])
+
+
+def test_chunked_transpose_spmm(gpu_lib, monkeypatch):
+    monkeypatch.setenv('SLQ_TRANSPOSE_CHUNK', '777')
+    from imate_amd import _lib
+    D = sparse_gaussian_device(900, 350, 0.08, seed=8)
+    A = D.to_csr()
+    rng = numpy.random.default_rng(0)
+    U = rng.standard_normal((900, 64))
+    Y = numpy.empty((350, 64))
+    _lib.check(gpu_lib.slq_op_spmm(D.gpu_Aop.handle, 0.0, 64, 1, _lib.ptr(U),
+                                   _lib.ptr(Y)), 'slq_op_spmm')
+    ref = A.T @ U
+    assert numpy.max(numpy.abs(Y - ref)) <= 1e-13 * numpy.abs(ref).max()
