@@ -118,3 +118,30 @@ def test_recurrences_every_spmm_layout(gpu_lib, monkeypatch, tile_rows, gram,
     scale = max(1.0, numpy.abs(ra).max())
     assert numpy.max(numpy.abs(a - ra)) < 1e-10 * scale
     assert numpy.max(numpy.abs(b - rb)) < 1e-10 * scale
+
+
+@pytest.mark.parametrize('tile_rows', ['0', '8', '16'])
+@pytest.mark.parametrize('dtype', ['float64', 'float32'])
+@pytest.mark.parametrize('transpose', [0, 1])
+@pytest.mark.parametrize('t', [0.0, 0.7])
+def test_spmm_layouts_match_scipy(gpu_lib, monkeypatch, tile_rows, dtype,
+                                  transpose, t):
+    """The block SpMM through every layout (CSR, 8-row tiles, 16-row tiles on
+    the matrix cores), plain and affine (A + tI), against scipy."""
+    monkeypatch.setenv('SLQ_TILE_ROWS', tile_rows)
+    A = _corr(2500, 40, 5, dtype=dtype)
+    if t:
+        from imate_amd.linear_operator import DeviceOperator
+        dev = DeviceOperator.affine(_op(A))
+    else:
+        dev = _op(A)
+    rng = numpy.random.default_rng(1)
+    X = rng.standard_normal((2500, 64)).astype(dtype)
+    Y = numpy.empty((2500, 64), dtype=dtype)
+    _lib.check(gpu_lib.slq_op_spmm(dev.handle, t, 64, transpose, _lib.ptr(X),
+                                   _lib.ptr(Y)), 'slq_op_spmm')
+    A64 = A.astype(numpy.float64)
+    ref = (A64.T if transpose else A64) @ X.astype(numpy.float64) + \
+        t * X.astype(numpy.float64)
+    tol = 1e-13 if dtype == 'float64' else 3e-6
+    assert numpy.max(numpy.abs(Y - ref)) <= tol * numpy.abs(ref).max()
