@@ -23,11 +23,11 @@
 // 4 MB L2.
 //
 // Reductions (alpha, beta, norms) are per-probe sums over all rows, computed
-// deterministically: each workgroup writes one partial row; the last
-// workgroup of each group of 32 (agent-scope atomic ticket after a fence)
-// adds its group's partials in a fixed order; the last group adds the group
-// partials. The result is bitwise reproducible run to run and the serial
-// tail is two rounds of at most four loads per lane.
+// deterministically and without fences: each workgroup (or, in the tiled
+// SpMM, each row tile) writes one partial row, and a following launch of
+// one workgroup adds the rows in a fixed order and finishes the value (the
+// kernel boundary orders the two). The result is bitwise reproducible run
+// to run.
 
 #include <cmath>
 #include <cstdio>
@@ -40,9 +40,6 @@
 namespace slq {
 
 namespace {
-
-constexpr int kGroup = 32;                         // workgroups per level-1 group
-constexpr int kMaxGroups = (kMaxGrid + kGroup - 1) / kGroup;
 
 // ---------------------------------------------------------------- helpers
 
@@ -90,6 +87,18 @@ __device__ __forceinline__ Sweep sweep(int64_t N, int wave)
     s.end = b + span < N ? b + span : N;
     s.first = b + local * kWavesPerWG + wave;
     s.stride = per_xcd * kWavesPerWG;
+    return s;
+}
+
+// The same units swept grid-wide: wave w of the grid takes units w, w + W,
+// ... (W = waves in the grid), so at any moment the whole chip streams one
+// contiguous window of memory.
+__device__ __forceinline__ Sweep sweep_linear(int64_t N, int wave)
+{
+    Sweep s;
+    s.first = (int64_t) blockIdx.x * kWavesPerWG + wave;
+    s.stride = (int64_t) gridDim.x * kWavesPerWG;
+    s.end = N;
     return s;
 }
 
@@ -145,9 +154,7 @@ __device__ __forceinline__ T row_gather(int64_t beg, int64_t end,
 // ---------------------------------------------------------------- reduction
 
 struct RedBuf {
-    double* partial;            // [G][64]
-    double* gpartial;           // [ceil(G/32)][64]
-    unsigned int* counters;     // [1 + ceil(G/32)], zero between launches
+    double* partial;            // [G][64], one row per workgroup
     // in-order tile schedule of the tiled SpMM: per XCD a ticket counter
     // (uint 64 * x) and a finished-wave counter (uint 64 * x + 32), zero
     // between launches, and one partial row per tile ([ntiles][64])
@@ -156,70 +163,57 @@ struct RedBuf {
 };
 constexpr int kTicketWords = kXcds * 64;
 
-// Per-lane sum of v over the whole grid (see the header). Returns true in
-// exactly one workgroup, whose wave 0 then holds the totals in `total`.
-__device__ __forceinline__ bool grid_total(double v, const RedBuf& rb, double& total)
+// Adds v over the workgroup's wavefronts (lane = probe) and stores the sum
+// as this workgroup's partial row rb.partial[blockIdx.x]. A separate
+// one-workgroup launch adds the rows in order (ordered_total), so no kernel
+// needs a grid-wide fence: each such fence writes back the XCD's L2, which
+// cost a streaming kernel over a tenth of its time.
+__device__ __forceinline__ void wg_partial(double v, const RedBuf& rb)
 {
     __shared__ double red[kWavesPerWG][kWave];
-    __shared__ int flag;
     const int lane = threadIdx.x % kWave;
     const int wave = wave_id();
-    const int G = gridDim.x;
-    const int bid = blockIdx.x;
     red[wave][lane] = v;
     __syncthreads();
-    if (wave == 0)
-    {
-        double s = 0.0;
-#pragma unroll
-        for (int w = 0; w < kWavesPerWG; ++w) s += red[w][lane];
-        rb.partial[(int64_t) bid * kWave + lane] = s;
-    }
-    const int group = bid / kGroup;
-    const int g0 = group * kGroup;
-    const int gsz = (G - g0) < kGroup ? (G - g0) : kGroup;
-    const int ngroups = (G + kGroup - 1) / kGroup;
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0)
-        flag = (atomicAdd(&rb.counters[1 + group], 1u) == (unsigned int) (gsz - 1));
-    __syncthreads();
-    if (!flag) return false;
-    __threadfence();
+    if (wave != 0) return;
     double s = 0.0;
-    for (int r = wave; r < gsz; r += kWavesPerWG)
-        s += rb.partial[(int64_t) (g0 + r) * kWave + lane];
-    __syncthreads();
+#pragma unroll
+    for (int w = 0; w < kWavesPerWG; ++w) s += red[w][lane];
+    rb.partial[(int64_t) blockIdx.x * kWave + lane] = s;
+}
+
+// Sum over rows [0, G) of rows[g][lane] for a workgroup of kTotalWaves
+// wavefronts: wave w adds the rows of its contiguous range in order, eight
+// independent loads at a time (the rows come from another XCD's writes, so
+// each load is a long round trip), and the wave sums are added in wave
+// order: the result does not depend on timing. Valid in wave 0.
+constexpr int kTotalWaves = 16;
+
+__device__ __forceinline__ double ordered_total(const double* __restrict__ rows, int64_t G)
+{
+    __shared__ double red[kTotalWaves][kWave];
+    const int lane = threadIdx.x % kWave;
+    const int wave = threadIdx.x / kWave;
+    const int64_t per = (G + kTotalWaves - 1) / kTotalWaves;
+    const int64_t b0 = wave * per;
+    const int64_t b1 = b0 + per < G ? b0 + per : G;
+    double s = 0.0;
+    int64_t b = b0;
+    for (; b + 8 <= b1; b += 8)
+    {
+        double v[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) v[k] = rows[(b + k) * kWave + lane];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) s += v[k];
+    }
+    for (; b < b1; ++b) s += rows[b * kWave + lane];
     red[wave][lane] = s;
     __syncthreads();
+    double t = 0.0;
     if (wave == 0)
-    {
-        double t = 0.0;
-#pragma unroll
-        for (int w = 0; w < kWavesPerWG; ++w) t += red[w][lane];
-        rb.gpartial[group * kWave + lane] = t;
-        if (lane == 0) rb.counters[1 + group] = 0u;
-    }
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0)
-        flag = (atomicAdd(&rb.counters[0], 1u) == (unsigned int) (ngroups - 1));
-    __syncthreads();
-    if (!flag) return false;
-    __threadfence();
-    s = 0.0;
-    for (int r = wave; r < ngroups; r += kWavesPerWG) s += rb.gpartial[r * kWave + lane];
-    __syncthreads();
-    red[wave][lane] = s;
-    __syncthreads();
-    total = 0.0;
-    if (wave == 0)
-    {
-#pragma unroll
-        for (int w = 0; w < kWavesPerWG; ++w) total += red[w][lane];
-        if (lane == 0) rb.counters[0] = 0u;
-    }
-    return true;
+        for (int w = 0; w < kTotalWaves; ++w) t += red[w][lane];
+    return t;
 }
 
 // beta_j = sqrt(total); breakdown test of the reference (beta < tol sqrt(n))
@@ -317,14 +311,12 @@ __device__ __forceinline__ void spmm_store_total(const SpmmArgs<T>& a, double to
     finish_norm(total, lane, a.j, a.tol_sqrt_n, a.red_out, a.scale_out, a.sizes);
 }
 
+// Partial row of the SpMM's reduction term; total_kernel finishes it.
 template <typename T>
-__device__ __forceinline__ void spmm_finish(const SpmmArgs<T>& a, double red, int lane)
+__device__ __forceinline__ void spmm_finish(const SpmmArgs<T>& a, double red)
 {
     if (a.red_mode == 0) return;
-    double total;
-    if (!grid_total(red, a.rb, total)) return;
-    if (wave_id() != 0) return;
-    spmm_store_total(a, total, lane);
+    wg_partial(red, a.rb);
 }
 
 // CSR form: one wavefront per row.
@@ -351,7 +343,7 @@ __global__ __launch_bounds__(kThreads) void spmm_kernel(SpmmArgs<T> a,
                                        a.b_vals, X, s, lane);
         spmm_epilogue<T, NT>(a, row, lane, y, s, tt, sub_c, sub_s, X, Y, Vout, Sub, red);
     }
-    spmm_finish<T>(a, red, lane);
+    spmm_finish<T>(a, red);
 }
 
 // Row-tiled form: one wavefront per tile of R rows, R accumulators per lane.
@@ -424,7 +416,7 @@ __global__ __launch_bounds__(kThreads) void tiled_spmm_kernel(
                                      Sub, red);
         }
     }
-    spmm_finish<T>(a, red, lane);
+    spmm_finish<T>(a, red);
 }
 
 // One chunk (up to 64 slabs) of a tile, scalar-loaded slab values: the
@@ -479,8 +471,7 @@ __device__ __forceinline__ void slabs_scalar(const T* __restrict__ X,
 // times.) Which wavefront handles which tile varies from run to run, so the
 // reduction terms are written per tile (tile_partial[t][probe], rows of the
 // tile added in order) and summed in tile order by tile_sum_kernel and
-// tile_total_kernel: the
-// result does not depend on the schedule.
+// total_kernel: the result does not depend on the schedule.
 template <typename T, int R, bool NT>
 __global__ __launch_bounds__(kThreads) void tiled_spmm_sched_kernel(
     SpmmArgs<T> a, const T* __restrict__ X, T* __restrict__ Y, T* __restrict__ Vout,
@@ -709,7 +700,7 @@ __global__ __launch_bounds__(kThreads) void mfma_spmm_kernel(
 // kernel boundary orders them; no fences): tile_sum_kernel gives workgroup g
 // a contiguous range of tiles, its wavefronts interleaved with four
 // accumulators each, combined in a fixed order into rb.partial[g];
-// tile_total_kernel (one workgroup) adds the rows of rb.partial in order and
+// total_kernel (one workgroup) adds the rows of rb.partial in order and
 // stores the total like the SpMM's own reduction.
 constexpr int kTileSumGrid = 256;
 
@@ -746,22 +737,11 @@ __global__ __launch_bounds__(kThreads) void tile_sum_kernel(const double* __rest
 }
 
 template <typename T>
-__global__ __launch_bounds__(1024) void tile_total_kernel(SpmmArgs<T> a, int G)
+__global__ __launch_bounds__(kTotalWaves * kWave) void total_kernel(SpmmArgs<T> a, int G)
 {
-    __shared__ double red[16][kWave];
-    const int lane = threadIdx.x % kWave;
-    const int grp = threadIdx.x / kWave;
-    double s = 0.0;
-    for (int g = grp; g < G; g += 16) s += a.rb.partial[(int64_t) g * kWave + lane];
-    red[grp][lane] = s;
-    __syncthreads();
-    if (grp == 0)
-    {
-        double t = 0.0;
-#pragma unroll
-        for (int i = 0; i < 16; ++i) t += red[i][lane];
-        spmm_store_total(a, t, lane);
-    }
+    const double t = ordered_total(a.rb.partial, G);
+    if (threadIdx.x >= kWave) return;
+    spmm_store_total(a, t, threadIdx.x);
 }
 
 // ---------------------------------------------------------------- update
@@ -810,7 +790,7 @@ __global__ __launch_bounds__(kThreads) void lanczos_update_kernel(
     const int wave = wave_id();
     // unit = pair of rows; lane l covers row 2u + (l >= 32), probes 2(l%32), +1
     const int64_t npairs = (a.nrows + 1) / 2;
-    const Sweep sw = sweep(npairs, wave);
+    const Sweep sw = sweep_linear(npairs, wave);
     const int half = lane >> 5;
     const int p0 = 2 * (lane & 31);
     const double al0 = a.alpha[p0], al1 = a.alpha[p0 + 1];
@@ -887,10 +867,19 @@ __global__ __launch_bounds__(kThreads) void lanczos_update_kernel(
     // probe q = lane: pair slot (q % 2), owner lanes q/2 and q/2 + 32
     const int q = lane;
     const double mine = ex[wave][q & 1][q >> 1] + ex[wave][q & 1][(q >> 1) + 32];
-    double total;
-    if (!grid_total(mine, a.rb, total)) return;
-    if (wave != 0) return;
-    finish_norm(total, lane, a.j, a.tol_sqrt_n, a.beta_out, a.scale_out, a.sizes);
+    wg_partial(mine, a.rb);
+}
+
+// Finishes a norm from the G partial rows of the kernel before: beta =
+// sqrt(sum) and the breakdown test.
+__global__ __launch_bounds__(kTotalWaves * kWave) void norm_total_kernel(
+    const double* __restrict__ partial, int G, int j, double tol_sqrt_n,
+    double* __restrict__ beta_out, double* __restrict__ scale_out,
+    int32_t* __restrict__ sizes)
+{
+    const double t = ordered_total(partial, G);
+    if (threadIdx.x >= kWave) return;
+    finish_norm(t, threadIdx.x, j, tol_sqrt_n, beta_out, scale_out, sizes);
 }
 
 // ------------------------------------------------ Gram-Schmidt (CGS2)
@@ -972,10 +961,7 @@ __global__ __launch_bounds__(kThreads) void gs_kernel(GsArgs a, T* __restrict__ 
         const double r = (double) R[row * kBlock + lane];
         red += r * r;
     }
-    double total;
-    if (!grid_total(red, a.rb, total)) return;
-    if (wave != 0) return;
-    finish_norm(total, lane, a.j, a.tol_sqrt_n, a.norm_out, a.scale_out, a.sizes);
+    wg_partial(red, a.rb);    // norm_total_kernel finishes the norm
 }
 
 // out[k][p] = sum_g partial[g][k][p] in a fixed order; one workgroup per k,
@@ -1128,8 +1114,6 @@ RedBuf take_redbuf(Bump& bump, int64_t tiles)
 {
     RedBuf rb;
     rb.partial = bump.take<double>((size_t) kMaxGrid * kBlock);
-    rb.gpartial = bump.take<double>((size_t) kMaxGroups * kBlock);
-    rb.counters = bump.take<unsigned int>(1 + kMaxGroups);
     rb.tickets = bump.take<unsigned int>(kTicketWords);
     rb.tile_partial = tiles > 0 ? bump.take<double>((size_t) tiles * kBlock) : nullptr;
     if (tiles > 0 && !rb.tile_partial) rb.partial = nullptr;   // reported as too small
@@ -1138,12 +1122,11 @@ RedBuf take_redbuf(Bump& bump, int64_t tiles)
 
 bool redbuf_ok(const RedBuf& rb)
 {
-    return rb.partial && rb.gpartial && rb.counters && rb.tickets;
+    return rb.partial && rb.tickets;
 }
 
 void zero_counters(hipStream_t s, const RedBuf& rb)
 {
-    hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, s, rb.counters, 1 + kMaxGroups);
     hipLaunchKernelGGL(zero_u32_kernel, dim3(kTicketWords / 64), dim3(64), 0, s, rb.tickets,
                        kTicketWords);
 }
@@ -1151,8 +1134,7 @@ void zero_counters(hipStream_t s, const RedBuf& rb)
 size_t redbuf_bytes(int64_t tiles)
 {
     auto r = [](size_t b) { return (b + 255) & ~(size_t) 255; };
-    return r(sizeof(double) * kMaxGrid * kBlock) + r(sizeof(double) * kMaxGroups * kBlock) +
-           r(sizeof(unsigned int) * (1 + kMaxGroups)) + r(sizeof(unsigned int) * kTicketWords) +
+    return r(sizeof(double) * kMaxGrid * kBlock) + r(sizeof(unsigned int) * kTicketWords) +
            (tiles > 0 ? r(sizeof(double) * (size_t) tiles * kBlock) : 0);
 }
 
@@ -1262,23 +1244,34 @@ void launch_spmm_nt(slq_op* op, const SpmmArgs<T>& s, const SpmmIO<T>& io, const
                 hipLaunchKernelGGL(tile_sum_kernel, dim3(rg), dim3(kThreads), 0, op->stream,
                                    (const double*) s.rb.tile_partial, tl->ntiles,
                                    s.rb.partial);
-                hipLaunchKernelGGL((tile_total_kernel<T>), dim3(1), dim3(1024), 0, op->stream,
-                                   s, rg);
+                hipLaunchKernelGGL((total_kernel<T>), dim3(1), dim3(kTotalWaves * kWave), 0,
+                                   op->stream, s, rg);
             }
         }
-        else if (tl->R == 8)
-            hipLaunchKernelGGL((tiled_spmm_kernel<T, 8, NT>), dim3(grid), dim3(kThreads), 0,
-                               op->stream, s, io.X, io.Y, io.Vout, io.Sub, tl->ntiles,
-                               tl->ptr, tl->cols, tv);
         else
-            hipLaunchKernelGGL((tiled_spmm_kernel<T, 16, NT>), dim3(grid), dim3(kThreads), 0,
-                               op->stream, s, io.X, io.Y, io.Vout, io.Sub, tl->ntiles,
-                               tl->ptr, tl->cols, tv);
+        {
+            if (tl->R == 8)
+                hipLaunchKernelGGL((tiled_spmm_kernel<T, 8, NT>), dim3(grid), dim3(kThreads), 0,
+                                   op->stream, s, io.X, io.Y, io.Vout, io.Sub, tl->ntiles,
+                                   tl->ptr, tl->cols, tv);
+            else
+                hipLaunchKernelGGL((tiled_spmm_kernel<T, 16, NT>), dim3(grid), dim3(kThreads),
+                                   0, op->stream, s, io.X, io.Y, io.Vout, io.Sub, tl->ntiles,
+                                   tl->ptr, tl->cols, tv);
+            if (s.red_mode)
+                hipLaunchKernelGGL((total_kernel<T>), dim3(1), dim3(kTotalWaves * kWave), 0,
+                                   op->stream, s, grid);
+        }
     }
     else
-        hipLaunchKernelGGL((spmm_kernel<T, NT>), dim3(grid_for(s.nrows, op->spmm_grid)),
-                           dim3(kThreads), 0,
-                           op->stream, s, io.X, io.Y, io.Vout, io.Sub);
+    {
+        const int grid = grid_for(s.nrows, op->spmm_grid);
+        hipLaunchKernelGGL((spmm_kernel<T, NT>), dim3(grid), dim3(kThreads), 0, op->stream, s,
+                           io.X, io.Y, io.Vout, io.Sub);
+        if (s.red_mode)
+            hipLaunchKernelGGL((total_kernel<T>), dim3(1), dim3(kTotalWaves * kWave), 0,
+                               op->stream, s, grid);
+    }
 }
 
 template <typename T>
@@ -1299,10 +1292,17 @@ int launch_update(slq_op* op, const UpdateArgs& u, T* R, const T* Vj, const T* V
 {
     hipEvent_t ev = nullptr;
     prof_begin(op, &ev);
-    hipLaunchKernelGGL(lanczos_update_kernel<T>,
-                       dim3(stream_grid((u.nrows + 3) / 4, op->stream_grid)),
-                       dim3(kThreads), 0, op->stream, u, R, Vj, Vjm1);
+    const int G = stream_grid((u.nrows + 3) / 4, op->stream_grid);
+    hipLaunchKernelGGL(lanczos_update_kernel<T>, dim3(G), dim3(kThreads), 0, op->stream, u,
+                       R, Vj, Vjm1);
     SLQ_CHECK(hipGetLastError());
+    if (u.do_norm)
+    {
+        hipLaunchKernelGGL(norm_total_kernel, dim3(1), dim3(kTotalWaves * kWave), 0,
+                           op->stream, (const double*) u.rb.partial, G, u.j, u.tol_sqrt_n,
+                           u.beta_out, u.scale_out, u.sizes);
+        SLQ_CHECK(hipGetLastError());
+    }
     prof_end(op, ev, 1);
     return SLQ_OK;
 }
@@ -1354,6 +1354,10 @@ int reorthogonalize(slq_op* op, int64_t nrows, T* R, const T* basis, int last,
     // pass 3: r -= V h2; norm
     g.mode = 1;
     hipLaunchKernelGGL(gs_kernel<T>, dim3(G), dim3(kThreads), 0, op->stream, g, R, basis);
+    SLQ_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(norm_total_kernel, dim3(1), dim3(kTotalWaves * kWave), 0, op->stream,
+                       (const double*) g.rb.partial, G, g.j, g.tol_sqrt_n, g.norm_out,
+                       g.scale_out, g.sizes);
     SLQ_CHECK(hipGetLastError());
     return SLQ_OK;
 }
