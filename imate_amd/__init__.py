@@ -18,10 +18,13 @@ from .trexp import trexp
 from .trlinfrac import trlinfrac
 from .eigencount import eigencount
 from .density import density
+from .interpolator import InterpolateSchatten, InterpolateLogdet, \
+    InterpolateTrace
 from . import sample_matrices
 from ._lib import device_properties
 
 __all__ = ['logdet', 'traceinv', 'trace', 'schatten', 'trexp', 'trlinfrac',
-           'eigencount', 'density', 'LinearOperator',
+           'eigencount', 'density', 'InterpolateSchatten',
+           'InterpolateLogdet', 'InterpolateTrace', 'LinearOperator',
            'Matrix', 'AffineMatrixFunction', 'sample_matrices',
            'device_properties', '__version__']
