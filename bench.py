@@ -234,6 +234,58 @@ def batched_form(args):
     print(json.dumps(out), flush=True)
 
 
+def interpolator_baseline(fam, ls, ts, t_range, m, samples, slq, ci, pred,
+                          hs, ht, n):
+    """InterpolateLogdet (kind 'imbf', ortho2 basis) per held-out scale over
+    the held-out t values: its errors against the SLQ judge and the dense
+    Cholesky values, beside the learned surrogate's, and its cost per scale
+    (upload of K(l), one SLQ call for the 6 points, the fit)."""
+    from imate_amd import InterpolateLogdet
+    ti = numpy.geomspace(t_range[0], t_range[1], 6)
+    opts = dict(method='slq', gpu=True, lanczos_degree=m,
+                min_num_samples=samples, max_num_samples=samples, seed=707)
+    q = ts.shape[1]
+    ipred = numpy.empty(len(ht))
+    build = fit = 0.0
+    for i, scale in enumerate(ls):
+        t0 = time.perf_counter()
+        K = fam.matrix(scale)
+        t1 = time.perf_counter()
+        L = InterpolateLogdet(K, kind='imbf', ti=ti, options=opts)
+        ipred[i * q:(i + 1) * q] = L.interpolate(ts[i])
+        t2 = time.perf_counter()
+        build += t1 - t0
+        fit += t2 - t1
+    res = {'kind': 'imbf (ortho2)', 'points': len(ti),
+           'matrix_build_seconds_per_scale': build / len(ls),
+           'upload_slq_fit_seconds_per_scale': fit / len(ls),
+           'inside_ci': float(numpy.mean(numpy.abs(ipred - slq) <= ci)),
+           'median_abs_err_per_n_vs_slq':
+               float(numpy.median(numpy.abs(ipred - slq)) / n),
+           'max_abs_err_per_n_vs_slq':
+               float(numpy.max(numpy.abs(ipred - slq)) / n),
+           'median_abs_err_per_n_surrogate_vs_slq':
+               float(numpy.median(numpy.abs(pred - slq)) / n)}
+    path = os.path.join(HERE, 'tests', 'golden', 'surrogate_cholesky.json')
+    if os.path.exists(path):
+        with open(path) as f:
+            ref = json.load(f)
+        ei, es = [], []
+        for e in ref['entries']:
+            i = e['scale_index'] * q + e['t_index']
+            assert abs(e['scale'] - hs[i]) < 1e-12 * hs[i]
+            ei.append(abs(ipred[i] - e['logdet']))
+            es.append(abs(pred[i] - e['logdet']))
+        res['vs_cholesky'] = {
+            'count': len(ei),
+            'median_abs_err_per_n_interpolator': float(numpy.median(ei) / n),
+            'median_abs_err_per_n_surrogate': float(numpy.median(es) / n),
+            'max_abs_err_per_n_interpolator': float(numpy.max(ei) / n),
+            'max_abs_err_per_n_surrogate': float(numpy.max(es) / n)}
+    log('interpolator baseline: %s' % json.dumps(res))
+    return res
+
+
 def surrogate_form(args):
     """Learned form. Family: n = 65536 correlation matrices K(l) over the
     length scale l in [0.01, 0.1] with the support radius fixed at
@@ -326,6 +378,10 @@ def surrogate_form(args):
             'surrogate_within_ci_of_exact':
                 float(numpy.mean(col['surrogate'] <= col['ci'])),
             'slq_within_ci_of_exact': float(numpy.mean(col['slq'] <= col['ci']))}
+    # the reference's own surrogate over t, one interpolator per scale
+    # (imbf, ortho2 basis, 6 points over the t range, estimated in one call)
+    held['interpolator'] = interpolator_baseline(
+        fam, ls, ts, sur.t_range, m, 128, slq, ci, pred, hs, ht, n)
     out['held_out'] = held
     log('held-out: %s' % json.dumps(held))
     if args.dump:

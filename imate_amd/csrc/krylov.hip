@@ -189,7 +189,8 @@ __device__ __forceinline__ void wg_partial(double v, const RedBuf& rb)
 // order: the result does not depend on timing. Valid in wave 0.
 constexpr int kTotalWaves = 16;
 
-__device__ __forceinline__ double ordered_total(const double* __restrict__ rows, int64_t G)
+__device__ __forceinline__ double ordered_total(const double* __restrict__ rows, int64_t G,
+                                                int64_t row_stride = kWave)
 {
     __shared__ double red[kTotalWaves][kWave];
     const int lane = threadIdx.x % kWave;
@@ -203,11 +204,11 @@ __device__ __forceinline__ double ordered_total(const double* __restrict__ rows,
     {
         double v[8];
 #pragma unroll
-        for (int k = 0; k < 8; ++k) v[k] = rows[(b + k) * kWave + lane];
+        for (int k = 0; k < 8; ++k) v[k] = rows[(b + k) * row_stride + lane];
 #pragma unroll
         for (int k = 0; k < 8; ++k) s += v[k];
     }
-    for (; b < b1; ++b) s += rows[b * kWave + lane];
+    for (; b < b1; ++b) s += rows[b * row_stride + lane];
     red[wave][lane] = s;
     __syncthreads();
     double t = 0.0;
@@ -884,112 +885,171 @@ __global__ __launch_bounds__(kTotalWaves * kWave) void norm_total_kernel(
 
 // ------------------------------------------------ Gram-Schmidt (CGS2)
 
-constexpr int kGsGroup = 8;     // basis vectors per register group
+// Classical Gram-Schmidt, twice, against the K newest basis vectors of each
+// probe (basis layout [slot][row][64]; vector k of a pass is slot last - k):
+// h1 = V^T r, r -= V h1, h2 = V^T r, r -= V h2, |r|. Every probe has its own
+// basis, so each product is a GEMV per probe (0.5 flop per byte): the
+// kernels are HBM streams, written for loads in flight: a wavefront takes
+// one row per iteration and issues the loads of kGsChunk basis vectors at
+// once.
+constexpr int kGsChunk = 32;     // basis vectors per pass over the rows
 
-// Optionally r -= sum_k h_k basis_k (h from a previous pass), then either the
-// per-probe dots h'_k = <basis_k, r> (partials [G][K][64], mode 0) or the
-// norm |r| with the breakdown test (mode 1).
+// Element stride between the stored basis vectors of a re-orthogonalised
+// recurrence: (rows + pad) x 64. The Gram-Schmidt kernels read the same
+// row of up to kGsChunk vectors at once; with an unpadded power-of-two
+// stride those addresses all fall in the same memory channel (an FP32
+// block of 2^16 rows is exactly 16 MiB). SLQ_SLOT_PAD overrides the pad.
+int slot_pad_rows()
+{
+    static int v = [] {
+        const char* e = std::getenv("SLQ_SLOT_PAD");
+        return e ? std::atoi(e) : 3;
+    }();
+    return v;
+}
+
+size_t slot_stride(int64_t rows) { return (size_t) (rows + slot_pad_rows()) * kBlock; }
+
 struct GsArgs {
     int64_t nrows;
+    int64_t slot;            // element stride between basis vectors
     int last;                // slot of the newest vector; vector k is slot last - k
     int K;
-    const double* h_sub;     // [K][64] or nullptr (no subtraction)
-    int mode;
-    double* partial_k;       // [G][K][64] (mode 0)
-    RedBuf rb;               // mode 1
-    double* norm_out;
-    double* scale_out;
-    int32_t* sizes;
-    int j;
-    double tol_sqrt_n;
+    const double* h;         // [K][64] coefficients (gs_sub_kernel)
+    double* partial_k;       // [G][K][64] (gs_dots_kernel)
+    int do_norm;             // gs_sub_kernel: partial rows of |r|^2 in rb
+    RedBuf rb;
 };
 
-template <typename T>
-__global__ __launch_bounds__(kThreads) void gs_kernel(GsArgs a, T* __restrict__ R,
-                                                      const T* __restrict__ basis)
+// The row loops of the two kernels for a chunk of kn <= C vectors starting at
+// `base` (vector g at base - g * stride). Vectors past the last are loaded
+// again as the last (a cache hit), so every load is unconditional and all C
+// are in flight at once; chunks are cut to C = 8, 16 or 32 to keep the
+// repeats few.
+template <typename T, int C>
+__device__ __forceinline__ void dots_rows(const T* __restrict__ base, const T* __restrict__ R,
+                                          int64_t stride, int kn, const Sweep& sw, int lane,
+                                          double (&acc)[kGsChunk])
 {
-    const int lane = threadIdx.x % kWave;
-    const int wave = wave_id();
-    const Sweep sw = sweep(a.nrows, wave);
-    const int64_t stride = a.nrows * kBlock;
-    if (a.h_sub)
+    for (int64_t row = sw.first; row < sw.end; row += sw.stride)
     {
-        for (int64_t row = sw.first; row < sw.end; row += sw.stride)
-        {
-            const int64_t o = row * kBlock + lane;
-            double r = (double) R[o];
-            for (int k = 0; k < a.K; ++k)
-                r -= a.h_sub[k * kWave + lane] * (double) basis[(int64_t) (a.last - k) * stride + o];
-            R[o] = (T) r;
-        }
+        const int64_t o = row * kBlock + lane;
+        T v[C];
+#pragma unroll
+        for (int g = 0; g < C; ++g) v[g] = base[o - (int64_t) (g < kn ? g : kn - 1) * stride];
+        const double r = (double) R[o];
+#pragma unroll
+        for (int g = 0; g < C; ++g) acc[g] += r * (double) v[g];
     }
-    if (a.mode == 0)
-    {
-        __shared__ double red[kWavesPerWG][kGsGroup][kWave];
-        for (int k0 = 0; k0 < a.K; k0 += kGsGroup)
-        {
-            const int kn = (a.K - k0) < kGsGroup ? (a.K - k0) : kGsGroup;
-            double acc[kGsGroup];
-#pragma unroll
-            for (int g = 0; g < kGsGroup; ++g) acc[g] = 0.0;
-            for (int64_t row = sw.first; row < sw.end; row += sw.stride)
-            {
-                const int64_t o = row * kBlock + lane;
-                const double r = (double) R[o];
-#pragma unroll
-                for (int g = 0; g < kGsGroup; ++g)
-                    if (g < kn)
-                        acc[g] += r * (double) basis[(int64_t) (a.last - k0 - g) * stride + o];
-            }
-#pragma unroll
-            for (int g = 0; g < kGsGroup; ++g) red[wave][g][lane] = acc[g];
-            __syncthreads();
-            if (wave < kn)
-            {
-                double s = 0.0;
-#pragma unroll
-                for (int w = 0; w < kWavesPerWG; ++w) s += red[w][wave][lane];
-                a.partial_k[((int64_t) blockIdx.x * a.K + k0 + wave) * kWave + lane] = s;
-            }
-            __syncthreads();
-        }
-        return;
-    }
+}
+
+// r -= sum_g hs[g] v_g (hs is zero past the last vector: r - 0 * v = r).
+template <typename T, int C>
+__device__ __forceinline__ double sub_rows(const T* __restrict__ base, T* __restrict__ R,
+                                           int64_t stride, int kn, const Sweep& sw, int lane,
+                                           const double (*hs)[kWave], bool norm)
+{
     double red = 0.0;
     for (int64_t row = sw.first; row < sw.end; row += sw.stride)
     {
-        const double r = (double) R[row * kBlock + lane];
-        red += r * r;
+        const int64_t o = row * kBlock + lane;
+        T v[C];
+#pragma unroll
+        for (int g = 0; g < C; ++g) v[g] = base[o - (int64_t) (g < kn ? g : kn - 1) * stride];
+        double r = (double) R[o];
+#pragma unroll
+        for (int g = 0; g < C; ++g) r -= hs[g][lane] * (double) v[g];
+        const T rt = (T) r;
+        R[o] = rt;
+        if (norm) red += (double) rt * (double) rt;
     }
-    wg_partial(red, a.rb);    // norm_total_kernel finishes the norm
+    return red;
 }
 
-// out[k][p] = sum_g partial[g][k][p] in a fixed order; one workgroup per k,
-// 16 row groups of 64 lanes each then combined in order.
-__global__ __launch_bounds__(1024) void reduce_partials_kernel(
+// partial_k[g][k] = sum over the rows of workgroup g of basis_k * r.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void gs_dots_kernel(GsArgs a, const T* __restrict__ R,
+                                                           const T* __restrict__ basis)
+{
+    __shared__ double red[kWavesPerWG][8][kWave];
+    const int lane = threadIdx.x % kWave;
+    const int wave = wave_id();
+    const Sweep sw = sweep_linear(a.nrows, wave);
+    const int64_t stride = a.slot;
+    for (int k0 = 0; k0 < a.K; k0 += kGsChunk)
+    {
+        const int kn = (a.K - k0) < kGsChunk ? (a.K - k0) : kGsChunk;
+        const T* __restrict__ base = basis + (int64_t) (a.last - k0) * stride;
+        double acc[kGsChunk];
+#pragma unroll
+        for (int g = 0; g < kGsChunk; ++g) acc[g] = 0.0;
+        if (kn > 16) dots_rows<T, 32>(base, R, stride, kn, sw, lane, acc);
+        else if (kn > 8) dots_rows<T, 16>(base, R, stride, kn, sw, lane, acc);
+        else dots_rows<T, 8>(base, R, stride, kn, sw, lane, acc);
+        // workgroup sums, eight vectors at a time through LDS
+#pragma unroll
+        for (int g0 = 0; g0 < kGsChunk; g0 += 8)
+        {
+            if (g0 >= kn) break;
+#pragma unroll
+            for (int g = 0; g < 8; ++g) red[wave][g][lane] = acc[g0 + g];
+            __syncthreads();
+            if (wave < 8 && g0 + wave < kn)
+            {
+                double t = 0.0;
+#pragma unroll
+                for (int w = 0; w < kWavesPerWG; ++w) t += red[w][wave][lane];
+                a.partial_k[((int64_t) blockIdx.x * a.K + k0 + g0 + wave) * kWave + lane] = t;
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// r -= sum_k h_k basis_k (in k order), optionally with the partial rows of
+// |r|^2 for norm_total_kernel.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void gs_sub_kernel(GsArgs a, T* __restrict__ R,
+                                                          const T* __restrict__ basis)
+{
+    __shared__ double hs[kGsChunk][kWave];
+    const int lane = threadIdx.x % kWave;
+    const int wave = wave_id();
+    const Sweep sw = sweep_linear(a.nrows, wave);
+    const int64_t stride = a.slot;
+    double red = 0.0;
+    if (a.K == 0)
+    {
+        // norm only
+        for (int64_t row = sw.first; row < sw.end; row += sw.stride)
+        {
+            const double r = (double) R[row * kBlock + lane];
+            red += r * r;
+        }
+    }
+    for (int k0 = 0; k0 < a.K; k0 += kGsChunk)
+    {
+        const int kn = (a.K - k0) < kGsChunk ? (a.K - k0) : kGsChunk;
+        const bool norm = a.do_norm && k0 + kGsChunk >= a.K;
+        __syncthreads();
+        for (int i = threadIdx.x; i < kGsChunk * kWave; i += blockDim.x)
+            hs[i / kWave][i % kWave] = i < kn * kWave ? a.h[(int64_t) k0 * kWave + i] : 0.0;
+        __syncthreads();
+        const T* __restrict__ base = basis + (int64_t) (a.last - k0) * stride;
+        if (kn > 16) red = sub_rows<T, 32>(base, R, stride, kn, sw, lane, hs, norm);
+        else if (kn > 8) red = sub_rows<T, 16>(base, R, stride, kn, sw, lane, hs, norm);
+        else red = sub_rows<T, 8>(base, R, stride, kn, sw, lane, hs, norm);
+    }
+    if (a.do_norm) wg_partial(red, a.rb);
+}
+
+// out[k][p] = sum_g partial[g][k][p] in a fixed order; one workgroup per k.
+__global__ __launch_bounds__(kTotalWaves * kWave) void reduce_partials_kernel(
     const double* __restrict__ partial, int G, int K, double* __restrict__ out)
 {
-    __shared__ double red[16][kWave];
-    const int lane = threadIdx.x % kWave;
-    const int grp = threadIdx.x / kWave;
     const int k = blockIdx.x;
-    double s0 = 0.0, s1 = 0.0;
-    int g = grp;
-    for (; g + 16 < G; g += 32)
-    {
-        s0 += partial[((int64_t) g * K + k) * kWave + lane];
-        s1 += partial[((int64_t) (g + 16) * K + k) * kWave + lane];
-    }
-    if (g < G) s0 += partial[((int64_t) g * K + k) * kWave + lane];
-    red[grp][lane] = s0 + s1;
-    __syncthreads();
-    if (grp == 0)
-    {
-        double t = 0.0;
-#pragma unroll
-        for (int i = 0; i < 16; ++i) t += red[i][lane];
-        out[k * kWave + lane] = t;
-    }
+    const double t = ordered_total(partial + (int64_t) k * kWave, G, (int64_t) K * kWave);
+    if (threadIdx.x < kWave) out[k * kWave + threadIdx.x] = t;
 }
 
 // Block set-up: scale[p] = s0 for probes p < count (0 otherwise), sizes = 0.
@@ -1314,8 +1374,8 @@ int gs_grid(int64_t rows, int cap) { return stream_grid(rows, cap); }
 // CGS2 of r against the K newest basis vectors (slots last, last-1, ...);
 // the final pass finishes the norm (beta_j, or alpha_j for Golub-Kahn u).
 template <typename T>
-int reorthogonalize(slq_op* op, int64_t nrows, T* R, const T* basis, int last,
-                    int K, double* h1, double* h2, double* partial_k,
+int reorthogonalize(slq_op* op, int64_t nrows, T* R, const T* basis, size_t slot,
+                    int last, int K, double* h1, double* h2, double* partial_k,
                     const RedBuf& rb, double* norm_out, double* scale_out,
                     int32_t* sizes, int j, double tol_sqrt_n)
 {
@@ -1323,41 +1383,35 @@ int reorthogonalize(slq_op* op, int64_t nrows, T* R, const T* basis, int last,
     GsArgs g;
     std::memset(&g, 0, sizeof(g));
     g.nrows = nrows;
+    g.slot = (int64_t) slot;
     g.last = last;
     g.K = K;
     g.partial_k = partial_k;
     g.rb = rb;
-    g.norm_out = norm_out;
-    g.scale_out = scale_out;
-    g.sizes = sizes;
-    g.j = j;
-    g.tol_sqrt_n = tol_sqrt_n;
     if (K > 0)
     {
-        // pass 1: h1 = V^T r
-        g.h_sub = nullptr;
-        g.mode = 0;
-        hipLaunchKernelGGL(gs_kernel<T>, dim3(G), dim3(kThreads), 0, op->stream, g, R, basis);
+        // h1 = V^T r; r -= V h1; h2 = V^T r; then r -= V h2 below
+        const dim3 tg(kTotalWaves * kWave);
+        hipLaunchKernelGGL(gs_dots_kernel<T>, dim3(G), dim3(kThreads), 0, op->stream, g, R,
+                           basis);
+        hipLaunchKernelGGL(reduce_partials_kernel, dim3(K), tg, 0, op->stream, partial_k, G,
+                           K, h1);
+        g.h = h1;
+        hipLaunchKernelGGL(gs_sub_kernel<T>, dim3(G), dim3(kThreads), 0, op->stream, g, R,
+                           basis);
+        hipLaunchKernelGGL(gs_dots_kernel<T>, dim3(G), dim3(kThreads), 0, op->stream, g, R,
+                           basis);
+        hipLaunchKernelGGL(reduce_partials_kernel, dim3(K), tg, 0, op->stream, partial_k, G,
+                           K, h2);
         SLQ_CHECK(hipGetLastError());
-        hipLaunchKernelGGL(reduce_partials_kernel, dim3(K), dim3(1024), 0, op->stream,
-                           partial_k, G, K, h1);
-        SLQ_CHECK(hipGetLastError());
-        // pass 2: r -= V h1; h2 = V^T r
-        g.h_sub = h1;
-        hipLaunchKernelGGL(gs_kernel<T>, dim3(G), dim3(kThreads), 0, op->stream, g, R, basis);
-        SLQ_CHECK(hipGetLastError());
-        hipLaunchKernelGGL(reduce_partials_kernel, dim3(K), dim3(1024), 0, op->stream,
-                           partial_k, G, K, h2);
-        SLQ_CHECK(hipGetLastError());
-        g.h_sub = h2;
+        g.h = h2;
     }
-    // pass 3: r -= V h2; norm
-    g.mode = 1;
-    hipLaunchKernelGGL(gs_kernel<T>, dim3(G), dim3(kThreads), 0, op->stream, g, R, basis);
+    g.do_norm = 1;
+    hipLaunchKernelGGL(gs_sub_kernel<T>, dim3(G), dim3(kThreads), 0, op->stream, g, R, basis);
     SLQ_CHECK(hipGetLastError());
     hipLaunchKernelGGL(norm_total_kernel, dim3(1), dim3(kTotalWaves * kWave), 0, op->stream,
-                       (const double*) g.rb.partial, G, g.j, g.tol_sqrt_n, g.norm_out,
-                       g.scale_out, g.sizes);
+                       (const double*) rb.partial, G, j, tol_sqrt_n, norm_out, scale_out,
+                       sizes);
     SLQ_CHECK(hipGetLastError());
     return SLQ_OK;
 }
@@ -1463,7 +1517,8 @@ int lanczos(const RecurrenceArgs& ra, Bump& bump)
     if (!reorth) return lanczos_ring<T>(ra, bump);
     T* P = bump.take<T>(blk);
     T* Q = bump.take<T>(blk);
-    T* V = bump.take<T>(blk * nslots);
+    const size_t vst = slot_stride(n);
+    T* V = bump.take<T>(vst * nslots);
     double* scale = bump.take<double>(kBlock);
     const RedBuf rb = take_redbuf(bump, sched_tiles(op));
     double* h1 = nullptr;
@@ -1494,8 +1549,8 @@ int lanczos(const RecurrenceArgs& ra, Bump& bump)
     for (int j = 0; j < m; ++j)
     {
         T* W = bufs[j & 1];
-        T* Vj = V + (size_t) (reorth ? j : (j & 1)) * blk;
-        T* Vjm1 = j > 0 ? V + (size_t) (reorth ? j - 1 : ((j - 1) & 1)) * blk : nullptr;
+        T* Vj = V + (size_t) (reorth ? j : (j & 1)) * vst;
+        T* Vjm1 = j > 0 ? V + (size_t) (reorth ? j - 1 : ((j - 1) & 1)) * vst : nullptr;
 
         SpmmArgs<T> s = spmm_defaults<T>();
         fill_matrix<T>(s, A, op->kind == OP_AFFINE_B ? op->B.get() : nullptr, false, f32);
@@ -1529,7 +1584,7 @@ int lanczos(const RecurrenceArgs& ra, Bump& bump)
         if (reorth)
         {
             const int num_ortho = j < buffer ? j + 1 : buffer;
-            SLQ_TRY(reorthogonalize<T>(op, n, W, V, j, num_ortho, h1, h2, partial_k, rb,
+            SLQ_TRY(reorthogonalize<T>(op, n, W, V, vst, j, num_ortho, h1, h2, partial_k, rb,
                                        ra.beta + (size_t) j * kBlock, scale, ra.sizes, j,
                                        tol_sqrt_n));
         }
@@ -1560,8 +1615,9 @@ int golub_kahn(const RecurrenceArgs& ra, Bump& bump)
     const int nslots = reorth ? m + 1 : 3;
     const size_t ublk = (size_t) nr * kBlock, vblk = (size_t) nc * kBlock;
 
-    T* U = bump.take<T>(ublk * nslots);
-    T* V = bump.take<T>(vblk * nslots);
+    const size_t ust = slot_stride(nr), vst = slot_stride(nc);
+    T* U = bump.take<T>(ust * nslots);
+    T* V = bump.take<T>(vst * nslots);
     double* su = bump.take<double>((size_t) (m + 1) * kBlock);   // 1/alpha_j
     double* sv = bump.take<double>((size_t) (m + 1) * kBlock);   // 1/beta_{j-1}
     const RedBuf rb = take_redbuf(bump, sched_tiles(op));
@@ -1589,8 +1645,8 @@ int golub_kahn(const RecurrenceArgs& ra, Bump& bump)
     hipLaunchKernelGGL(init_block_kernel, dim3(1), dim3(kBlock), 0, op->stream,
                        su, usizes, kBlock, 0.0);
     SLQ_CHECK(hipGetLastError());
-    auto uslot = [&](int j) { return U + (size_t) (reorth ? j : j % 3) * ublk; };
-    auto vslot = [&](int j) { return V + (size_t) (reorth ? j : j % 3) * vblk; };
+    auto uslot = [&](int j) { return U + (size_t) (reorth ? j : j % 3) * ust; };
+    auto vslot = [&](int j) { return V + (size_t) (reorth ? j : j % 3) * vst; };
     {
         const int64_t g = ((int64_t) vblk + 255) / 256;
         hipLaunchKernelGGL(copy_block_kernel<T>, dim3((unsigned) g), dim3(256), 0,
@@ -1635,7 +1691,7 @@ int golub_kahn(const RecurrenceArgs& ra, Bump& bump)
         if (reorth)
         {
             const int num_ortho = j < buffer ? j : buffer - 1;
-            SLQ_TRY(reorthogonalize<T>(op, nr, uslot(j), U, j - 1, j > 0 ? num_ortho : 0,
+            SLQ_TRY(reorthogonalize<T>(op, nr, uslot(j), U, ust, j - 1, j > 0 ? num_ortho : 0,
                                        h1, h2, partial_k, rb, ra.alpha + (size_t) j * kBlock,
                                        su + (size_t) j * kBlock, usizes, j, -1.0));
             SLQ_TRY(normalize_in_place<T>(op, uslot(j), nr, su + (size_t) j * kBlock));
@@ -1667,7 +1723,7 @@ int golub_kahn(const RecurrenceArgs& ra, Bump& bump)
         if (reorth)
         {
             const int num_ortho = j < buffer ? j : buffer - 1;
-            SLQ_TRY(reorthogonalize<T>(op, nc, vslot(j + 1), V, j, num_ortho, h1, h2,
+            SLQ_TRY(reorthogonalize<T>(op, nc, vslot(j + 1), V, vst, j, num_ortho, h1, h2,
                                        partial_k, rb, ra.beta + (size_t) j * kBlock,
                                        sv + (size_t) (j + 1) * kBlock, ra.sizes, j,
                                        tol_sqrt_n));
@@ -1696,7 +1752,7 @@ size_t recurrence_scratch_bytes(const slq_op* op, int gram, int m, int ortho,
         {
             add(blk);
             add(blk);
-            add(blk * m);
+            add(slot_stride(nr) * value_bytes * m);
             add(sizeof(double) * kBlock);
         }
         else
@@ -1708,8 +1764,8 @@ size_t recurrence_scratch_bytes(const slq_op* op, int gram, int m, int ortho,
     else
     {
         const int ns = reorth ? m + 1 : 3;
-        add((size_t) nr * kBlock * value_bytes * ns);
-        add((size_t) nc * kBlock * value_bytes * ns);
+        add(slot_stride(nr) * value_bytes * ns);
+        add(slot_stride(nc) * value_bytes * ns);
         add(sizeof(double) * (m + 1) * kBlock);
         add(sizeof(double) * (m + 1) * kBlock);
         add(sizeof(int32_t) * kBlock);
