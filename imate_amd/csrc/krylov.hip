@@ -895,20 +895,9 @@ __global__ __launch_bounds__(kTotalWaves * kWave) void norm_total_kernel(
 constexpr int kGsChunk = 32;     // basis vectors per pass over the rows
 
 // Element stride between the stored basis vectors of a re-orthogonalised
-// recurrence: (rows + pad) x 64. The Gram-Schmidt kernels read the same
-// row of up to kGsChunk vectors at once; with an unpadded power-of-two
-// stride those addresses all fall in the same memory channel (an FP32
-// block of 2^16 rows is exactly 16 MiB). SLQ_SLOT_PAD overrides the pad.
-int slot_pad_rows()
-{
-    static int v = [] {
-        const char* e = std::getenv("SLQ_SLOT_PAD");
-        return e ? std::atoi(e) : 3;
-    }();
-    return v;
-}
-
-size_t slot_stride(int64_t rows) { return (size_t) (rows + slot_pad_rows()) * kBlock; }
+// recurrence. (Padding it away from a power of two was measured and made no
+// difference to the Gram-Schmidt kernels.)
+size_t slot_stride(int64_t rows) { return (size_t) rows * kBlock; }
 
 struct GsArgs {
     int64_t nrows;
@@ -1613,7 +1602,7 @@ int golub_kahn(const RecurrenceArgs& ra, Bump& bump)
     else if (ra.ortho < 0 || ra.ortho > m - 1) buffer = m;
     else buffer = ra.ortho + 1;
     const int nslots = reorth ? m + 1 : 3;
-    const size_t ublk = (size_t) nr * kBlock, vblk = (size_t) nc * kBlock;
+    const size_t vblk = (size_t) nc * kBlock;
 
     const size_t ust = slot_stride(nr), vst = slot_stride(nc);
     T* U = bump.take<T>(ust * nslots);
