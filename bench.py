@@ -98,17 +98,18 @@ def cholesky_reference(n, k, seed):
     return None
 
 
-def cpu_baseline(A, m, threads, seed):
+def cpu_baseline(A, m, threads, seed, per_thread=8):
     """Oracle restatement of imate's CPU SLQ recurrences on `threads` host
-    threads, two probes per thread (same matrix, same lanczos_degree)."""
+    threads, `per_thread` probes per thread (same matrix, same
+    lanczos_degree): about 11 s at n = 2^20 on 16 threads."""
     from oracle import oracle
     op = oracle.Operator(A)
     eps = float(numpy.finfo(numpy.float64).eps)
     # one short warm-up to page in the library and matrix
     oracle.recurrences(op, seed, 1, 2, eps, 0, num_threads=1)
     t = time.perf_counter()
-    a, b, sizes = oracle.recurrences(op, seed, 2 * threads, m, eps, 0,
-                                     num_threads=threads)
+    a, b, sizes = oracle.recurrences(op, seed, per_thread * threads, m, eps,
+                                     0, num_threads=threads)
     dt = time.perf_counter() - t
     steps = int(numpy.sum(sizes))
     return steps / dt, dt, steps
@@ -582,9 +583,9 @@ def main():
         rate, dt, steps = cpu_baseline(A, m, threads, args.seed)
         cpu = {'value': rate, 'unit': 'matvec-steps/s', 'cores': threads,
                'kind': 'port',
-               'sample': '%d probes x %d Lanczos steps (two probes per '
+               'sample': '%d probes x %d Lanczos steps (eight probes per '
                          'thread) on the same n=%d matrix: %d steps in %.1fs'
-                         % (2 * threads, m, n, steps, dt)}
+                         % (8 * threads, m, n, steps, dt)}
         log('cpu baseline: %.1f matvec-steps/s on %d threads' % (rate, threads))
 
     out = {

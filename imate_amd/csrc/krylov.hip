@@ -563,6 +563,94 @@ struct Mfma16<float> {
     }
 };
 
+// Four consecutive values p[0..3] (16 B aligned): one 32 B row segment as
+// two 16 B loads (FP64) or one (FP32).
+__device__ __forceinline__ void load_row4(const double* __restrict__ p, double (&v)[4])
+{
+    typedef double D2 __attribute__((ext_vector_type(2)));
+    const D2 lo = *reinterpret_cast<const D2*>(p);
+    const D2 hi = *reinterpret_cast<const D2*>(p + 2);
+    v[0] = lo.x;
+    v[1] = lo.y;
+    v[2] = hi.x;
+    v[3] = hi.y;
+}
+
+__device__ __forceinline__ void load_row4(const float* __restrict__ p, float (&v)[4])
+{
+    typedef float F4 __attribute__((ext_vector_type(4)));
+    const F4 w = *reinterpret_cast<const F4*>(p);
+    v[0] = w.x;
+    v[1] = w.y;
+    v[2] = w.z;
+    v[3] = w.w;
+}
+
+template <bool NT>
+__device__ __forceinline__ void store_row4(double* __restrict__ p, const double (&v)[4])
+{
+    typedef double D2 __attribute__((ext_vector_type(2)));
+    const D2 lo = {v[0], v[1]}, hi = {v[2], v[3]};
+    if constexpr (NT)
+    {
+        __builtin_nontemporal_store(lo, reinterpret_cast<D2*>(p));
+        __builtin_nontemporal_store(hi, reinterpret_cast<D2*>(p + 2));
+    }
+    else
+    {
+        *reinterpret_cast<D2*>(p) = lo;
+        *reinterpret_cast<D2*>(p + 2) = hi;
+    }
+}
+
+template <bool NT>
+__device__ __forceinline__ void store_row4(float* __restrict__ p, const float (&v)[4])
+{
+    typedef float F4 __attribute__((ext_vector_type(4)));
+    const F4 w = {v[0], v[1], v[2], v[3]};
+    if constexpr (NT) __builtin_nontemporal_store(w, reinterpret_cast<F4*>(p));
+    else *reinterpret_cast<F4*>(p) = w;
+}
+
+// spmm_epilogue for the four consecutive probes 4 i0 .. 4 i0 + 3 of one
+// row, with 16 B loads and stores (the same arithmetic per probe).
+template <typename T, bool NT>
+__device__ __forceinline__ void spmm_epilogue4(const SpmmArgs<T>& a, int64_t row, int i0,
+                                               const T (&yin)[4], const T (&s)[4], T tt,
+                                               const T (&sub_c)[4], const T (&sub_s)[4],
+                                               const T* __restrict__ X, T* __restrict__ Y,
+                                               T* __restrict__ Vout, const T* __restrict__ Sub,
+                                               double (&red)[4])
+{
+    const int64_t o = row * kBlock + 4 * i0;
+    const bool shift_row = a.shift_mode == 1 && row < a.shift_len;
+    T v[4] = {T(0), T(0), T(0), T(0)}, y[4], sb[4];
+    if (Vout || a.red_mode == 1 || shift_row)
+    {
+        T xv[4];
+        load_row4(X + o, xv);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) v[q] = s[q] * xv[q];
+    }
+    if (Sub) load_row4(Sub + o, sb);
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+    {
+        y[q] = yin[q];
+        // imate adds t x through subtract_scaled_vector(-t): y - (-t) x
+        if (shift_row) y[q] = y[q] + tt * v[q];
+        if (Sub) y[q] = y[q] - sub_c[q] * (sub_s[q] * sb[q]);
+    }
+    if (Vout) store_row4<NT>(Vout + o, v);
+    store_row4<NT>(Y + o, y);
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+    {
+        if (a.red_mode == 1) red[q] += (double) v[q] * (double) y[q];
+        else if (a.red_mode == 2) red[q] += (double) y[q] * (double) y[q];
+    }
+}
+
 // Tiled SpMM on the matrix cores. A 16-row tile times the gathered operand
 // rows of its slabs is a dense product (16 x slabs) (slabs x 64 probes):
 // four slabs at a time, A = the 16 x 4 slab values (64 consecutive values of
@@ -595,9 +683,11 @@ __global__ __launch_bounds__(kThreads) void mfma_spmm_kernel(
     const unsigned int waves = (unsigned int) ((gridDim.x - x + V - 1) / V) * kWavesPerWG;
     unsigned int* ticket = a.rb.tickets + 64 * x;
     unsigned int* done = ticket + 32;
+    // block q of the four 16-probe MFMA blocks holds probes 4 i0 + q, so a
+    // lane's four B values of one gathered row are contiguous
     T sq[4];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) sq[q] = a.xscale ? (T) a.xscale[16 * q + i0] : T(1);
+    for (int q = 0; q < 4; ++q) sq[q] = a.xscale ? (T) a.xscale[4 * i0 + q] : T(1);
     const T tt = (T) a.t;
     for (;;)
     {
@@ -626,27 +716,21 @@ __global__ __launch_bounds__(kThreads) void mfma_spmm_kernel(
             T a0, a1, b0[4], b1[4];
             {
                 a0 = tvals[base * 16 + lane];
-                const int64_t c = col_of(0);
-#pragma unroll
-                for (int q = 0; q < 4; ++q) b0[q] = X[c * kBlock + 16 * q + i0];
+                load_row4(X + col_of(0) * kBlock + 4 * i0, b0);
             }
             int g = 0;
             for (; g + 2 <= ng; g += 2)
             {
                 {
                     a1 = tvals[(base + 4 * (g + 1)) * 16 + lane];
-                    const int64_t c = col_of(g + 1);
-#pragma unroll
-                    for (int q = 0; q < 4; ++q) b1[q] = X[c * kBlock + 16 * q + i0];
+                    load_row4(X + col_of(g + 1) * kBlock + 4 * i0, b1);
                 }
 #pragma unroll
                 for (int q = 0; q < 4; ++q) acc[q] = Mfma16<T>::op(a0, sq[q] * b0[q], acc[q]);
                 if (g + 2 < ng)
                 {
                     a0 = tvals[(base + 4 * (g + 2)) * 16 + lane];
-                    const int64_t c = col_of(g + 2);
-#pragma unroll
-                    for (int q = 0; q < 4; ++q) b0[q] = X[c * kBlock + 16 * q + i0];
+                    load_row4(X + col_of(g + 2) * kBlock + 4 * i0, b0);
                 }
 #pragma unroll
                 for (int q = 0; q < 4; ++q) acc[q] = Mfma16<T>::op(a1, sq[q] * b1[q], acc[q]);
@@ -658,21 +742,22 @@ __global__ __launch_bounds__(kThreads) void mfma_spmm_kernel(
             }
         }
         double red[4] = {0.0, 0.0, 0.0, 0.0};
+        T sub_c[4], sub_s[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q)
         {
-            const int p = 16 * q + i0;
-            const T sub_c = Sub ? (T) a.sub_coef[p] : T(0);
-            const T sub_s = Sub ? (T) a.sub_scale[p] : T(0);
+            sub_c[q] = Sub ? (T) a.sub_coef[4 * i0 + q] : T(0);
+            sub_s[q] = Sub ? (T) a.sub_scale[4 * i0 + q] : T(0);
+        }
 #pragma unroll
-            for (int m = 0; m < 4; ++m)
-            {
-                const int64_t row = t * 16 +
-                                    (Mfma16<T>::kRowStride == 4 ? kq + 4 * m : 4 * kq + m);
-                if (row < a.nrows)
-                    spmm_epilogue<T, NT>(a, row, p, acc[q][m], sq[q], tt, sub_c, sub_s, X, Y,
-                                         Vout, Sub, red[q]);
-            }
+        for (int m = 0; m < 4; ++m)
+        {
+            const int64_t row = t * 16 +
+                                (Mfma16<T>::kRowStride == 4 ? kq + 4 * m : 4 * kq + m);
+            const T y[4] = {acc[0][m], acc[1][m], acc[2][m], acc[3][m]};
+            if (row < a.nrows)
+                spmm_epilogue4<T, NT>(a, row, i0, y, sq, tt, sub_c, sub_s, X, Y, Vout, Sub,
+                                      red);
         }
         if (a.red_mode)
         {
@@ -684,7 +769,7 @@ __global__ __launch_bounds__(kThreads) void mfma_spmm_kernel(
                 const double r2 = r1 + __shfl_xor(r1, 32);
                 if (q == kq) mine = r2;
             }
-            a.rb.tile_partial[t * kWave + 16 * kq + i0] = mine;
+            a.rb.tile_partial[t * kWave + 4 * i0 + kq] = mine;
         }
     }
     unsigned int d = 0;

@@ -84,7 +84,8 @@ __device__ __forceinline__ double ffunc(int func, const double* fp, double x)
 
 // Implicit QL on (d, e) of size K (e[i] couples i and i+1, e[K-1] = 0),
 // tracking z = first row of the eigenvector matrix. Arrays are strided by S.
-__device__ int ql_first_row(double* d, double* e, double* z, int K, int S)
+__device__ int ql_first_row(double* __restrict__ d, double* __restrict__ e,
+                            double* __restrict__ z, int K, int S)
 {
 #define D(i) d[(i) * S]
 #define E(i) e[(i) * S]
@@ -107,32 +108,41 @@ __device__ int ql_first_row(double* d, double* e, double* z, int K, int S)
             g = D(mm) - D(l) + E(l) / (g + copysign(r, g));
             double s = 1.0, c = 1.0, p = 0.0;
             bool early = false;
+            // The rotation sweep carries D(i + 1) (never written before it
+            // is read) and the rotated Z(i + 1) in registers and loads the
+            // next E, D, Z ahead, so the dependent chain runs through
+            // arithmetic rather than LDS round trips; the arithmetic and
+            // its order are unchanged.
+            double d1 = D(mm), z1 = Z(mm);
             for (int i = mm - 1; i >= l; --i)
             {
-                const double f = s * E(i);
-                const double b = c * E(i);
+                const double ei = E(i), di = D(i), zi = Z(i);
+                const double f = s * ei;
+                const double b = c * ei;
                 r = hypot(f, g);
                 E(i + 1) = r;
                 if (r == 0.0)
                 {
-                    D(i + 1) -= p;
+                    D(i + 1) = d1 - p;
+                    Z(i + 1) = z1;
                     E(mm) = 0.0;
                     early = true;
                     break;
                 }
                 s = f / r;
                 c = g / r;
-                g = D(i + 1) - p;
-                r = (D(i) - g) * s + 2.0 * c * b;
+                g = d1 - p;
+                r = (di - g) * s + 2.0 * c * b;
                 p = s * r;
                 D(i + 1) = g + p;
                 g = c * r - b;
-                const double zf = Z(i + 1);
-                Z(i + 1) = s * Z(i) + c * zf;
-                Z(i) = c * Z(i) - s * zf;
+                Z(i + 1) = s * zi + c * z1;
+                z1 = c * zi - s * z1;
+                d1 = di;
             }
             if (early) continue;
-            D(l) -= p;
+            Z(l) = z1;
+            D(l) = d1 - p;
             E(l) = g;
             E(mm) = 0.0;
         }
