@@ -104,6 +104,19 @@ int Workspace::reserve(size_t need)
 
 // ---------------------------------------------------------------- tiles
 
+// Position of the value (slab, row r) in the tile value array. R = 8: slab
+// by slab, R values each. R = 16 (matrix-core layout): a group of four
+// slabs is one MFMA A operand, lane l = 16 (slab % 4) + r; the two groups of
+// each eight slabs are interleaved per lane ([slab / 8][lane][2]) so that a
+// lane loads both of its A values with one access.
+size_t tile_slot(int R, int64_t slab, int r)
+{
+    if (R != 16) return (size_t) slab * R + r;
+    const int64_t pair = slab >> 3, half = (slab >> 2) & 1;
+    const int lane = 16 * (int) (slab & 3) + r;
+    return (size_t) pair * 128 + (size_t) lane * 2 + (size_t) half;
+}
+
 int build_tiles(int64_t nrows, int64_t ncols, const int64_t* indptr,
                 const int32_t* indices, const void* data, int dtype, int R,
                 double min_density, DevTiles& out)
@@ -125,11 +138,12 @@ int build_tiles(int64_t nrows, int64_t ncols, const int64_t* indptr,
             buf.insert(buf.end(), indices + indptr[row], indices + indptr[row + 1]);
         std::sort(buf.begin(), buf.end());
         buf.erase(std::unique(buf.begin(), buf.end()), buf.end());
-        // 16-row tiles feed the matrix-core SpMM four slabs at a time: pad
-        // each tile's slab list to a multiple of four with zero slabs that
-        // repeat its last column (a valid row of the operand)
+        // 16-row tiles feed the matrix-core SpMM eight slabs (two MFMA
+        // groups) at a time: pad each tile's slab list to a multiple of
+        // eight with zero slabs that repeat its last column (a valid row of
+        // the operand)
         if (R == 16 && !buf.empty())
-            while (buf.size() % 4) buf.push_back(buf.back());
+            while (buf.size() % 8) buf.push_back(buf.back());
         cols.insert(cols.end(), buf.begin(), buf.end());
         tptr[t + 1] = (int64_t) cols.size();
     }
@@ -152,7 +166,7 @@ int build_tiles(int64_t nrows, int64_t ncols, const int64_t* indptr,
             for (int64_t k = indptr[row]; k < indptr[row + 1]; ++k)
             {
                 p = std::lower_bound(p, ce, indices[k]);
-                const size_t slot = (size_t) (p - cols.data()) * R + r;
+                const size_t slot = tile_slot(R, p - cols.data(), r);
                 if (dtype == SLQ_F64)
                     ((double*) vals.data())[slot] += ((const double*) data)[k];
                 else

@@ -563,6 +563,19 @@ struct Mfma16<float> {
     }
 };
 
+template <typename T>
+struct Pair2Of;
+template <>
+struct Pair2Of<double> {
+    typedef double type __attribute__((ext_vector_type(2)));
+};
+template <>
+struct Pair2Of<float> {
+    typedef float type __attribute__((ext_vector_type(2)));
+};
+template <typename T>
+using Pair2 = typename Pair2Of<T>::type;
+
 // Four consecutive values p[0..3] (16 B aligned): one 32 B row segment as
 // two 16 B loads (FP64) or one (FP32).
 __device__ __forceinline__ void load_row4(const double* __restrict__ p, double (&v)[4])
@@ -712,33 +725,27 @@ __global__ __launch_bounds__(kThreads) void mfma_spmm_kernel(
                 const int c3 = readlane(cl, 4 * g + 3);
                 return (int64_t) (kq == 0 ? c0 : (kq == 1 ? c1 : (kq == 2 ? c2 : c3)));
             };
-            const int ng = cnt / 4;
-            T a0, a1, b0[4], b1[4];
+            // pairs of MFMA groups (eight slabs): one access loads a lane's
+            // A values of both groups
+            const int np = cnt / 8;
+            const Pair2<T>* __restrict__ ap2 =
+                reinterpret_cast<const Pair2<T>*>(tvals + base * 16) + lane;
+            Pair2<T> ap = ap2[0], an;
+            T b0[4], b1[4];
+            load_row4(X + col_of(0) * kBlock + 4 * i0, b0);
+            for (int P = 0; P < np; ++P)
             {
-                a0 = tvals[base * 16 + lane];
-                load_row4(X + col_of(0) * kBlock + 4 * i0, b0);
-            }
-            int g = 0;
-            for (; g + 2 <= ng; g += 2)
-            {
+                load_row4(X + col_of(2 * P + 1) * kBlock + 4 * i0, b1);
+#pragma unroll
+                for (int q = 0; q < 4; ++q) acc[q] = Mfma16<T>::op(ap.x, sq[q] * b0[q], acc[q]);
+                if (P + 1 < np)
                 {
-                    a1 = tvals[(base + 4 * (g + 1)) * 16 + lane];
-                    load_row4(X + col_of(g + 1) * kBlock + 4 * i0, b1);
+                    an = ap2[(P + 1) * kWave];
+                    load_row4(X + col_of(2 * P + 2) * kBlock + 4 * i0, b0);
                 }
 #pragma unroll
-                for (int q = 0; q < 4; ++q) acc[q] = Mfma16<T>::op(a0, sq[q] * b0[q], acc[q]);
-                if (g + 2 < ng)
-                {
-                    a0 = tvals[(base + 4 * (g + 2)) * 16 + lane];
-                    load_row4(X + col_of(g + 2) * kBlock + 4 * i0, b0);
-                }
-#pragma unroll
-                for (int q = 0; q < 4; ++q) acc[q] = Mfma16<T>::op(a1, sq[q] * b1[q], acc[q]);
-            }
-            if (g < ng)
-            {
-#pragma unroll
-                for (int q = 0; q < 4; ++q) acc[q] = Mfma16<T>::op(a0, sq[q] * b0[q], acc[q]);
+                for (int q = 0; q < 4; ++q) acc[q] = Mfma16<T>::op(ap.y, sq[q] * b1[q], acc[q]);
+                ap = an;
             }
         }
         double red[4] = {0.0, 0.0, 0.0, 0.0};
@@ -1307,11 +1314,11 @@ int nt_default()
     return v;
 }
 
-// Tiled SpMM schedule: 1 (default) = tiles handed out in order (16-row
-// tiles on the matrix cores, 8-row tiles with scalar-loaded slabs), 0 = the
-// static interleaved sweep (tiled_spmm_kernel). SLQ_SPMM overrides; the
-// static form is kept for comparison and for launches without ticket
-// counters.
+// Schedule of 8-row tiles: 1 (default) = handed out in order
+// (tiled_spmm_sched_kernel), 0 = the static interleaved sweep
+// (tiled_spmm_kernel), kept for comparison and for launches without ticket
+// counters. SLQ_SPMM overrides. 16-row tiles always run the in-order
+// matrix-core kernel (their value layout is the MFMA operand layout).
 int spmm_variant()
 {
     static int v = [] {
@@ -1359,8 +1366,10 @@ void launch_spmm_nt(slq_op* op, const SpmmArgs<T>& s, const SpmmIO<T>& io, const
         const T* tv = (sizeof(T) == 4 && op->A->csr.dtype == SLQ_F64)
                           ? (const T*) tl->vals_f32 : (const T*) tl->vals;
         const int grid = grid_for(tl->ntiles, op->spmm_grid);
-        const bool sched = spmm_variant() != 0 && s.rb.tickets &&
-                           (s.red_mode == 0 || s.rb.tile_partial);
+        // 16-row tiles (matrix-core layout) always take the in-order
+        // schedule; launch_spmm has checked its buffers
+        const bool sched = tl->R == 16 || (spmm_variant() != 0 && s.rb.tickets &&
+                                           (s.red_mode == 0 || s.rb.tile_partial));
         if (sched)
         {
             if (tl->R == 16)
@@ -1384,14 +1393,9 @@ void launch_spmm_nt(slq_op* op, const SpmmArgs<T>& s, const SpmmIO<T>& io, const
         }
         else
         {
-            if (tl->R == 8)
-                hipLaunchKernelGGL((tiled_spmm_kernel<T, 8, NT>), dim3(grid), dim3(kThreads), 0,
-                                   op->stream, s, io.X, io.Y, io.Vout, io.Sub, tl->ntiles,
-                                   tl->ptr, tl->cols, tv);
-            else
-                hipLaunchKernelGGL((tiled_spmm_kernel<T, 16, NT>), dim3(grid), dim3(kThreads),
-                                   0, op->stream, s, io.X, io.Y, io.Vout, io.Sub, tl->ntiles,
-                                   tl->ptr, tl->cols, tv);
+            hipLaunchKernelGGL((tiled_spmm_kernel<T, 8, NT>), dim3(grid), dim3(kThreads), 0,
+                               op->stream, s, io.X, io.Y, io.Vout, io.Sub, tl->ntiles, tl->ptr,
+                               tl->cols, tv);
             if (s.red_mode)
                 hipLaunchKernelGGL((total_kernel<T>), dim3(1), dim3(kTotalWaves * kWave), 0,
                                    op->stream, s, grid);
@@ -1412,6 +1416,11 @@ template <typename T>
 int launch_spmm(slq_op* op, const SpmmArgs<T>& s, const SpmmIO<T>& io,
                 const DevTiles* tl = nullptr)
 {
+    if (tl && tl->R == 16 && (!s.rb.tickets || (s.red_mode && !s.rb.tile_partial)))
+    {
+        set_error("internal: the matrix-core SpMM needs ticket counters and tile partials");
+        return SLQ_E_ARG;
+    }
     hipEvent_t ev = nullptr;
     prof_begin(op, &ev);
     if (s.nt_store) launch_spmm_nt<T, true>(op, s, io, tl);
