@@ -10,10 +10,12 @@
 //
 // Parallel generation: the state transition of xoshiro256** is linear over
 // GF(2), so the state k steps ahead is M^k s. The host builds the 256 x 256
-// bit matrices M^(2^e), e = 0..63, once; each device thread jumps straight to
-// its (probe, chunk) offset by multiplying the base state with the matrices of
-// the set bits of the offset, then steps its chunk of words sequentially.
-// A second kernel expands the packed words into the [len][64] probe block.
+// bit matrices M^(2^e), e = 0..63, once; each wavefront jumps to its chunk
+// group's offset by multiplying the base state with the matrices of the set
+// bits of the offset (each product shared by the 64 lanes), hands each lane
+// the state of its chunk, and every lane steps its chunk of words
+// sequentially. A second kernel expands the packed words into the [len][64]
+// probe block.
 
 #include <mutex>
 #include <vector>
@@ -118,47 +120,77 @@ __device__ __forceinline__ uint64_t rotl_d(uint64_t x, int k)
     return (x << k) | (x >> (64 - k));
 }
 
-// y = M x over GF(2), M column-major [256][4] in global memory (the same
-// column is read by every lane of a wavefront at once).
-__device__ __forceinline__ void apply_d(const uint64_t* __restrict__ M,
-                                        uint64_t& x0, uint64_t& x1,
-                                        uint64_t& x2, uint64_t& x3)
+// y = M x over GF(2) by the 64 lanes of a wavefront together (x uniform
+// across the wavefront): lane l takes columns 4 l .. 4 l + 3 of M (column
+// major [256][4] in global memory), XORs those selected by its four bits of
+// x, and a butterfly of XOR shuffles gives every lane the full product.
+__device__ __forceinline__ void apply_wave(const uint64_t* __restrict__ M, const uint64_t (&x)[4],
+                                           uint64_t (&y)[4], int lane)
 {
-    uint64_t r0 = 0, r1 = 0, r2 = 0, r3 = 0;
-    const uint64_t xs[4] = {x0, x1, x2, x3};
+    const uint64_t xw = lane < 16 ? x[0] : (lane < 32 ? x[1] : (lane < 48 ? x[2] : x[3]));
+    const uint32_t bits = (uint32_t) (xw >> (4 * (lane & 15))) & 0xFu;
+    const uint64_t* __restrict__ c = M + (size_t) lane * 16;
+    uint64_t r[4] = {0, 0, 0, 0};
 #pragma unroll
-    for (int w = 0; w < 4; ++w)
+    for (int i = 0; i < 4; ++i)
     {
-        const uint64_t xw = xs[w];
-        const uint64_t* c = M + (size_t) w * 64 * 4;
-        for (int b = 0; b < 64; ++b)
-        {
-            const uint64_t mask = 0 - ((xw >> b) & 1);
-            r0 ^= c[b * 4 + 0] & mask;
-            r1 ^= c[b * 4 + 1] & mask;
-            r2 ^= c[b * 4 + 2] & mask;
-            r3 ^= c[b * 4 + 3] & mask;
-        }
+        const uint64_t mask = 0 - (uint64_t) ((bits >> i) & 1u);
+#pragma unroll
+        for (int w = 0; w < 4; ++w) r[w] ^= c[4 * i + w] & mask;
     }
-    x0 = r0; x1 = r1; x2 = r2; x3 = r3;
+#pragma unroll
+    for (int sh = 1; sh < kWave; sh <<= 1)
+    {
+#pragma unroll
+        for (int w = 0; w < 4; ++w) r[w] ^= __shfl_xor(r[w], sh);
+    }
+#pragma unroll
+    for (int w = 0; w < 4; ++w) y[w] = r[w];
 }
 
-// Thread (p, c): words [c*L, min((c+1)*L, W)) of probe p, i.e. stream words
-// starting at offset (first_sample + p) * W + c * L from the base state.
+static_assert(kChunkWords == 64, "lane states are spaced by jump tables 6 + k");
+
+// One wavefront per (probe p, group g of 64 chunks): lane l generates words
+// [c L, min((c + 1) L, W)) of probe p, c = 64 g + l, i.e. stream words from
+// offset (first_sample + p) W + c L. The wavefront jumps the base state to
+// the group's first chunk with the matrices of the set bits of that offset,
+// then spreads it to its lanes by doubling (lanes s + 2^k take M^(2^k L) of
+// lane s, k = 0..5), every product computed by the whole wavefront.
 __global__ __launch_bounds__(256) void rng_words_kernel(
     const uint64_t* __restrict__ tables, uint64_t s0, uint64_t s1, uint64_t s2,
-    uint64_t s3, int64_t W, int64_t first_sample, int32_t count,
-    int64_t chunks, uint64_t* __restrict__ words)
+    uint64_t s3, int64_t W, int64_t first_sample, int32_t count, int64_t chunks,
+    int64_t groups, uint64_t* __restrict__ words)
 {
-    const int64_t gid = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t p = gid / chunks;
-    const int64_t c = gid - p * chunks;
+    const int lane = threadIdx.x % kWave;
+    const int64_t wv = ((int64_t) blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+    const int64_t p = wv / groups;
+    const int64_t g = wv - p * groups;
     if (p >= count) return;
-    const uint64_t offset = (uint64_t)(first_sample + p) * (uint64_t) W +
-                            (uint64_t)(c * kChunkWords);
-    uint64_t x0 = s0, x1 = s1, x2 = s2, x3 = s3;
+    const uint64_t offset = (uint64_t) (first_sample + p) * (uint64_t) W +
+                            (uint64_t) (g * kWave * kChunkWords);
+    uint64_t x[4] = {s0, s1, s2, s3};
     for (int e = 0; e < 64; ++e)
-        if ((offset >> e) & 1) apply_d(tables + (size_t) e * 256 * 4, x0, x1, x2, x3);
+        if ((offset >> e) & 1) apply_wave(tables + (size_t) e * 256 * 4, x, x, lane);
+    uint64_t st[4] = {x[0], x[1], x[2], x[3]};
+    for (int k = 0; k < 6; ++k)
+    {
+        const int half = 1 << k;
+        for (int src = 0; src < half; ++src)
+        {
+            uint64_t v[4], r[4];
+#pragma unroll
+            for (int w = 0; w < 4; ++w) v[w] = __shfl(st[w], src);
+            apply_wave(tables + (size_t) (6 + k) * 256 * 4, v, r, lane);
+            if (lane == src + half)
+            {
+#pragma unroll
+                for (int w = 0; w < 4; ++w) st[w] = r[w];
+            }
+        }
+    }
+    const int64_t c = g * kWave + lane;
+    if (c >= chunks) return;
+    uint64_t x0 = st[0], x1 = st[1], x2 = st[2], x3 = st[3];
     uint64_t* out = words + p * W;
     const int64_t beg = c * kChunkWords;
     const int64_t end = (beg + kChunkWords < W) ? beg + kChunkWords : W;
@@ -176,23 +208,29 @@ __global__ __launch_bounds__(256) void rng_words_kernel(
 }
 
 // X[i][p] = +1 / -1 from bit (i mod 64) of word floor(i/64) of probe p; zero
-// for p >= count. One wavefront per row i, lane = probe.
+// for p >= count. A thread writes two probes of a row (one access).
 template <typename T>
 __global__ __launch_bounds__(256) void rng_expand_kernel(
     const uint64_t* __restrict__ words, int64_t W, int64_t len, int32_t count,
     T* __restrict__ X)
 {
+    typedef T T2 __attribute__((ext_vector_type(2)));
     const int64_t gid = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t i = gid / kBlock;
-    const int p = (int) (gid - i * kBlock);
+    const int64_t i = gid / (kBlock / 2);
+    const int p = 2 * (int) (gid - i * (kBlock / 2));
     if (i >= len) return;
-    T v = 0;
+    T2 v = {T(0), T(0)};
     if (p < count)
     {
         const uint64_t w = words[(int64_t) p * W + (i >> 6)];
-        v = ((w >> (i & 63)) & 1) ? T(1) : T(-1);
+        v.x = ((w >> (i & 63)) & 1) ? T(1) : T(-1);
     }
-    X[gid] = v;
+    if (p + 1 < count)
+    {
+        const uint64_t w = words[(int64_t) (p + 1) * W + (i >> 6)];
+        v.y = ((w >> (i & 63)) & 1) ? T(1) : T(-1);
+    }
+    *reinterpret_cast<T2*>(X + i * kBlock + p) = v;
 }
 
 }  // namespace
@@ -237,11 +275,12 @@ int rng_words(hipStream_t s, const uint64_t base[4], int64_t W,
     }
     if (count <= 0) return SLQ_OK;
     const int64_t chunks = (W + kChunkWords - 1) / kChunkWords;
-    const int64_t threads = chunks * count;
+    const int64_t groups = (chunks + kWave - 1) / kWave;
+    const int64_t threads = groups * count * kWave;
     const int64_t grid = (threads + 255) / 256;
     hipLaunchKernelGGL(rng_words_kernel, dim3((unsigned) grid), dim3(256), 0, s,
                        tables, base[0], base[1], base[2], base[3], W,
-                       first_sample, count, chunks, words);
+                       first_sample, count, chunks, groups, words);
     SLQ_CHECK(hipGetLastError());
     return SLQ_OK;
 }
@@ -249,7 +288,7 @@ int rng_words(hipStream_t s, const uint64_t base[4], int64_t W,
 int rng_expand(hipStream_t s, const uint64_t* words, int64_t W, int64_t len,
                int32_t count, int dtype, void* X)
 {
-    const int64_t threads = len * kBlock;
+    const int64_t threads = len * (kBlock / 2);
     const int64_t grid = (threads + 255) / 256;
     if (dtype == SLQ_F64)
         hipLaunchKernelGGL(rng_expand_kernel<double>, dim3((unsigned) grid),

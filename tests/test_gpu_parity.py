@@ -33,6 +33,25 @@ def test_rademacher_bit_exact(gpu_lib, n, seed):
     assert numpy.array_equal(words.ravel(), allw[W * first:])
 
 
+@pytest.mark.parametrize('n', [300000, 1 << 20])
+def test_rademacher_bit_exact_many_chunks(gpu_lib, n):
+    """Several 64-chunk groups per probe: every lane of the jump doubling and
+    the group offsets are exercised."""
+    count, first, seed = 5, 70, 7
+    W = n // 64 + 1
+    out = numpy.empty((count, n), dtype=numpy.float64)
+    words = numpy.empty((count, W), dtype=numpy.uint64)
+    _lib.check(gpu_lib.slq_rademacher(seed, n, first, count, _lib.SLQ_F64,
+                                      _lib.ptr(out), _lib.ptr(words)),
+               'slq_rademacher')
+    s = oracle.thread_state(seed, 0)
+    allw = oracle.xoshiro_words(s, W * (first + count))
+    assert numpy.array_equal(words.ravel(), allw[W * first:])
+    bits = numpy.unpackbits(words.view(numpy.uint8), axis=1,
+                            bitorder='little')[:, :n]
+    assert numpy.array_equal(out, numpy.where(bits == 1, 1.0, -1.0))
+
+
 def _op(A):
     from imate_amd.linear_operator import DeviceOperator
     return DeviceOperator.from_matrix(A)
