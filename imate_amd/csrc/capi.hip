@@ -424,6 +424,7 @@ static int build_transpose_chunked(hipStream_t s, const DevCSR& A, DevCSR& T)
     T.ncols = A.nrows;
     T.nnz = nnz;
     T.dtype = A.dtype;
+    T.sorted = true;   // each chunk is sorted stably by column: rows ascend
     SLQ_CHECK(hipMalloc(&T.indptr, sizeof(int64_t) * (T.nrows + 1)));
     SLQ_CHECK(hipMalloc(&T.indices, sizeof(int32_t) * nnz));
     SLQ_CHECK(hipMalloc(&T.data, vb * nnz));
@@ -560,6 +561,7 @@ int build_transpose(hipStream_t s, const DevCSR& A, DevCSR& T)
     T.ncols = A.nrows;
     T.nnz = nnz;
     T.dtype = A.dtype;
+    T.sorted = true;   // stable sort by column: the rows of A ascend
     SLQ_CHECK(hipMalloc(&T.indptr, sizeof(int64_t) * (T.nrows + 1)));
     SLQ_CHECK(hipMalloc(&T.indices, sizeof(int32_t) * (nnz > 0 ? nnz : 1)));
     SLQ_CHECK(hipMalloc(&T.data, vb * (nnz > 0 ? nnz : 1)));
@@ -1333,6 +1335,14 @@ int slq_op_create_csr(int64_t nrows, int64_t ncols, int64_t nnz,
     c.ncols = ncols;
     c.nnz = nnz;
     c.dtype = dtype;
+    c.sorted = true;
+    for (int64_t r = 0; r < nrows && c.sorted; ++r)
+        for (int64_t k = indptr[r] + 1; k < indptr[r + 1]; ++k)
+            if (indices[k] < indices[k - 1])
+            {
+                c.sorted = false;
+                break;
+            }
     const size_t vb = dtype == SLQ_F64 ? 8 : 4;
     auto fail = [&](int code) {
         delete op;
@@ -1446,6 +1456,7 @@ int slq_op_create_sparse_gaussian(int64_t nrows, int64_t ncols, int32_t per_row,
     c.ncols = ncols;
     c.nnz = nrows * per_row;
     c.dtype = dtype;
+    c.sorted = true;   // one entry per column stratum, strata in order
     const size_t vb = dtype == SLQ_F64 ? 8 : 4;
     auto fail = [&](int code) {
         delete op;

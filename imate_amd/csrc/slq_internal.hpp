@@ -61,6 +61,7 @@ struct DevCSR {
     int64_t* indptr = nullptr;
     int32_t* indices = nullptr;
     void* data = nullptr;
+    bool sorted = false;   // column indices non-decreasing within every row
 };
 
 // Row-tiled "column slab" form of a CSR matrix for the SpMM. Rows are grouped
@@ -101,9 +102,9 @@ struct MatrixStore {
 
 // Host construction of the tiled form (R in {8, 16}); returns R = 0 in
 // `out` when fewer than `min_density` of the slab entries would be non-zero.
-// With R = 16 every tile's slab count is padded to a multiple of four (zero
-// slabs repeating the tile's last column), the layout of the matrix-core
-// SpMM.
+// With R = 16 every tile's slab count is padded to a multiple of eight (zero
+// slabs repeating the tile's last column) and the values are laid out as the
+// matrix-core SpMM's operands (tile_slot in capi.hip).
 int build_tiles(int64_t nrows, int64_t ncols, const int64_t* indptr,
                 const int32_t* indices, const void* data, int dtype, int R,
                 double min_density, DevTiles& out);

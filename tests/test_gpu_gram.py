@@ -80,3 +80,62 @@ def test_chunked_transpose_spmm(gpu_lib, monkeypatch):
                                    _lib.ptr(Y)), 'slq_op_spmm')
     ref = A.T @ U
     assert numpy.max(numpy.abs(Y - ref)) <= 1e-13 * numpy.abs(ref).max()
+
+
+def test_blocked_spmm_matches_scipy(gpu_lib):
+    """Rows of ~280 non-zeros over 70,000 columns (a 36 MB operand): the
+    column-blocked CSR kernel for A X, the row kernel for A^T U; and the
+    same matrix from host arrays, sorted (blocked) and with rows reversed
+    (unsorted: the row kernel)."""
+    import ctypes
+    from imate_amd import _lib
+    D = sparse_gaussian_device(2048, 70000, 0.004, seed=11)
+    A = D.to_csr()
+    rng = numpy.random.default_rng(1)
+    X = rng.standard_normal((70000, 64))
+    Y = numpy.empty((2048, 64))
+    _lib.check(gpu_lib.slq_op_spmm(D.gpu_Aop.handle, 0.0, 64, 0, _lib.ptr(X),
+                                   _lib.ptr(Y)), 'slq_op_spmm')
+    ref = A @ X
+    assert numpy.max(numpy.abs(Y - ref)) <= 1e-13 * numpy.abs(ref).max()
+    U = rng.standard_normal((2048, 64))
+    Z = numpy.empty((70000, 64))
+    _lib.check(gpu_lib.slq_op_spmm(D.gpu_Aop.handle, 0.0, 64, 1, _lib.ptr(U),
+                                   _lib.ptr(Z)), 'slq_op_spmm')
+    refz = A.T @ U
+    assert numpy.max(numpy.abs(Z - refz)) <= 1e-13 * numpy.abs(refz).max()
+    indptr = A.indptr.astype(numpy.int64)
+    rev_ix = A.indices.astype(numpy.int32).copy()
+    rev_v = A.data.copy()
+    for r in range(A.shape[0]):
+        a, b = indptr[r], indptr[r + 1]
+        rev_ix[a:b] = rev_ix[a:b][::-1].copy()
+        rev_v[a:b] = rev_v[a:b][::-1].copy()
+    for ix, v in ((A.indices.astype(numpy.int32), A.data), (rev_ix, rev_v)):
+        h = ctypes.c_void_p()
+        _lib.check(gpu_lib.slq_op_create_csr(2048, 70000, A.nnz, _lib.ptr(indptr),
+                                             _lib.ptr(ix), _lib.ptr(v),
+                                             _lib.SLQ_F64, ctypes.byref(h)),
+                   'slq_op_create_csr')
+        try:
+            Y2 = numpy.empty((2048, 64))
+            _lib.check(gpu_lib.slq_op_spmm(h, 0.0, 64, 0, _lib.ptr(X),
+                                           _lib.ptr(Y2)), 'slq_op_spmm')
+        finally:
+            gpu_lib.slq_op_destroy(h)
+        assert numpy.max(numpy.abs(Y2 - ref)) <= 1e-13 * numpy.abs(ref).max()
+
+
+def test_gram_blocked_transpose_matches_oracle(gpu_lib):
+    """Tall A (70,000 x 2048, ~8 non-zeros per row): Golub-Kahn's A^T u
+    gathers a 36 MB operand through the column-blocked kernel."""
+    D = sparse_gaussian_device(70000, 2048, 0.004, seed=12)
+    A = D.to_csr()
+    val, info = imate_amd.logdet(D, gram=True, method='slq', gpu=True,
+                                 lanczos_degree=20, min_num_samples=64,
+                                 max_num_samples=64, seed=5, return_info=True)
+    ref = oracle.slq(A, func='log', gram=True, lanczos_degree=20,
+                     min_num_samples=64, max_num_samples=64, seed=5)
+    got = info['convergence']['samples']
+    want = ref['samples'][:, 0]
+    assert numpy.max(numpy.abs(got - want) / numpy.abs(want)) < 1e-10
