@@ -71,6 +71,10 @@ static void free_tiles(DevTiles& t)
 
 MatrixStore::~MatrixStore()
 {
+    if (dn.data) (void) hipFree(dn.data);
+    if (dn.trans) (void) hipFree(dn.trans);
+    if (dn.data_f32) (void) hipFree(dn.data_f32);
+    if (dn.trans_f32) (void) hipFree(dn.trans_f32);
     free_tiles(tiles);
     free_tiles(trans_tiles);
     if (csr.indptr) (void) hipFree(csr.indptr);
@@ -809,6 +813,12 @@ int value_dtype(const slq_op* op, int precision)
 
 int prepare(slq_op* op, int gram, int precision)
 {
+    if (op->A->dense)
+    {
+        if (gram) SLQ_TRY(dense_transpose(op->stream, *op->A));
+        if (precision == SLQ_PREC_MIXED) SLQ_TRY(ensure_f32(op->stream, *op->A, gram != 0));
+        return SLQ_OK;
+    }
     if (gram)
     {
         if (!op->A->has_trans)
@@ -1490,6 +1500,11 @@ int slq_op_get_csr(slq_op* op, int64_t* indptr, int32_t* indices, void* data)
         set_error("slq_op_get_csr: a matrix handle and an indptr buffer are required");
         return SLQ_E_ARG;
     }
+    if (op->A->dense)
+    {
+        set_error("slq_op_get_csr: the handle holds a dense matrix");
+        return SLQ_E_ARG;
+    }
     DeviceGuard guard(op->device);
     const DevCSR& c = op->A->csr;
     const size_t vb = c.dtype == SLQ_F64 ? 8 : 4;
@@ -1531,6 +1546,12 @@ int slq_op_create_affine(slq_op* A, slq_op* B, slq_op** out)
         set_error("slq_op_create_affine: B must have the shape and dtype of A");
         return SLQ_E_ARG;
     }
+    if (B && (A->A->dense || B->A->dense))
+    {
+        set_error("slq_op_create_affine: A + tB with a general B needs CSR handles for A "
+                  "and B");
+        return SLQ_E_ARG;
+    }
     slq_op* op = new (std::nothrow) slq_op();
     if (!op)
     {
@@ -1551,6 +1572,60 @@ int slq_op_create_affine(slq_op* A, slq_op* B, slq_op** out)
     return SLQ_OK;
 }
 
+int slq_op_create_dense(int64_t nrows, int64_t ncols, const void* data, int dtype,
+                        slq_op** out)
+{
+    if (!out || nrows < 1 || ncols < 1 || !data || (dtype != SLQ_F64 && dtype != SLQ_F32))
+    {
+        set_error("slq_op_create_dense: invalid argument");
+        return SLQ_E_ARG;
+    }
+    if (nrows > INT32_MAX || ncols > INT32_MAX)
+    {
+        set_error("slq_op_create_dense: more than 2^31-1 rows or columns");
+        return SLQ_E_ARG;
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+    {
+        set_error("no HIP device found");
+        return SLQ_E_NODEVICE;
+    }
+    slq_op* op = new (std::nothrow) slq_op();
+    if (!op)
+    {
+        set_error("host allocation failed");
+        return SLQ_E_OOM;
+    }
+    (void) hipGetDevice(&op->device);
+    op->A = std::make_shared<MatrixStore>();
+    MatrixStore& M = *op->A;
+    M.dense = true;
+    M.csr.nrows = nrows;
+    M.csr.ncols = ncols;
+    M.csr.nnz = nrows * ncols;
+    M.csr.dtype = dtype;
+    M.dn.ld = (ncols + kDenseLd - 1) / kDenseLd * kDenseLd;
+    const size_t vb = dtype == SLQ_F64 ? 8 : 4;
+    auto fail = [&](int code) {
+        delete op;
+        return code;
+    };
+    hipError_t e;
+    if ((e = hipStreamCreateWithFlags(&op->stream, hipStreamNonBlocking)) != hipSuccess)
+        return fail(hip_status(e, "hipStreamCreate", __FILE__, __LINE__));
+    const size_t bytes = (size_t) nrows * (size_t) M.dn.ld * vb;
+    if ((e = hipMalloc(&M.dn.data, bytes)) != hipSuccess)
+        return fail(hip_status(e, "dense upload", __FILE__, __LINE__));
+    if ((e = hipMemset(M.dn.data, 0, bytes)) != hipSuccess ||
+        (e = hipMemcpy2D(M.dn.data, (size_t) M.dn.ld * vb, data, (size_t) ncols * vb,
+                         (size_t) ncols * vb, (size_t) nrows, hipMemcpyHostToDevice)) !=
+            hipSuccess)
+        return fail(hip_status(e, "dense upload", __FILE__, __LINE__));
+    *out = op;
+    return SLQ_OK;
+}
+
 int slq_op_set_values(slq_op* op, const void* data)
 {
     if (!op || op->kind != OP_MATRIX || (!data && op->A->csr.nnz > 0))
@@ -1560,6 +1635,22 @@ int slq_op_set_values(slq_op* op, const void* data)
     }
     DeviceGuard guard(op->device);
     MatrixStore& M = *op->A;
+    if (M.dense)
+    {
+        // new row-major values: replace the padded copy, drop derived forms
+        const size_t vb = M.csr.dtype == SLQ_F64 ? 8 : 4;
+        SLQ_CHECK(hipStreamSynchronize(op->stream));
+        SLQ_CHECK(hipMemcpy2D(M.dn.data, (size_t) M.dn.ld * vb, data,
+                              (size_t) M.csr.ncols * vb, (size_t) M.csr.ncols * vb,
+                              (size_t) M.csr.nrows, hipMemcpyHostToDevice));
+        for (void** p : {&M.dn.trans, (void**) &M.dn.data_f32, (void**) &M.dn.trans_f32})
+            if (*p)
+            {
+                (void) hipFree(*p);
+                *p = nullptr;
+            }
+        return SLQ_OK;
+    }
     if (M.tiles.R > 0 && !M.tiles.src)
     {
         set_error("slq_op_set_values: the matrix has duplicate entries or too many "

@@ -252,6 +252,8 @@ struct SpmmArgs {
     int64_t ncols;          // columns of A (rows of the gathered operand)
     int sorted;             // A's column indices ascend within every row
     double avg_row_nnz;     // non-zeros of A per row
+    const T* dense;         // dense A (row-major, rows of ld values) or nullptr
+    int64_t ld;
     int64_t shift_len;      // rows that get + t x[i] (identity shift)
     int shift_mode;         // 0 none, 1 + t x (B = I), 2 + t B x
     double t;
@@ -904,6 +906,147 @@ __global__ __launch_bounds__(kThreads) void mfma_spmm_kernel(
     }
 }
 
+// Dense SpMM on the matrix cores: Y = A (s X) for a row-major A padded to
+// rows of ld values (ld a multiple of 64, zero padding). A workgroup of 8
+// wavefronts takes 128 rows (16 per wavefront, one MFMA row block) and
+// sweeps the columns in chunks of 64: the 64 scaled operand rows of a chunk
+// are staged in LDS once for the workgroup (double-buffered, the next chunk
+// loaded while the current one is multiplied), and each wavefront streams
+// its 16 x 64 slice of A from HBM, four consecutive values per lane. MFMA
+// group (c, g) gives k-slot kq to column 16 c + 4 kq + g of the chunk (A
+// and B alike, so the product is unchanged); the four 16-probe blocks hold
+// probes 4 i + q as in mfma_spmm_kernel, whose epilogue it shares. At 64
+// probes the product does 16 flops per FP64 matrix byte: the bound is the
+// matrix cores (78.6 TFLOP/s is 4.9 TB/s of A) as much as HBM.
+constexpr int kDenseKC = 64;
+
+template <typename T>
+__device__ __forceinline__ void load_vals4(const T* __restrict__ p, T (&v)[4])
+{
+    load_row4(p, v);
+}
+
+template <typename T, bool NT>
+__global__ __launch_bounds__(kThreads) void dense_spmm_kernel(SpmmArgs<T> a,
+                                                              const T* __restrict__ X,
+                                                              T* __restrict__ Y,
+                                                              T* __restrict__ Vout,
+                                                              const T* __restrict__ Sub)
+{
+    using V4 = typename Mfma16<T>::V4;
+    __shared__ T xs[2][kDenseKC][kWave];
+    const int lane = threadIdx.x % kWave;
+    const int wave = wave_id();
+    const int i0 = lane % 16;
+    const int kq = lane / 16;
+    // staging: thread -> chunk row kk, probes pp .. pp + 7
+    const int kk = (int) threadIdx.x >> 3;
+    const int pp = ((int) threadIdx.x & 7) * 8;
+    T ssc[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) ssc[u] = a.xscale ? (T) a.xscale[pp + u] : T(1);
+    T sq[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) sq[q] = a.xscale ? (T) a.xscale[4 * i0 + q] : T(1);
+    const T tt = (T) a.t;
+    T sub_c[4], sub_s[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+    {
+        sub_c[q] = Sub ? (T) a.sub_coef[4 * i0 + q] : T(0);
+        sub_s[q] = Sub ? (T) a.sub_scale[4 * i0 + q] : T(0);
+    }
+    const int64_t nchunk = (a.ncols + kDenseKC - 1) / kDenseKC;
+    auto stage = [&](int64_t ci, int buf) {
+        const int64_t k = ci * kDenseKC + kk;
+        T v[8];
+        if (k < a.ncols)
+        {
+            T w[4];
+            load_row4(X + k * kBlock + pp, w);
+#pragma unroll
+            for (int u = 0; u < 4; ++u) v[u] = ssc[u] * w[u];
+            load_row4(X + k * kBlock + pp + 4, w);
+#pragma unroll
+            for (int u = 0; u < 4; ++u) v[4 + u] = ssc[4 + u] * w[u];
+        }
+        else
+        {
+#pragma unroll
+            for (int u = 0; u < 8; ++u) v[u] = T(0);
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) xs[buf][kk][pp + u] = v[u];
+    };
+    double red[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int64_t rb = blockIdx.x; rb * 128 < a.nrows; rb += gridDim.x)
+    {
+        const int64_t r0 = rb * 128 + wave * 16;
+        const int64_t myrow = r0 + i0 < a.nrows ? r0 + i0 : a.nrows - 1;
+        const T* __restrict__ arow = a.dense + myrow * a.ld + 4 * kq;
+        V4 acc[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) acc[q] = V4{T(0), T(0), T(0), T(0)};
+        stage(0, 0);
+        __syncthreads();
+        for (int64_t ci = 0; ci < nchunk; ++ci)
+        {
+            const int buf = (int) (ci & 1);
+            T av[4][4];
+#pragma unroll
+            for (int c = 0; c < 4; ++c) load_vals4(arow + ci * kDenseKC + 16 * c, av[c]);
+            if (ci + 1 < nchunk) stage(ci + 1, buf ^ 1);
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+            {
+#pragma unroll
+                for (int g = 0; g < 4; ++g)
+                {
+                    T bv[4];
+                    load_row4(&xs[buf][16 * c + 4 * kq + g][4 * i0], bv);
+#pragma unroll
+                    for (int q = 0; q < 4; ++q)
+                        acc[q] = Mfma16<T>::op(av[c][g], bv[q], acc[q]);
+                }
+            }
+            __syncthreads();
+        }
+#pragma unroll
+        for (int m = 0; m < 4; ++m)
+        {
+            const int64_t row = r0 + (Mfma16<T>::kRowStride == 4 ? kq + 4 * m : 4 * kq + m);
+            const T y[4] = {acc[0][m], acc[1][m], acc[2][m], acc[3][m]};
+            if (row < a.nrows)
+                spmm_epilogue4<T, NT>(a, row, i0, y, sq, tt, sub_c, sub_s, X, Y, Vout, Sub,
+                                      red);
+        }
+    }
+    if (a.red_mode)
+    {
+        // per-probe totals of this lane's rows, over the four row groups
+        double mine = 0.0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+        {
+            const double r1 = red[q] + __shfl_xor(red[q], 16);
+            const double r2 = r1 + __shfl_xor(r1, 32);
+            if (q == kq) mine = r2;
+        }
+        // lane (kq, i0) now holds probe 4 i0 + kq: move it to lane = probe
+        // through LDS and add the wavefronts in order
+        __shared__ double pr[kWavesPerWG][kWave];
+        pr[wave][4 * i0 + kq] = mine;
+        __syncthreads();
+        if (wave == 0)
+        {
+            double t = 0.0;
+#pragma unroll
+            for (int w = 0; w < kWavesPerWG; ++w) t += pr[w][lane];
+            a.rb.partial[(int64_t) blockIdx.x * kWave + lane] = t;
+        }
+    }
+}
+
 // Sum of the per-tile partial rows in tile order, in two launches (the
 // kernel boundary orders them; no fences): tile_sum_kernel gives workgroup g
 // a contiguous range of tiles, its wavefronts interleaved with four
@@ -1406,6 +1549,17 @@ template <typename T>
 void fill_matrix(SpmmArgs<T>& s, MatrixStore& A, MatrixStore* B, bool trans,
                  bool f32)
 {
+    if (A.dense)
+    {
+        const bool cvt = f32 && A.csr.dtype == SLQ_F64;
+        const void* d = trans ? (cvt ? (const void*) A.dn.trans_f32 : A.dn.trans)
+                              : (cvt ? (const void*) A.dn.data_f32 : A.dn.data);
+        s.dense = (const T*) d;
+        s.ld = trans ? A.dn.trans_ld : A.dn.ld;
+        s.nrows = trans ? A.csr.ncols : A.csr.nrows;
+        s.ncols = trans ? A.csr.nrows : A.csr.ncols;
+        return;
+    }
     const DevCSR& c = trans ? A.trans : A.csr;
     s.indptr = c.indptr;
     s.indices = c.indices;
@@ -1496,6 +1650,17 @@ struct SpmmIO {
 template <typename T, bool NT>
 void launch_spmm_nt(slq_op* op, const SpmmArgs<T>& s, const SpmmIO<T>& io, const DevTiles* tl)
 {
+    if (s.dense)
+    {
+        const int64_t rblocks = (s.nrows + 127) / 128;
+        const int grid = (int) (rblocks < kMaxGrid ? rblocks : kMaxGrid);
+        hipLaunchKernelGGL((dense_spmm_kernel<T, NT>), dim3(grid), dim3(kThreads), 0,
+                           op->stream, s, io.X, io.Y, io.Vout, io.Sub);
+        if (s.red_mode)
+            hipLaunchKernelGGL((total_kernel<T>), dim3(1), dim3(kTotalWaves * kWave), 0,
+                               op->stream, s, grid);
+        return;
+    }
     if (tl)
     {
         const T* tv = (sizeof(T) == 4 && op->A->csr.dtype == SLQ_F64)
@@ -2012,7 +2177,7 @@ int run_recurrence(const RecurrenceArgs& ra, void* scratch, size_t scratch_bytes
 {
     Bump bump{(char*) scratch, scratch_bytes};
     const bool f32 = ra.precision == SLQ_PREC_MIXED || ra.op->A->csr.dtype == SLQ_F32;
-    if (ra.gram && !ra.op->A->has_trans)
+    if (ra.gram && !ra.op->A->has_trans && !ra.op->A->dn.trans)
     {
         set_error("internal: transpose not built");
         return SLQ_E_ARG;
@@ -2120,9 +2285,65 @@ int tiles_f32(hipStream_t s, DevTiles& t, int dtype)
     return SLQ_OK;
 }
 
+// out[c][r] = in[r][c] for r < nrows, c < ncols; 32 x 32 tiles via LDS.
+template <typename T>
+__global__ void dense_transpose_kernel(const T* __restrict__ in, int64_t nrows, int64_t ncols,
+                                       int64_t ld_in, T* __restrict__ out, int64_t ld_out)
+{
+    __shared__ T tile[32][33];
+    const int64_t c0 = (int64_t) blockIdx.x * 32, r0 = (int64_t) blockIdx.y * 32;
+    for (int j = threadIdx.y; j < 32; j += blockDim.y)
+    {
+        const int64_t r = r0 + j, c = c0 + threadIdx.x;
+        tile[j][threadIdx.x] = (r < nrows && c < ncols) ? in[r * ld_in + c] : T(0);
+    }
+    __syncthreads();
+    for (int j = threadIdx.y; j < 32; j += blockDim.y)
+    {
+        const int64_t c = c0 + j, r = r0 + threadIdx.x;
+        if (c < ncols && r < nrows) out[c * ld_out + r] = tile[threadIdx.x][j];
+    }
+}
+
+int dense_transpose(hipStream_t s, MatrixStore& M)
+{
+    if (M.dn.trans) return SLQ_OK;
+    const int64_t nr = M.csr.nrows, nc = M.csr.ncols;
+    const size_t vb = M.csr.dtype == SLQ_F64 ? 8 : 4;
+    M.dn.trans_ld = (nr + kDenseLd - 1) / kDenseLd * kDenseLd;
+    const size_t bytes = (size_t) nc * (size_t) M.dn.trans_ld * vb;
+    SLQ_CHECK(hipMalloc(&M.dn.trans, bytes));
+    SLQ_CHECK(hipMemsetAsync(M.dn.trans, 0, bytes, s));
+    const dim3 grid((unsigned) ((nc + 31) / 32), (unsigned) ((nr + 31) / 32));
+    if (M.csr.dtype == SLQ_F64)
+        hipLaunchKernelGGL(dense_transpose_kernel<double>, grid, dim3(32, 8), 0, s,
+                           (const double*) M.dn.data, nr, nc, M.dn.ld, (double*) M.dn.trans,
+                           M.dn.trans_ld);
+    else
+        hipLaunchKernelGGL(dense_transpose_kernel<float>, grid, dim3(32, 8), 0, s,
+                           (const float*) M.dn.data, nr, nc, M.dn.ld, (float*) M.dn.trans,
+                           M.dn.trans_ld);
+    SLQ_CHECK(hipGetLastError());
+    return SLQ_OK;
+}
+
 int ensure_f32(hipStream_t s, MatrixStore& M, bool need_trans)
 {
     if (M.csr.dtype != SLQ_F64) return SLQ_OK;
+    if (M.dense)
+    {
+        auto convert = [&](const void* src, float** dst, int64_t n) -> int {
+            if (*dst || !src) return SLQ_OK;
+            SLQ_CHECK(hipMalloc(dst, sizeof(float) * n));
+            hipLaunchKernelGGL((convert_kernel<double, float>), dim3((unsigned) ((n + 255) / 256)),
+                               dim3(256), 0, s, (const double*) src, *dst, n);
+            SLQ_CHECK(hipGetLastError());
+            return SLQ_OK;
+        };
+        SLQ_TRY(convert(M.dn.data, &M.dn.data_f32, M.csr.nrows * M.dn.ld));
+        if (need_trans) SLQ_TRY(convert(M.dn.trans, &M.dn.trans_f32, M.csr.ncols * M.dn.trans_ld));
+        return SLQ_OK;
+    }
     SLQ_TRY(tiles_f32(s, M.tiles, M.csr.dtype));
     if (need_trans) SLQ_TRY(tiles_f32(s, M.trans_tiles, M.csr.dtype));
     if (!M.data_f32)

@@ -88,7 +88,22 @@ struct DevTiles {
 // Device copy of a matrix plus derived forms built on demand: the transpose
 // (CSR of A^T, used as a gather for A^T u), an FP32 copy of the values of an
 // FP64 matrix (for the mixed-precision form) and the row-tiled form.
+// Dense row-major form (a dense input matrix): rows padded to `ld`, a
+// multiple of 64, with zeros; the transpose and FP32 copies are made on
+// first use (Golub-Kahn, mixed precision).
+struct DevDense {
+    int64_t ld = 0;
+    void* data = nullptr;          // [nrows][ld], dtype of the matrix
+    int64_t trans_ld = 0;
+    void* trans = nullptr;         // [ncols][trans_ld]
+    float* data_f32 = nullptr;
+    float* trans_f32 = nullptr;
+};
+constexpr int64_t kDenseLd = 64;
+
 struct MatrixStore {
+    bool dense = false;    // csr holds only nrows, ncols, nnz and dtype
+    DevDense dn;
     DevCSR csr;
     DevCSR trans;          // valid when has_trans
     bool has_trans = false;
@@ -194,6 +209,8 @@ int run_spmm(slq_op* op, double t, int transpose, const void* X, void* Y,
 // Transpose of a device CSR (stable: rows ascending within a column).
 int build_transpose(hipStream_t s, const DevCSR& A, DevCSR& T);
 int ensure_f32(hipStream_t s, MatrixStore& M, bool need_trans);
+// Transposed copy of a dense matrix (for Golub-Kahn).
+int dense_transpose(hipStream_t s, MatrixStore& M);
 
 // Profiling hooks around kernel launches.
 void prof_begin(slq_op* op, hipEvent_t* ev);

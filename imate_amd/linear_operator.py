@@ -29,11 +29,21 @@ class DeviceOperator(object):
         self._parents = parents  # keep referenced handles alive
 
     @staticmethod
-    def from_matrix(A):
+    def from_matrix(A, dense_ok=True):
+        """Device copy of A: a dense handle for a numpy array (the matrix
+        cores multiply it as is, imate's cuDenseMatrix), a CSR handle for a
+        scipy sparse matrix or when ``dense_ok`` is False."""
         lib = _lib.load()
         if A.ndim != 2:
             raise ValueError('Input matrix should be a 2-dimensional array.')
         code = _lib.dtype_code(A.dtype)
+        if dense_ok and not scipy.sparse.issparse(A):
+            D = numpy.ascontiguousarray(A)
+            h = ctypes.c_void_p()
+            _lib.check(lib.slq_op_create_dense(
+                D.shape[0], D.shape[1], _lib.ptr(D), code, ctypes.byref(h)),
+                'slq_op_create_dense')
+            return DeviceOperator(h)
         if scipy.sparse.issparse(A):
             M = scipy.sparse.csr_matrix(A)
         else:
@@ -234,12 +244,14 @@ class AffineMatrixFunction(LinearOperator):
     def initialize(self, gpu, num_gpu_devices):
         if gpu:
             if not self.initialized_on_gpu:
-                self._A_dev = DeviceOperator.from_matrix(self.A)
+                # A + tB with a general B runs the fused CSR SpMM of A and B
+                dense_ok = self.B is None
+                self._A_dev = DeviceOperator.from_matrix(self.A, dense_ok)
                 if self.B is not None:
                     B = self.B
                     if B.dtype != self.A.dtype:
                         B = B.astype(self.A.dtype)
-                    self._B_dev = DeviceOperator.from_matrix(B)
+                    self._B_dev = DeviceOperator.from_matrix(B, False)
                 self.gpu_Aop = DeviceOperator.affine(self._A_dev, self._B_dev)
                 self.initialized_on_gpu = True
         else:

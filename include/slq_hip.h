@@ -79,6 +79,16 @@ int slq_op_create_csr(int64_t nrows, int64_t ncols, int64_t nnz,
                       const int64_t* indptr, const int32_t* indices,
                       const void* data, int dtype, slq_op** out);
 
+/* Dense matrix handle from host row-major data[nrows][ncols] (dtype
+ * SLQ_F64 or SLQ_F32). The SpMM runs on the matrix cores with operand rows
+ * staged through LDS; A + tI over it is allowed (slq_op_create_affine with
+ * B == NULL), a general B is not (use CSR handles). Replaces the dense
+ * branch of pycuMatrix and cuDenseMatrix (imate/_cu_linear_operator/
+ * py_cu_matrix.pyx:249,306; cu_dense_matrix.cu:38-218, whose dot and
+ * transpose_dot call cuBLAS gemv per vector). */
+int slq_op_create_dense(int64_t nrows, int64_t ncols, const void* data, int dtype,
+                        slq_op** out);
+
 /* Affine operator A + tB over existing handles (B == NULL means B = I, for
  * which the Ritz values of A + tI follow from one Lanczos run by a shift).
  * The new handle references A's (and B's) device arrays; destroy it before
