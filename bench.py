@@ -23,6 +23,7 @@ import sys
 import time
 
 import numpy
+import scipy.sparse
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
@@ -55,7 +56,7 @@ def parse():
                         'points to this .npz file')
     p.add_argument('--form', default='plain',
                    choices=['plain', 'precision', 'batched', 'surrogate',
-                            'gram'],
+                            'gram', 'dense'],
                    help='plain: the headline workload (default); the others '
                         'measure the north-star forms and print their own '
                         'JSON lines')
@@ -489,6 +490,79 @@ def gram_form(args):
     print(json.dumps(out), flush=True)
 
 
+def dense_form(args):
+    """Dense operator (imate's cuDenseMatrix): logdet of an SPD dense matrix
+    A = (R + R^T)/2 + 1.5 sqrt(n) I, R uniform in [-1, 1], FP64, n = 32768
+    (8.6 GB), lanczos_degree 30, 64 probes. The SpMM is a (n x n)(n x 64)
+    product on the matrix cores: 2 n^2 64 flops and n^2 8 bytes of matrix per
+    launch, timed with HIP events; against the same matrix through the CSR
+    path (dense converted to CSR and 16-row tiles) at n = 8192."""
+    import imate_amd
+    from imate_amd import _lib
+    lib = _lib.load()
+    out = {'form': 'dense', 'lanczos_degree': args.lanczos_degree,
+           'samples': 64}
+
+    def spd(n, seed):
+        rng = numpy.random.default_rng(seed)
+        A = rng.uniform(-1.0, 1.0, (n, n))
+        A += A.T
+        A *= 0.5
+        A[numpy.diag_indices(n)] += 1.5 * numpy.sqrt(n)
+        return A
+
+    def run(M, n, label):
+        kw = dict(method='slq', gpu=True, lanczos_degree=args.lanczos_degree,
+                  min_num_samples=64, max_num_samples=64, seed=args.seed,
+                  return_info=True)
+        imate_amd.logdet(M, **kw)                        # warm-up
+        h = M.gpu_Aop.handle
+        _lib.check(lib.slq_profile_enable(h, 1), 'slq_profile_enable')
+        t0 = time.perf_counter()
+        val, info = imate_amd.logdet(M, **kw)
+        el = time.perf_counter() - t0
+        ms = ctypes.c_double(0)
+        cnt = ctypes.c_int64(0)
+        ums = ctypes.c_double(0)
+        ucnt = ctypes.c_int64(0)
+        _lib.check(lib.slq_profile_read(h, ctypes.byref(ms), ctypes.byref(cnt),
+                                        ctypes.byref(ums), ctypes.byref(ucnt)),
+                   'slq_profile_read')
+        lib.slq_profile_enable(h, 0)
+        spmm = ms.value / max(cnt.value, 1)
+        flops = 2.0 * n * n * 64
+        res = {'path': label, 'n': n, 'seconds': el,
+               'matvec_steps_per_s': 64 * args.lanczos_degree / el,
+               'spmm_ms': spmm, 'spmm_tflops': flops / spmm / 1e9,
+               'spmm_frac_fp64_mfma_peak': flops / spmm / 1e9 / 78.6,
+               'spmm_matrix_GBps': 8.0 * n * n / spmm / 1e6,
+               'logdet': float(val),
+               'mc_error_95': float(info['error']['absolute_error'])}
+        log('dense: %s' % json.dumps(res))
+        return res
+
+    n = 32768
+    t0 = time.perf_counter()
+    A = spd(n, args.seed)
+    log('dense n=%d built in %.1fs' % (n, time.perf_counter() - t0))
+    M = imate_amd.Matrix(A)
+    M.get_linear_operator(gpu=True)
+    out['dense_32768'] = run(M, n, 'dense')
+    del M, A
+    n = 8192
+    A = spd(n, args.seed)
+    sign, ld_exact = numpy.linalg.slogdet(A)
+    out['dense_8192'] = run(imate_amd.Matrix(A), n, 'dense')
+    t0 = time.perf_counter()
+    Mc = imate_amd.Matrix(scipy.sparse.csr_matrix(A))
+    Mc.get_linear_operator(gpu=True)
+    out['csr_tiles_8192_upload_seconds'] = time.perf_counter() - t0
+    out['csr_tiles_8192'] = run(Mc, n, 'csr (16-row tiles)')
+    out['exact_logdet_8192'] = float(ld_exact)
+    out['rel_err_8192'] = abs(out['dense_8192']['logdet'] - ld_exact) / abs(ld_exact)
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -518,6 +592,8 @@ def main():
         return surrogate_form(args)
     if args.form == 'gram':
         return gram_form(args)
+    if args.form == 'dense':
+        return dense_form(args)
 
     from imate_amd.sample_matrices import sparse_correlation_matrix
     n, k, m, s = args.n, args.nnz_per_row, args.lanczos_degree, args.samples

@@ -926,12 +926,45 @@ __device__ __forceinline__ void load_vals4(const T* __restrict__ p, T (&v)[4])
     load_row4(p, v);
 }
 
+// Per-probe reduction terms of a workgroup's rows (red[q] of lane (kq, i0)
+// holds probe 4 i0 + q over that lane's rows) into its partial row.
+template <typename T>
+__device__ __forceinline__ void dense_red_partial(const SpmmArgs<T>& a, const double (&red)[4],
+                                                  int wave, int kq, int i0, int lane)
+{
+    double mine = 0.0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+    {
+        const double r1 = red[q] + __shfl_xor(red[q], 16);
+        const double r2 = r1 + __shfl_xor(r1, 32);
+        if (q == kq) mine = r2;
+    }
+    // lane (kq, i0) holds probe 4 i0 + kq: to lane = probe through LDS, then
+    // the wavefronts in order
+    __shared__ double pr[kWavesPerWG][kWave];
+    pr[wave][4 * i0 + kq] = mine;
+    __syncthreads();
+    if (wave == 0)
+    {
+        double t = 0.0;
+#pragma unroll
+        for (int w = 0; w < kWavesPerWG; ++w) t += pr[w][lane];
+        a.rb.partial[(int64_t) blockIdx.x * kWave + lane] = t;
+    }
+}
+
+// Work item it of the dense SpMM: row block it / ksplit, column chunks of
+// split it % ksplit. With ksplit > 1 (too few row blocks to fill the chip)
+// each item stores its partial product in scratch[split][row][64] and
+// dense_reduce_kernel adds the splits in order and finishes the rows.
 template <typename T, bool NT>
 __global__ __launch_bounds__(kThreads) void dense_spmm_kernel(SpmmArgs<T> a,
                                                               const T* __restrict__ X,
                                                               T* __restrict__ Y,
                                                               T* __restrict__ Vout,
-                                                              const T* __restrict__ Sub)
+                                                              const T* __restrict__ Sub,
+                                                              int ksplit, T* __restrict__ scratch)
 {
     using V4 = typename Mfma16<T>::V4;
     __shared__ T xs[2][kDenseKC][kWave];
@@ -957,6 +990,8 @@ __global__ __launch_bounds__(kThreads) void dense_spmm_kernel(SpmmArgs<T> a,
         sub_s[q] = Sub ? (T) a.sub_scale[4 * i0 + q] : T(0);
     }
     const int64_t nchunk = (a.ncols + kDenseKC - 1) / kDenseKC;
+    const int64_t cps = (nchunk + ksplit - 1) / ksplit;
+    const int64_t rblocks = (a.nrows + 127) / 128;
     auto stage = [&](int64_t ci, int buf) {
         const int64_t k = ci * kDenseKC + kk;
         T v[8];
@@ -979,23 +1014,27 @@ __global__ __launch_bounds__(kThreads) void dense_spmm_kernel(SpmmArgs<T> a,
         for (int u = 0; u < 8; ++u) xs[buf][kk][pp + u] = v[u];
     };
     double red[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int64_t rb = blockIdx.x; rb * 128 < a.nrows; rb += gridDim.x)
+    for (int64_t it = blockIdx.x; it < rblocks * ksplit; it += gridDim.x)
     {
+        const int64_t rb = it / ksplit;
+        const int sp = (int) (it - rb * ksplit);
+        const int64_t c0 = sp * cps;
+        const int64_t c1 = c0 + cps < nchunk ? c0 + cps : nchunk;
         const int64_t r0 = rb * 128 + wave * 16;
         const int64_t myrow = r0 + i0 < a.nrows ? r0 + i0 : a.nrows - 1;
         const T* __restrict__ arow = a.dense + myrow * a.ld + 4 * kq;
         V4 acc[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q) acc[q] = V4{T(0), T(0), T(0), T(0)};
-        stage(0, 0);
+        if (c0 < c1) stage(c0, 0);
         __syncthreads();
-        for (int64_t ci = 0; ci < nchunk; ++ci)
+        for (int64_t ci = c0; ci < c1; ++ci)
         {
-            const int buf = (int) (ci & 1);
+            const int buf = (int) ((ci - c0) & 1);
             T av[4][4];
 #pragma unroll
-            for (int c = 0; c < 4; ++c) load_vals4(arow + ci * kDenseKC + 16 * c, av[c]);
-            if (ci + 1 < nchunk) stage(ci + 1, buf ^ 1);
+            for (int c = 0; c < 4; ++c) load_row4(arow + ci * kDenseKC + 16 * c, av[c]);
+            if (ci + 1 < c1) stage(ci + 1, buf ^ 1);
 #pragma unroll
             for (int c = 0; c < 4; ++c)
             {
@@ -1016,35 +1055,75 @@ __global__ __launch_bounds__(kThreads) void dense_spmm_kernel(SpmmArgs<T> a,
         {
             const int64_t row = r0 + (Mfma16<T>::kRowStride == 4 ? kq + 4 * m : 4 * kq + m);
             const T y[4] = {acc[0][m], acc[1][m], acc[2][m], acc[3][m]};
-            if (row < a.nrows)
+            if (row >= a.nrows) continue;
+            if (ksplit > 1)
+                store_row4<false>(scratch + ((int64_t) sp * a.nrows + row) * kBlock + 4 * i0, y);
+            else
                 spmm_epilogue4<T, NT>(a, row, i0, y, sq, tt, sub_c, sub_s, X, Y, Vout, Sub,
                                       red);
         }
     }
-    if (a.red_mode)
+    if (ksplit == 1 && a.red_mode) dense_red_partial<T>(a, red, wave, kq, i0, lane);
+}
+
+// Adds the ksplit partial products of each row in split order and finishes
+// the rows as dense_spmm_kernel does without a split.
+template <typename T, bool NT>
+__global__ __launch_bounds__(kThreads) void dense_reduce_kernel(SpmmArgs<T> a,
+                                                                const T* __restrict__ X,
+                                                                T* __restrict__ Y,
+                                                                T* __restrict__ Vout,
+                                                                const T* __restrict__ Sub,
+                                                                int ksplit,
+                                                                const T* __restrict__ scratch)
+{
+    const int lane = threadIdx.x % kWave;
+    const int wave = wave_id();
+    const int i0 = lane % 16;
+    const int kq = lane / 16;
+    T sq[4], sub_c[4], sub_s[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
     {
-        // per-probe totals of this lane's rows, over the four row groups
-        double mine = 0.0;
+        sq[q] = a.xscale ? (T) a.xscale[4 * i0 + q] : T(1);
+        sub_c[q] = Sub ? (T) a.sub_coef[4 * i0 + q] : T(0);
+        sub_s[q] = Sub ? (T) a.sub_scale[4 * i0 + q] : T(0);
+    }
+    const T tt = (T) a.t;
+    double red[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int64_t rb = blockIdx.x; rb * 128 < a.nrows; rb += gridDim.x)
+    {
+        const int64_t r0 = rb * 128 + wave * 16;
 #pragma unroll
-        for (int q = 0; q < 4; ++q)
+        for (int m = 0; m < 4; ++m)
         {
-            const double r1 = red[q] + __shfl_xor(red[q], 16);
-            const double r2 = r1 + __shfl_xor(r1, 32);
-            if (q == kq) mine = r2;
-        }
-        // lane (kq, i0) now holds probe 4 i0 + kq: move it to lane = probe
-        // through LDS and add the wavefronts in order
-        __shared__ double pr[kWavesPerWG][kWave];
-        pr[wave][4 * i0 + kq] = mine;
-        __syncthreads();
-        if (wave == 0)
-        {
-            double t = 0.0;
+            const int64_t row = r0 + (Mfma16<T>::kRowStride == 4 ? kq + 4 * m : 4 * kq + m);
+            if (row >= a.nrows) continue;
+            T y[4] = {T(0), T(0), T(0), T(0)};
+            for (int sp = 0; sp < ksplit; ++sp)
+            {
+                T v[4];
+                load_row4(scratch + ((int64_t) sp * a.nrows + row) * kBlock + 4 * i0, v);
 #pragma unroll
-            for (int w = 0; w < kWavesPerWG; ++w) t += pr[w][lane];
-            a.rb.partial[(int64_t) blockIdx.x * kWave + lane] = t;
+                for (int q = 0; q < 4; ++q) y[q] += v[q];
+            }
+            spmm_epilogue4<T, NT>(a, row, i0, y, sq, tt, sub_c, sub_s, X, Y, Vout, Sub, red);
         }
     }
+    if (a.red_mode) dense_red_partial<T>(a, red, wave, kq, i0, lane);
+}
+
+// Column splits of the dense SpMM for an nrows x ncols operand: enough work
+// items (row blocks x splits) for ~512 workgroups, at most one split per
+// 64-column chunk and 64 in all.
+int dense_ksplit(int64_t nrows, int64_t ncols)
+{
+    const int64_t rblocks = (nrows + 127) / 128;
+    const int64_t nchunk = (ncols + kDenseKC - 1) / kDenseKC;
+    int64_t k = (512 + rblocks - 1) / rblocks;
+    if (k > nchunk) k = nchunk;
+    if (k > 64) k = 64;
+    return k < 1 ? 1 : (int) k;
 }
 
 // Sum of the per-tile partial rows in tile order, in two launches (the
@@ -1503,6 +1582,14 @@ struct Bump {
 int64_t sched_tiles(const slq_op* op)
 {
     const MatrixStore& A = *op->A;
+    if (A.dense)
+    {
+        // rows of 64 values of the dense SpMM's split scratch, either way
+        const int64_t nr = A.csr.nrows, nc = A.csr.ncols;
+        const int64_t a = dense_ksplit(nr, nc) > 1 ? dense_ksplit(nr, nc) * nr : 0;
+        const int64_t b = dense_ksplit(nc, nr) > 1 ? dense_ksplit(nc, nr) * nc : 0;
+        return a > b ? a : b;
+    }
     if (A.tiles.R == 0) return 0;
     const int64_t R = A.tiles.R;
     const int64_t a = (A.csr.nrows + R - 1) / R, b = (A.csr.ncols + R - 1) / R;
@@ -1653,12 +1740,25 @@ void launch_spmm_nt(slq_op* op, const SpmmArgs<T>& s, const SpmmIO<T>& io, const
     if (s.dense)
     {
         const int64_t rblocks = (s.nrows + 127) / 128;
-        const int grid = (int) (rblocks < kMaxGrid ? rblocks : kMaxGrid);
+        const int ks = dense_ksplit(s.nrows, s.ncols);
+        T* scratch = (T*) s.rb.tile_partial;
+        // without a split each workgroup writes a partial row (kMaxGrid rows)
+        const int64_t items = rblocks * ks;
+        const int64_t cap = ks > 1 ? 2048 : kMaxGrid;
+        const int grid = (int) (items < cap ? items : cap);
         hipLaunchKernelGGL((dense_spmm_kernel<T, NT>), dim3(grid), dim3(kThreads), 0,
-                           op->stream, s, io.X, io.Y, io.Vout, io.Sub);
+                           op->stream, s, io.X, io.Y, io.Vout, io.Sub, ks, scratch);
+        int rg = grid;
+        if (ks > 1)
+        {
+            rg = (int) (rblocks < kMaxGrid ? rblocks : kMaxGrid);
+            hipLaunchKernelGGL((dense_reduce_kernel<T, NT>), dim3(rg), dim3(kThreads), 0,
+                               op->stream, s, io.X, io.Y, io.Vout, io.Sub, ks,
+                               (const T*) scratch);
+        }
         if (s.red_mode)
             hipLaunchKernelGGL((total_kernel<T>), dim3(1), dim3(kTotalWaves * kWave), 0,
-                               op->stream, s, grid);
+                               op->stream, s, rg);
         return;
     }
     if (tl)
@@ -1726,6 +1826,11 @@ template <typename T>
 int launch_spmm(slq_op* op, const SpmmArgs<T>& s, const SpmmIO<T>& io,
                 const DevTiles* tl = nullptr)
 {
+    if (s.dense && dense_ksplit(s.nrows, s.ncols) > 1 && !s.rb.tile_partial)
+    {
+        set_error("internal: the dense SpMM needs its split scratch");
+        return SLQ_E_ARG;
+    }
     if (tl && tl->R == 16 && (!s.rb.tickets || (s.red_mode && !s.rb.tile_partial)))
     {
         set_error("internal: the matrix-core SpMM needs ticket counters and tile partials");
@@ -2199,6 +2304,15 @@ int run_spmm(slq_op* op, double t, int transpose, const void* X, void* Y,
     SLQ_CHECK(hipMalloc(&tickets, sizeof(unsigned int) * kTicketWords));
     hipLaunchKernelGGL(zero_u32_kernel, dim3(kTicketWords / 64), dim3(64), 0, op->stream,
                        tickets, kTicketWords);
+    // split scratch of the dense SpMM (no reduction here, so no tile rows)
+    double* scratch = nullptr;
+    const int64_t srows = A.dense ? sched_tiles(op) : 0;
+    if (srows > 0 && hipMalloc(&scratch, sizeof(double) * (size_t) srows * kBlock) != hipSuccess)
+    {
+        (void) hipFree(tickets);
+        set_error("slq_op_spmm: out of device memory for the dense split scratch");
+        return SLQ_E_OOM;
+    }
     int r;
     if (value_dtype == SLQ_F32)
     {
@@ -2208,6 +2322,7 @@ int run_spmm(slq_op* op, double t, int transpose, const void* X, void* Y,
         s.shift_len = shift_len;
         s.t = t;
         s.rb.tickets = tickets;
+        s.rb.tile_partial = scratch;
         SpmmIO<float> io;
         io.X = (const float*) X;
         io.Y = (float*) Y;
@@ -2221,6 +2336,7 @@ int run_spmm(slq_op* op, double t, int transpose, const void* X, void* Y,
         s.shift_len = shift_len;
         s.t = t;
         s.rb.tickets = tickets;
+        s.rb.tile_partial = scratch;
         SpmmIO<double> io;
         io.X = (const double*) X;
         io.Y = (double*) Y;
@@ -2228,6 +2344,7 @@ int run_spmm(slq_op* op, double t, int transpose, const void* X, void* Y,
     }
     const hipError_t e = hipStreamSynchronize(op->stream);
     (void) hipFree(tickets);
+    if (scratch) (void) hipFree(scratch);
     if (r != SLQ_OK) return r;
     SLQ_CHECK(e);
     return SLQ_OK;
