@@ -123,7 +123,7 @@ size_t tile_slot(int R, int64_t slab, int r)
 
 int build_tiles(int64_t nrows, int64_t ncols, const int64_t* indptr,
                 const int32_t* indices, const void* data, int dtype, int R,
-                double min_density, DevTiles& out)
+                double min_density, DevTiles& out, std::vector<int32_t>* src_host)
 {
     (void) ncols;
     out = DevTiles();
@@ -171,7 +171,9 @@ int build_tiles(int64_t nrows, int64_t ncols, const int64_t* indptr,
             {
                 p = std::lower_bound(p, ce, indices[k]);
                 const size_t slot = tile_slot(R, p - cols.data(), r);
-                if (dtype == SLQ_F64)
+                if (!data)
+                    ;
+                else if (dtype == SLQ_F64)
                     ((double*) vals.data())[slot] += ((const double*) data)[k];
                 else
                     ((float*) vals.data())[slot] += ((const float*) data)[k];
@@ -204,6 +206,143 @@ int build_tiles(int64_t nrows, int64_t ncols, const int64_t* indptr,
         return hip_status(e, "tile upload", __FILE__, __LINE__);
     }
     out = d;
+    if (src_host)
+    {
+        if (keep_src) src_host->swap(src);
+        else src_host->clear();
+    }
+    return SLQ_OK;
+}
+
+PairTiles::~PairTiles()
+{
+    free_tiles(tiles);
+    if (src_a) (void) hipFree(src_a);
+    if (src_b) (void) hipFree(src_b);
+}
+
+namespace {
+
+// Union of the patterns of A and B (host CSR arrays, square, same shape):
+// per row the distinct columns in ascending order, and for each entry its
+// position in A's and in B's value arrays (-1: absent). False when a matrix
+// repeats a column within a row (its values would need summing) or when the
+// union holds more than 2^31 - 1 entries.
+bool union_pattern(int64_t nrows, const std::vector<int64_t>& ipa,
+                   const std::vector<int32_t>& ixa, const std::vector<int64_t>& ipb,
+                   const std::vector<int32_t>& ixb, std::vector<int64_t>& ipu,
+                   std::vector<int32_t>& ixu, std::vector<int32_t>& ka,
+                   std::vector<int32_t>& kb)
+{
+    if (ipa[nrows] + ipb[nrows] > (int64_t) INT32_MAX) return false;
+    ipu.assign(nrows + 1, 0);
+    ixu.clear();
+    ka.clear();
+    kb.clear();
+    ixu.reserve(ipa[nrows] + ipb[nrows]);
+    struct Entry { int32_t col; int32_t pos; bool in_b; };
+    std::vector<Entry> row;
+    for (int64_t r = 0; r < nrows; ++r)
+    {
+        row.clear();
+        for (int64_t k = ipa[r]; k < ipa[r + 1]; ++k)
+            row.push_back({ixa[k], (int32_t) k, false});
+        for (int64_t k = ipb[r]; k < ipb[r + 1]; ++k)
+            row.push_back({ixb[k], (int32_t) k, true});
+        std::sort(row.begin(), row.end(), [](const Entry& x, const Entry& y) {
+            return x.col != y.col ? x.col < y.col : (x.in_b < y.in_b);
+        });
+        for (size_t i = 0; i < row.size();)
+        {
+            const int32_t c = row[i].col;
+            int32_t pa = -1, pb = -1;
+            for (; i < row.size() && row[i].col == c; ++i)
+            {
+                int32_t& dst = row[i].in_b ? pb : pa;
+                if (dst >= 0) return false;   // repeated column
+                dst = row[i].pos;
+            }
+            ixu.push_back(c);
+            ka.push_back(pa);
+            kb.push_back(pb);
+        }
+        ipu[r + 1] = (int64_t) ixu.size();
+    }
+    return true;
+}
+
+}  // namespace
+
+// Tiled form of A + tB (R of A's tiles), built once per affine operator on
+// the host from device copies of both patterns. Leaves op->ab.tiles.R = 0
+// (the two-stream CSR SpMM) when A has no tiled form, a pattern repeats a
+// column or the union is too sparse for tiles.
+int ensure_pair_tiles(slq_op* op, bool need_f32)
+{
+    PairTiles& ab = op->ab;
+    if (!ab.tried)
+    {
+        ab.tried = true;
+        const MatrixStore& A = *op->A;
+        const MatrixStore& B = *op->B;
+        const int R = A.tiles.R;
+        if (R == 0 || A.dense || B.dense) return SLQ_OK;
+        const int64_t n = A.csr.nrows;
+        std::vector<int64_t> ipa(n + 1), ipb(n + 1), ipu;
+        std::vector<int32_t> ixa(A.csr.nnz > 0 ? A.csr.nnz : 1),
+            ixb(B.csr.nnz > 0 ? B.csr.nnz : 1), ixu, ka, kb;
+        SLQ_CHECK(hipMemcpy(ipa.data(), A.csr.indptr, sizeof(int64_t) * (n + 1),
+                            hipMemcpyDeviceToHost));
+        SLQ_CHECK(hipMemcpy(ipb.data(), B.csr.indptr, sizeof(int64_t) * (n + 1),
+                            hipMemcpyDeviceToHost));
+        if (A.csr.nnz > 0)
+            SLQ_CHECK(hipMemcpy(ixa.data(), A.csr.indices, sizeof(int32_t) * A.csr.nnz,
+                                hipMemcpyDeviceToHost));
+        if (B.csr.nnz > 0)
+            SLQ_CHECK(hipMemcpy(ixb.data(), B.csr.indices, sizeof(int32_t) * B.csr.nnz,
+                                hipMemcpyDeviceToHost));
+        if (!union_pattern(n, ipa, ixa, ipb, ixb, ipu, ixu, ka, kb)) return SLQ_OK;
+        std::vector<int32_t> src;
+        DevTiles t;
+        SLQ_TRY(build_tiles(n, n, ipu.data(), ixu.data(), nullptr, A.csr.dtype, R, 0.25, t,
+                            &src));
+        if (t.R == 0) return SLQ_OK;
+        if (src.empty())
+        {
+            free_tiles(t);
+            return SLQ_OK;
+        }
+        if (t.src)
+        {
+            (void) hipFree(t.src);
+            t.src = nullptr;
+        }
+        // slot -> union entry -> position in A's / B's values
+        std::vector<int32_t> sa(src.size()), sb(src.size());
+        for (size_t i = 0; i < src.size(); ++i)
+        {
+            sa[i] = src[i] >= 0 ? ka[src[i]] : -1;
+            sb[i] = src[i] >= 0 ? kb[src[i]] : -1;
+        }
+        hipError_t e;
+        if ((e = hipMalloc(&ab.src_a, sizeof(int32_t) * sa.size())) != hipSuccess ||
+            (e = hipMalloc(&ab.src_b, sizeof(int32_t) * sb.size())) != hipSuccess ||
+            (e = hipMemcpy(ab.src_a, sa.data(), sizeof(int32_t) * sa.size(),
+                           hipMemcpyHostToDevice)) != hipSuccess ||
+            (e = hipMemcpy(ab.src_b, sb.data(), sizeof(int32_t) * sb.size(),
+                           hipMemcpyHostToDevice)) != hipSuccess)
+        {
+            free_tiles(t);
+            if (ab.src_a) (void) hipFree(ab.src_a);
+            if (ab.src_b) (void) hipFree(ab.src_b);
+            ab.src_a = ab.src_b = nullptr;
+            return hip_status(e, "pair tile upload", __FILE__, __LINE__);
+        }
+        ab.tiles = t;
+    }
+    if (need_f32 && ab.tiles.R > 0 && op->A->csr.dtype == SLQ_F64 && !ab.tiles.vals_f32)
+        SLQ_CHECK(hipMalloc(&ab.tiles.vals_f32,
+                            sizeof(float) * (size_t) (ab.tiles.nslabs * ab.tiles.R)));
     return SLQ_OK;
 }
 
@@ -852,6 +991,8 @@ int prepare(slq_op* op, int gram, int precision)
             op->B->has_trans = true;
         }
     }
+    else if (op->kind == OP_AFFINE_B)
+        SLQ_TRY(ensure_pair_tiles(op, precision == SLQ_PREC_MIXED));
     if (precision == SLQ_PREC_MIXED)
     {
         SLQ_TRY(ensure_f32(op->stream, *op->A, gram != 0));

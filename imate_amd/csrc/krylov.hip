@@ -1725,6 +1725,82 @@ const DevTiles* tiles_for(slq_op* op, bool trans)
     return &t;
 }
 
+// Values of A + tB in the tile layout of the union pattern: out[i] =
+// a[src_a[i]] + t b[src_b[i]] (absent terms zero), rounded to D.
+template <typename S, typename D>
+__global__ __launch_bounds__(256) void pair_values_kernel(const int32_t* __restrict__ src_a,
+                                                          const int32_t* __restrict__ src_b,
+                                                          const S* __restrict__ a,
+                                                          const S* __restrict__ b, S t,
+                                                          D* __restrict__ out, int64_t count)
+{
+    const int64_t stride = (int64_t) gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride)
+    {
+        const int32_t ka = src_a[i], kb = src_b[i];
+        const S va = ka >= 0 ? a[ka] : S(0);
+        const S vb = kb >= 0 ? b[kb] : S(0);
+        out[i] = (D) (va + t * vb);
+    }
+}
+
+// The tiled A + tB path is on unless SLQ_PAIR_TILES=0 (read per call, so a
+// process can time both paths).
+bool pair_tiles_enabled()
+{
+    const char* e = std::getenv("SLQ_PAIR_TILES");
+    return !(e && std::atoi(e) == 0);
+}
+
+// Tiled form of A + tB at t for SpMMs of value type T, its values gathered
+// on the operator's stream; *out = nullptr keeps the two-stream CSR SpMM.
+template <typename T>
+int pair_tiles_at(slq_op* op, double t, const DevTiles** out)
+{
+    *out = nullptr;
+    PairTiles& ab = op->ab;
+    if (op->kind != OP_AFFINE_B || ab.tiles.R == 0 || !pair_tiles_enabled()) return SLQ_OK;
+    const bool a64 = op->A->csr.dtype == SLQ_F64;
+    if (sizeof(T) == 4 && a64 && !ab.tiles.vals_f32) return SLQ_OK;
+    const int64_t count = ab.tiles.nslabs * ab.tiles.R;
+    const int64_t g = (count + 255) / 256;
+    const dim3 grid((unsigned) (g < 16384 ? g : 16384));
+    if (a64 && sizeof(T) == 4)
+        hipLaunchKernelGGL((pair_values_kernel<double, float>), grid, dim3(256), 0, op->stream,
+                           ab.src_a, ab.src_b, (const double*) op->A->csr.data,
+                           (const double*) op->B->csr.data, t, ab.tiles.vals_f32, count);
+    else if (a64)
+        hipLaunchKernelGGL((pair_values_kernel<double, double>), grid, dim3(256), 0,
+                           op->stream, ab.src_a, ab.src_b, (const double*) op->A->csr.data,
+                           (const double*) op->B->csr.data, t, (double*) ab.tiles.vals,
+                           count);
+    else
+        hipLaunchKernelGGL((pair_values_kernel<float, float>), grid, dim3(256), 0, op->stream,
+                           ab.src_a, ab.src_b, (const float*) op->A->csr.data,
+                           (const float*) op->B->csr.data, (float) t,
+                           (float*) ab.tiles.vals, count);
+    SLQ_CHECK(hipGetLastError());
+    *out = &ab.tiles;
+    return SLQ_OK;
+}
+
+// Operand tiles of a non-transposed SpMM at t: the pair tiles for A + tB,
+// else tiles_for. *shift_mode is the epilogue's shift (none when the pair
+// tiles already hold A + tB).
+template <typename T>
+int operand_tiles(slq_op* op, double t, const DevTiles** out, int* shift_mode)
+{
+    *shift_mode = op->kind == OP_AFFINE_I ? 1 : (op->kind == OP_AFFINE_B ? 2 : 0);
+    if (op->kind != OP_AFFINE_B)
+    {
+        *out = tiles_for<T>(op, false);
+        return SLQ_OK;
+    }
+    SLQ_TRY(pair_tiles_at<T>(op, t, out));
+    if (*out) *shift_mode = 0;
+    return SLQ_OK;
+}
+
 // Buffers of one SpMM launch.
 template <typename T>
 struct SpmmIO {
@@ -1955,7 +2031,9 @@ int lanczos_ring(const RecurrenceArgs& ra, Bump& bump)
                        S[0], ra.sizes, kBlock, s0);
     SLQ_CHECK(hipGetLastError());
     const double tol_sqrt_n = ra.tol * std::sqrt((double) n);
-    const DevTiles* tl = tiles_for<T>(op, false);
+    const DevTiles* tl = nullptr;
+    int shift_mode = 0;
+    SLQ_TRY(operand_tiles<T>(op, ra.t, &tl, &shift_mode));
     const T* X0 = (const T*) ra.X0;
     for (int j = 0; j < m; ++j)
     {
@@ -1965,7 +2043,7 @@ int lanczos_ring(const RecurrenceArgs& ra, Bump& bump)
 
         SpmmArgs<T> s = spmm_defaults<T>();
         fill_matrix<T>(s, A, op->kind == OP_AFFINE_B ? op->B.get() : nullptr, false, f32);
-        s.shift_mode = op->kind == OP_AFFINE_I ? 1 : (op->kind == OP_AFFINE_B ? 2 : 0);
+        s.shift_mode = shift_mode;
         s.shift_len = A.csr.nrows < A.csr.ncols ? A.csr.nrows : A.csr.ncols;
         s.t = ra.t;
         s.xscale = S[j % 3];
@@ -2040,7 +2118,9 @@ int lanczos(const RecurrenceArgs& ra, Bump& bump)
                        scale, ra.sizes, kBlock, s0);
     SLQ_CHECK(hipGetLastError());
     const double tol_sqrt_n = ra.tol * std::sqrt((double) n);
-    const DevTiles* tl = tiles_for<T>(op, false);
+    const DevTiles* tl = nullptr;
+    int shift_mode = 0;
+    SLQ_TRY(operand_tiles<T>(op, ra.t, &tl, &shift_mode));
 
     const T* src = (const T*) ra.X0;
     T* bufs[2] = {Q, P};
@@ -2052,7 +2132,7 @@ int lanczos(const RecurrenceArgs& ra, Bump& bump)
 
         SpmmArgs<T> s = spmm_defaults<T>();
         fill_matrix<T>(s, A, op->kind == OP_AFFINE_B ? op->B.get() : nullptr, false, f32);
-        s.shift_mode = op->kind == OP_AFFINE_I ? 1 : (op->kind == OP_AFFINE_B ? 2 : 0);
+        s.shift_mode = shift_mode;
         s.shift_len = A.csr.nrows < A.csr.ncols ? A.csr.nrows : A.csr.ncols;
         s.t = ra.t;
         s.xscale = scale;
@@ -2299,6 +2379,8 @@ int run_spmm(slq_op* op, double t, int transpose, const void* X, void* Y,
     MatrixStore* Bm = op->kind == OP_AFFINE_B ? op->B.get() : nullptr;
     const int64_t shift_len = A.csr.nrows < A.csr.ncols ? A.csr.nrows : A.csr.ncols;
     const int mode = op->kind == OP_AFFINE_I ? 1 : (op->kind == OP_AFFINE_B ? 2 : 0);
+    if (op->kind == OP_AFFINE_B && !transpose)
+        SLQ_TRY(ensure_pair_tiles(op, value_dtype == SLQ_F32));
     // ticket counters of the in-order tile schedule (no reduction here)
     unsigned int* tickets = nullptr;
     SLQ_CHECK(hipMalloc(&tickets, sizeof(unsigned int) * kTicketWords));
@@ -2326,7 +2408,9 @@ int run_spmm(slq_op* op, double t, int transpose, const void* X, void* Y,
         SpmmIO<float> io;
         io.X = (const float*) X;
         io.Y = (float*) Y;
-        r = launch_spmm<float>(op, s, io, tiles_for<float>(op, transpose != 0));
+        const DevTiles* tl = tiles_for<float>(op, transpose != 0);
+        r = transpose ? SLQ_OK : operand_tiles<float>(op, t, &tl, &s.shift_mode);
+        if (r == SLQ_OK) r = launch_spmm<float>(op, s, io, tl);
     }
     else
     {
@@ -2340,7 +2424,9 @@ int run_spmm(slq_op* op, double t, int transpose, const void* X, void* Y,
         SpmmIO<double> io;
         io.X = (const double*) X;
         io.Y = (double*) Y;
-        r = launch_spmm<double>(op, s, io, tiles_for<double>(op, transpose != 0));
+        const DevTiles* tl = tiles_for<double>(op, transpose != 0);
+        r = transpose ? SLQ_OK : operand_tiles<double>(op, t, &tl, &s.shift_mode);
+        if (r == SLQ_OK) r = launch_spmm<double>(op, s, io, tl);
     }
     const hipError_t e = hipStreamSynchronize(op->stream);
     (void) hipFree(tickets);

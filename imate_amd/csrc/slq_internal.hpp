@@ -120,9 +120,12 @@ struct MatrixStore {
 // With R = 16 every tile's slab count is padded to a multiple of eight (zero
 // slabs repeating the tile's last column) and the values are laid out as the
 // matrix-core SpMM's operands (tile_slot in capi.hip).
+// `data` may be null (values left zero); `src_host`, when given, receives
+// the host copy of the slot -> CSR position map (empty if not kept).
 int build_tiles(int64_t nrows, int64_t ncols, const int64_t* indptr,
                 const int32_t* indices, const void* data, int dtype, int R,
-                double min_density, DevTiles& out);
+                double min_density, DevTiles& out,
+                std::vector<int32_t>* src_host = nullptr);
 int tiles_f32(hipStream_t s, DevTiles& t, int dtype);
 // Refill t.vals from new CSR values through t.src.
 int tiles_refill(hipStream_t s, DevTiles& t, const void* csr_data, int dtype);
@@ -146,6 +149,20 @@ struct ProfileState {
     std::vector<std::pair<hipEvent_t, hipEvent_t>> update_pending;
 };
 
+// A + tB in the tiled form (the general affine operator): the tile
+// structure of the union of A's and B's patterns, and for each slot the
+// position of its value in A's and in B's CSR value arrays (-1: none). The
+// values of A + tB for the t of a recurrence are gathered into tiles.vals
+// (dtype of A; tiles.vals_f32 for the mixed-precision form of an FP64 pair)
+// on the device before it, so new values set on A or B are always seen.
+struct PairTiles {
+    DevTiles tiles;            // tiles.src unused
+    int32_t* src_a = nullptr;  // [nslabs * R]
+    int32_t* src_b = nullptr;  // [nslabs * R]
+    bool tried = false;
+    ~PairTiles();
+};
+
 }  // namespace slq
 
 enum OpKind { OP_MATRIX = 0, OP_AFFINE_I = 1, OP_AFFINE_B = 2 };
@@ -158,6 +175,7 @@ struct slq_op {
     int device = 0;
     slq::Workspace ws;
     slq::ProfileState prof;
+    slq::PairTiles ab;                      // OP_AFFINE_B only
     // launch-grid caps chosen by the autotuner (0 = library default)
     int spmm_grid = 0;
     int stream_grid = 0;
@@ -209,6 +227,9 @@ int run_spmm(slq_op* op, double t, int transpose, const void* X, void* Y,
 // Transpose of a device CSR (stable: rows ascending within a column).
 int build_transpose(hipStream_t s, const DevCSR& A, DevCSR& T);
 int ensure_f32(hipStream_t s, MatrixStore& M, bool need_trans);
+// Builds op->ab for an A + tB operator (once; R = 0 if not profitable) and
+// the FP32 value buffer when `need_f32`.
+int ensure_pair_tiles(slq_op* op, bool need_f32);
 // Transposed copy of a dense matrix (for Golub-Kahn).
 int dense_transpose(hipStream_t s, MatrixStore& M);
 
