@@ -56,7 +56,7 @@ def parse():
                         'points to this .npz file')
     p.add_argument('--form', default='plain',
                    choices=['plain', 'precision', 'batched', 'surrogate',
-                            'gram', 'dense'],
+                            'gram', 'dense', 'general_b'],
                    help='plain: the headline workload (default); the others '
                         'measure the north-star forms and print their own '
                         'JSON lines')
@@ -171,7 +171,9 @@ def batched_form(args):
     run per probe with Ritz shifts for every t, against a loop of one SLQ run
     per t. (ii) 32 correlation matrices with sizes log-uniform in [1e3, 2e5],
     ~30 non-zeros per row, FP32: one hipGraph launch over all problems (with
-    autotuned launch grids) against a per-problem launch loop."""
+    autotuned launch grids) against a per-problem launch loop. (iii) A + tB
+    with a general B, 8 t, n = 262144: the tiled union-pattern SpMM against
+    the two-stream CSR SpMM (general_b_leg)."""
     import imate_amd
     from imate_amd.batch import slq_batch
     from imate_amd.sample_matrices import sparse_correlation_matrix
@@ -204,6 +206,9 @@ def batched_form(args):
         't_evaluations_per_s': len(ts) * s / t_shift,
         'max_rel_diff_shift_vs_direct': float(rel)}
     log('affine: %s' % json.dumps(out['affine_t256']))
+    # (iii) many t of A + tB, general B: one recurrence per t on the tiles of
+    # the union pattern, against the two-stream CSR SpMM (SLQ_PAIR_TILES=0)
+    out['affine_general_b'] = general_b_leg(args.seed, m, s)
     # (ii) 32 uneven matrices
     rng = numpy.random.default_rng(args.seed)
     sizes = numpy.round(numpy.exp(rng.uniform(numpy.log(1e3), numpy.log(2e5),
@@ -234,6 +239,49 @@ def batched_form(args):
     log('matrices: %s' % json.dumps({k: v for k, v in out['matrices32'].items()
                                      if k not in ('sizes', 'autotuned_grids')}))
     print(json.dumps(out), flush=True)
+
+
+def general_b_leg(seed, m, s, n=262144, num_t=8):
+    """A + tB for num_t log-spaced t in [1e-3, 10]: A an n x n correlation
+    matrix (~50 non-zeros per row), B SPD on a wider pattern of the same
+    points (~80 per row) plus about one random symmetric pair per row."""
+    import scipy.sparse
+    import imate_amd
+    from imate_amd.sample_matrices import sparse_correlation_matrix
+    A = sparse_correlation_matrix(n, nnz_per_row=50, seed=0)
+    C = sparse_correlation_matrix(n, nnz_per_row=80, seed=0)
+    rng = numpy.random.default_rng(1)
+    k = n // 2
+    R = scipy.sparse.csr_matrix((rng.random(k), (rng.integers(0, n, k),
+                                                 rng.integers(0, n, k))),
+                                shape=(n, n))
+    S = 0.1 * (R + R.T)
+    B = (C + S + scipy.sparse.diags(
+        numpy.asarray(abs(S).sum(axis=1)).ravel())).tocsr()
+    ts = numpy.logspace(-3, 1, num_t)
+    Aop = imate_amd.AffineMatrixFunction(A, B)
+    kw = dict(parameters=ts, method='slq', gpu=True, lanczos_degree=m,
+              min_num_samples=s, max_num_samples=s, seed=seed)
+    res = {'n': n, 'nnz_a': int(A.nnz), 'nnz_b': int(B.nnz), 'num_t': num_t}
+    vals = {}
+    for path, env in (('pair_tiles', None), ('csr', '0')):
+        if env is None:
+            os.environ.pop('SLQ_PAIR_TILES', None)
+        else:
+            os.environ['SLQ_PAIR_TILES'] = env
+        imate_amd.logdet(Aop, **kw)            # builds, warms up
+        t0 = time.perf_counter()
+        vals[path] = numpy.asarray(imate_amd.logdet(Aop, **kw))
+        sec = time.perf_counter() - t0
+        res[path] = {'seconds': sec,
+                     'matvec_steps_per_s': num_t * s * m / sec}
+    os.environ.pop('SLQ_PAIR_TILES', None)
+    res['speedup'] = res['csr']['seconds'] / res['pair_tiles']['seconds']
+    res['max_rel_diff'] = float(numpy.max(numpy.abs(vals['pair_tiles'] -
+                                                    vals['csr']) /
+                                          numpy.abs(vals['csr'])))
+    log('affine general B: %s' % json.dumps(res))
+    return res
 
 
 def interpolator_baseline(fam, ls, ts, t_range, m, samples, slq, ci, pred,
@@ -594,6 +642,12 @@ def main():
         return gram_form(args)
     if args.form == 'dense':
         return dense_form(args)
+    if args.form == 'general_b':
+        print(json.dumps({'form': 'general_b', 'lanczos_degree': 30,
+                          'samples': 64,
+                          'affine_general_b': general_b_leg(args.seed, 30, 64)}),
+              flush=True)
+        return
 
     from imate_amd.sample_matrices import sparse_correlation_matrix
     n, k, m, s = args.n, args.nnz_per_row, args.lanczos_degree, args.samples
