@@ -47,6 +47,8 @@ SIGNATURES = {
                                              ctypes.c_size_t]),
     'slq_op_create_csr': (ctypes.c_int, [_i64, _i64, _i64, _P, _P, _P,
                                          ctypes.c_int, _P]),
+    'slq_op_create_csc': (ctypes.c_int, [_i64, _i64, _i64, _P, _P, _P,
+                                         ctypes.c_int, _P]),
     'slq_op_create_affine': (ctypes.c_int, [_P, _P, _P]),
     'slq_op_create_dense': (ctypes.c_int, [_i64, _i64, _P, ctypes.c_int, _P]),
     'slq_op_destroy': (ctypes.c_int, [_P]),

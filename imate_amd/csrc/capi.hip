@@ -1528,6 +1528,66 @@ int slq_op_create_csr(int64_t nrows, int64_t ncols, int64_t nnz,
     return SLQ_OK;
 }
 
+int slq_op_create_csc(int64_t nrows, int64_t ncols, int64_t nnz,
+                      const int64_t* indptr, const int32_t* indices,
+                      const void* data, int dtype, slq_op** out)
+{
+    if (!out || nrows < 1 || ncols < 1 || nnz < 0 || !indptr ||
+        (nnz > 0 && (!indices || !data)) || (dtype != SLQ_F64 && dtype != SLQ_F32))
+    {
+        set_error("slq_op_create_csc: invalid argument");
+        return SLQ_E_ARG;
+    }
+    if (nrows > INT32_MAX || ncols > INT32_MAX)
+    {
+        set_error("slq_op_create_csc: more than 2^31-1 rows or columns");
+        return SLQ_E_ARG;
+    }
+    if (indptr[0] != 0 || indptr[ncols] != nnz)
+    {
+        set_error("slq_op_create_csc: indptr[0] must be 0 and indptr[ncols] == nnz");
+        return SLQ_E_ARG;
+    }
+    for (int64_t c = 0; c < ncols; ++c)
+        if (indptr[c + 1] < indptr[c])
+        {
+            set_error("slq_op_create_csc: indptr decreases at column %lld", (long long) c);
+            return SLQ_E_ARG;
+        }
+    for (int64_t k = 0; k < nnz; ++k)
+        if (indices[k] < 0 || (int64_t) indices[k] >= nrows)
+        {
+            set_error("slq_op_create_csc: row index %d out of range at %lld", indices[k],
+                      (long long) k);
+            return SLQ_E_ARG;
+        }
+    // CSR by a counting sort over rows; columns are visited in ascending
+    // order, so every row's column indices come out sorted (and repeated
+    // entries stay, in column order). order[j] is the CSC position of CSR
+    // entry j, kept for new values given in CSC order.
+    std::vector<int64_t> ip(nrows + 1, 0);
+    for (int64_t k = 0; k < nnz; ++k) ++ip[indices[k] + 1];
+    for (int64_t r = 0; r < nrows; ++r) ip[r + 1] += ip[r];
+    std::vector<int64_t> next(ip.begin(), ip.end() - 1);
+    std::vector<int32_t> ix(nnz > 0 ? nnz : 1);
+    std::vector<int64_t> order(nnz > 0 ? nnz : 1);
+    for (int64_t c = 0; c < ncols; ++c)
+        for (int64_t k = indptr[c]; k < indptr[c + 1]; ++k)
+        {
+            const int64_t j = next[indices[k]]++;
+            ix[j] = (int32_t) c;
+            order[j] = k;
+        }
+    const size_t vb = dtype == SLQ_F64 ? 8 : 4;
+    std::vector<char> dv(vb * (nnz > 0 ? nnz : 1));
+    for (int64_t j = 0; j < nnz; ++j)
+        std::memcpy(dv.data() + vb * j, (const char*) data + vb * order[j], vb);
+    SLQ_TRY(slq_op_create_csr(nrows, ncols, nnz, ip.data(), ix.data(), dv.data(), dtype, out));
+    (*out)->A->csc_order.swap(order);
+    if (nnz == 0) (*out)->A->csc_order.clear();
+    return SLQ_OK;
+}
+
 }  // extern "C"
 
 namespace {
@@ -1800,7 +1860,15 @@ int slq_op_set_values(slq_op* op, const void* data)
     }
     SLQ_CHECK(hipDeviceSynchronize());
     const size_t vb = M.csr.dtype == SLQ_F64 ? 8 : 4;
-    if (M.csr.nnz > 0)
+    if (M.csr.nnz > 0 && !M.csc_order.empty())
+    {
+        // values in the CSC order of slq_op_create_csc
+        std::vector<char> perm(vb * M.csr.nnz);
+        for (int64_t j = 0; j < M.csr.nnz; ++j)
+            std::memcpy(perm.data() + vb * j, (const char*) data + vb * M.csc_order[j], vb);
+        SLQ_CHECK(hipMemcpy(M.csr.data, perm.data(), perm.size(), hipMemcpyHostToDevice));
+    }
+    else if (M.csr.nnz > 0)
         SLQ_CHECK(hipMemcpy(M.csr.data, data, vb * M.csr.nnz, hipMemcpyHostToDevice));
     SLQ_TRY(tiles_refill(op->stream, M.tiles, M.csr.data, M.csr.dtype));
     // derived forms are rebuilt from the new values on their next use

@@ -112,6 +112,9 @@ struct MatrixStore {
     DevTiles tiles;                   // of csr
     DevTiles trans_tiles;             // of trans
     bool trans_tiles_tried = false;
+    // made from CSC arrays: CSC position of each CSR entry (host), so that
+    // new values can be given in CSC order; empty otherwise
+    std::vector<int64_t> csc_order;
     ~MatrixStore();
 };
 

@@ -31,7 +31,8 @@ class DeviceOperator(object):
     @staticmethod
     def from_matrix(A, dense_ok=True):
         """Device copy of A: a dense handle for a numpy array (the matrix
-        cores multiply it as is, imate's cuDenseMatrix), a CSR handle for a
+        cores multiply it as is, imate's cuDenseMatrix), a CSC handle for a
+        scipy CSC matrix (imate's cuCSCMatrix), a CSR handle for any other
         scipy sparse matrix or when ``dense_ok`` is False."""
         lib = _lib.load()
         if A.ndim != 2:
@@ -43,6 +44,17 @@ class DeviceOperator(object):
             _lib.check(lib.slq_op_create_dense(
                 D.shape[0], D.shape[1], _lib.ptr(D), code, ctypes.byref(h)),
                 'slq_op_create_dense')
+            return DeviceOperator(h)
+        if scipy.sparse.issparse(A) and A.format == 'csc':
+            # CSC handle (imate's cuCSCMatrix); the library converts it once
+            indptr = numpy.ascontiguousarray(A.indptr, dtype=numpy.int64)
+            indices = numpy.ascontiguousarray(A.indices, dtype=numpy.int32)
+            data = numpy.ascontiguousarray(A.data, dtype=A.dtype)
+            h = ctypes.c_void_p()
+            _lib.check(lib.slq_op_create_csc(
+                A.shape[0], A.shape[1], A.nnz, _lib.ptr(indptr),
+                _lib.ptr(indices), _lib.ptr(data), code, ctypes.byref(h)),
+                'slq_op_create_csc')
             return DeviceOperator(h)
         if scipy.sparse.issparse(A):
             M = scipy.sparse.csr_matrix(A)

@@ -79,6 +79,19 @@ int slq_op_create_csr(int64_t nrows, int64_t ncols, int64_t nnz,
                       const int64_t* indptr, const int32_t* indices,
                       const void* data, int dtype, slq_op** out);
 
+/* Upload a CSC matrix: indptr[ncols + 1], row indices[nnz] (any order
+ * within a column; repeated entries are summed by the products) and
+ * data[nnz]. The handle holds the matrix as sorted CSR (a counting sort on
+ * the host, once) and from there runs every path of a CSR handle: 16-row
+ * tiles on the matrix cores, the device transpose for Golub-Kahn, A + tB.
+ * slq_op_set_values on it takes values in this CSC order. Replaces the CSC
+ * branch of pycuMatrix and cuCSCMatrix<T> (imate/_cu_linear_operator/
+ * py_cu_matrix.pyx:118-200; cu_csc_matrix.cu:52-84,223-459, whose dot is a
+ * cuSPARSE CSC SpMV per vector). */
+int slq_op_create_csc(int64_t nrows, int64_t ncols, int64_t nnz,
+                      const int64_t* indptr, const int32_t* indices,
+                      const void* data, int dtype, slq_op** out);
+
 /* Dense matrix handle from host row-major data[nrows][ncols] (dtype
  * SLQ_F64 or SLQ_F32). The SpMM runs on the matrix cores with operand rows
  * staged through LDS; A + tI over it is allowed (slq_op_create_affine with
@@ -100,7 +113,8 @@ int slq_op_create_dense(int64_t nrows, int64_t ncols, const void* data, int dtyp
 int slq_op_create_affine(slq_op* A, slq_op* B, slq_op** out);
 
 /* Replace the values of a matrix handle, keeping its sparsity pattern:
- * data[nnz] in the handle's dtype and CSR order of slq_op_create_csr. The
+ * data[nnz] in the handle's dtype and CSR order of slq_op_create_csr (CSC
+ * order for a handle made by slq_op_create_csc). The
  * tiled layout is refilled on the device; the transpose and FP32 copies are
  * rebuilt on next use. Affine handles made from this matrix see the new
  * values. Fails with SLQ_E_ARG when the CSR had duplicate entries.

@@ -63,6 +63,24 @@ def test_invalid_csr_rejected_by_the_c_abi():
     assert r in (_lib.SLQ_E_ARG, _lib.SLQ_E_NODEVICE)
 
 
+def test_invalid_csc_rejected_by_the_c_abi():
+    lib = _lib.load()
+    indptr = numpy.array([0, 1, 2], dtype=numpy.int64)
+    data = numpy.ones(2)
+    h = ctypes.c_void_p()
+    for rows in ([0, 3], [-1, 0]):       # row 3 of 3 rows; a negative row
+        indices = numpy.array(rows, dtype=numpy.int32)
+        r = lib.slq_op_create_csc(3, 2, 2, _lib.ptr(indptr), _lib.ptr(indices),
+                                  _lib.ptr(data), _lib.SLQ_F64,
+                                  ctypes.byref(h))
+        assert r == _lib.SLQ_E_ARG
+    bad_ptr = numpy.array([0, 2, 1, 2], dtype=numpy.int64)   # decreasing
+    indices = numpy.array([0, 1], dtype=numpy.int32)
+    r = lib.slq_op_create_csc(3, 3, 2, _lib.ptr(bad_ptr), _lib.ptr(indices),
+                              _lib.ptr(data), _lib.SLQ_F64, ctypes.byref(h))
+    assert r == _lib.SLQ_E_ARG
+
+
 A_SMALL = toeplitz(2.0, 1.0, 20, gram=True)
 
 
