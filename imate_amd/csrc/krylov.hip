@@ -357,16 +357,17 @@ __global__ __launch_bounds__(kThreads) void spmm_kernel(SpmmArgs<T> a,
 // non-zero gathers a random operand row, and rows drawn from a table larger
 // than the L2 arrive at about half the rate of rows drawn from an
 // L2-resident table (measured: 14-15 vs 27.7 G rows/s for 512 B rows). A
-// wavefront owns kBlkRows rows and sweeps the columns in blocks of `cb`
-// (cb x 64 operand values fit the XCD's L2 several times over); the waves of
-// an XCD start together on neighbouring row groups and carry equal work per
-// block, so they sweep the blocks nearly in step and each block's operand
-// rows are fetched into the L2 once per row batch. Measured gain: 6 % on the
-// 2^20 x 2^18 Golub-Kahn case (the ideal is 1.85x; nothing keeps the waves
-// of an XCD in step, so they may drift apart over a sweep). Each row keeps a
-// window of 64 (column, value) pairs in registers with a cursor; the pairs of one
-// block are a prefix of what is left of the window (columns ascend), found
-// with a ballot. Per row the products are added in CSR order with the same
+// wavefront owns kBlkRows rows and sweeps the columns in blocks of `cb`; the
+// waves of a workgroup pass each block together (a barrier per block), and
+// the grid holds only as many workgroups as fit on the chip, so all of them
+// start the sweep together. The workgroups of an XCD still drift apart by a
+// few blocks, so a block must take well under the 4 MB L2: on the 2^20 x
+// 2^18 Golub-Kahn case the L2 hit rate is 92 % with 256-column blocks and
+// 13-14 % with 2048 or more, and 1024 columns (512 KB of operand rows) is
+// fastest (1.27x the row kernel; 256 columns pays more in per-block work
+// than it gains). Each row keeps a window of 64 (column, value) pairs in
+// registers with a cursor; the pairs of one block are a prefix of what is
+// left of the window (columns ascend), found with a ballot. Per row the products are added in CSR order with the same
 // fma chain as row_gather, so the result equals spmm_kernel's bit for bit.
 constexpr int kBlkRows = 8;
 
@@ -387,10 +388,17 @@ __global__ __launch_bounds__(kThreads) void csr_blocked_kernel(SpmmArgs<T> a,
     const T sub_c = Sub ? (T) a.sub_coef[lane] : T(0);
     const T sub_s = Sub ? (T) a.sub_scale[lane] : T(0);
     const int64_t nb = (a.ncols + cb - 1) / cb;
+    // Every wave of the grid runs the same number of rounds (a group index
+    // past its XCD's span is an empty group), so the workgroup barrier after
+    // each column block is reached by all of its waves: the waves of a
+    // workgroup move from block to block together.
+    const int64_t span = (ngroups + kXcds - 1) / kXcds;
+    const int64_t rounds = (span + sw.stride - 1) / sw.stride;
     double red = 0.0;
-    for (int64_t gi = sw.first; gi < sw.end; gi += sw.stride)
+    for (int64_t round = 0; round < rounds; ++round)
     {
-        const int64_t r0 = gi * kBlkRows;
+        const int64_t gi = sw.first + round * sw.stride;
+        const int64_t r0 = gi < sw.end ? gi * kBlkRows : a.nrows;
         int64_t cur[kBlkRows], end[kBlkRows];
         int wc[kBlkRows], wpos[kBlkRows], wcnt[kBlkRows];
         T wv[kBlkRows], acc[kBlkRows];
@@ -454,6 +462,7 @@ __global__ __launch_bounds__(kThreads) void csr_blocked_kernel(SpmmArgs<T> a,
                     load_window(k);
                 }
             }
+            __syncthreads();
         }
 #pragma unroll
         for (int k = 0; k < kBlkRows; ++k)
@@ -1687,7 +1696,7 @@ int csr_blocked_cols(const SpmmArgs<T>& s)
 {
     static int cb = [] {
         const char* e = std::getenv("SLQ_CSR_BLOCK_COLS");
-        return e ? std::atoi(e) : 2048;
+        return e ? std::atoi(e) : 1024;
     }();
     if (cb <= 0 || !s.sorted || s.shift_mode == 2 || s.nrows <= 0) return 0;
     if ((double) s.ncols * kBlock * sizeof(T) <= 16.0 * (1 << 20)) return 0;
@@ -1880,7 +1889,20 @@ void launch_spmm_nt(slq_op* op, const SpmmArgs<T>& s, const SpmmIO<T>& io, const
     else if (csr_blocked_cols(s) > 0)
     {
         const int64_t ngroups = (s.nrows + kBlkRows - 1) / kBlkRows;
-        const int grid = grid_for(ngroups, op->spmm_grid);
+        int grid = grid_for(ngroups, op->spmm_grid);
+        // no more workgroups than fit on the chip at once, so that all of
+        // them start the column sweep together
+        static const int resident = [] {
+            int per_cu = 0, cus = 0, dev = 0;
+            if (hipGetDevice(&dev) != hipSuccess ||
+                hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) !=
+                    hipSuccess ||
+                hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                    &per_cu, csr_blocked_kernel<T, NT>, kThreads, 0) != hipSuccess)
+                return 0;
+            return ((per_cu * cus) / kXcds) * kXcds;
+        }();
+        if (resident >= kXcds && grid > resident) grid = resident;
         hipLaunchKernelGGL((csr_blocked_kernel<T, NT>), dim3(grid), dim3(kThreads), 0,
                            op->stream, s, io.X, io.Y, io.Vout, io.Sub, csr_blocked_cols(s));
         if (s.red_mode)
