@@ -74,6 +74,23 @@ struct Sweep {
     int64_t stride;
 };
 
+template <int WPG>
+__device__ __forceinline__ Sweep sweep_wpg(int64_t N, int wave)
+{
+    const int64_t G = gridDim.x;
+    const int64_t bid = blockIdx.x;
+    const int64_t x = bid % kXcds;
+    const int64_t local = bid / kXcds;
+    const int64_t per_xcd = G / kXcds;
+    const int64_t span = (N + kXcds - 1) / kXcds;
+    const int64_t b = x * span;
+    Sweep s;
+    s.end = b + span < N ? b + span : N;
+    s.first = b + local * WPG + wave;
+    s.stride = per_xcd * WPG;
+    return s;
+}
+
 __device__ __forceinline__ Sweep sweep(int64_t N, int wave)
 {
     const int64_t G = gridDim.x;
@@ -168,9 +185,10 @@ constexpr int kTicketWords = kXcds * 64;
 // one-workgroup launch adds the rows in order (ordered_total), so no kernel
 // needs a grid-wide fence: each such fence writes back the XCD's L2, which
 // cost a streaming kernel over a tenth of its time.
+template <int WPG = kWavesPerWG>
 __device__ __forceinline__ void wg_partial(double v, const RedBuf& rb)
 {
-    __shared__ double red[kWavesPerWG][kWave];
+    __shared__ double red[WPG][kWave];
     const int lane = threadIdx.x % kWave;
     const int wave = wave_id();
     red[wave][lane] = v;
@@ -178,7 +196,7 @@ __device__ __forceinline__ void wg_partial(double v, const RedBuf& rb)
     if (wave != 0) return;
     double s = 0.0;
 #pragma unroll
-    for (int w = 0; w < kWavesPerWG; ++w) s += red[w][lane];
+    for (int w = 0; w < WPG; ++w) s += red[w][lane];
     rb.partial[(int64_t) blockIdx.x * kWave + lane] = s;
 }
 
@@ -358,31 +376,32 @@ __global__ __launch_bounds__(kThreads) void spmm_kernel(SpmmArgs<T> a,
 // than the L2 arrive at about half the rate of rows drawn from an
 // L2-resident table (measured: 14-15 vs 27.7 G rows/s for 512 B rows). A
 // wavefront owns kBlkRows rows and sweeps the columns in blocks of `cb`; the
-// waves of a workgroup pass each block together (a barrier per block), and
-// the grid holds only as many workgroups as fit on the chip, so all of them
-// start the sweep together. The workgroups of an XCD still drift apart by a
-// few blocks, so a block must take well under the 4 MB L2: on the 2^20 x
-// 2^18 Golub-Kahn case the L2 hit rate is 92 % with 256-column blocks and
-// 13-14 % with 2048 or more, and 1024 columns (512 KB of operand rows) is
-// fastest (1.27x the row kernel; 256 columns pays more in per-block work
-// than it gains). Each row keeps a window of 64 (column, value) pairs in
-// registers with a cursor; the pairs of one block are a prefix of what is
-// left of the window (columns ascend), found with a ballot. Per row the products are added in CSR order with the same
+// waves of a workgroup pass each block together (a barrier per block), the
+// workgroups are as large as allowed (16 wavefronts, one per CU) and the
+// grid holds only as many as fit on the chip, so all of them start the
+// sweep together. The workgroups of an XCD still drift apart by a few
+// blocks, so a block must take well under the 4 MB L2: on the 2^20 x 2^18
+// Golub-Kahn case (8-wavefront workgroups) the L2 hit rate was 92 % with
+// 256-column blocks and 13-14 % with 2048 or more; with 16-wavefront
+// workgroups 768 columns (384 KB of operand rows) is fastest, 1.64x the row
+// kernel. Each row keeps a window of 64 (column, value) pairs in registers
+// with a cursor; the pairs of one block are a prefix of what is left of the
+// window (columns ascend), found with a ballot. Per row the products are added in CSR order with the same
 // fma chain as row_gather, so the result equals spmm_kernel's bit for bit.
 constexpr int kBlkRows = 8;
 
-template <typename T, bool NT>
-__global__ __launch_bounds__(kThreads) void csr_blocked_kernel(SpmmArgs<T> a,
-                                                               const T* __restrict__ X,
-                                                               T* __restrict__ Y,
-                                                               T* __restrict__ Vout,
-                                                               const T* __restrict__ Sub,
-                                                               int cb)
+template <typename T, bool NT, int WPG>
+__global__ __launch_bounds__(WPG * kWave) void csr_blocked_kernel(SpmmArgs<T> a,
+                                                                  const T* __restrict__ X,
+                                                                  T* __restrict__ Y,
+                                                                  T* __restrict__ Vout,
+                                                                  const T* __restrict__ Sub,
+                                                                  int cb)
 {
     const int lane = threadIdx.x % kWave;
     const int wave = wave_id();
     const int64_t ngroups = (a.nrows + kBlkRows - 1) / kBlkRows;
-    const Sweep sw = sweep(ngroups, wave);
+    const Sweep sw = sweep_wpg<WPG>(ngroups, wave);
     const T s = a.xscale ? (T) a.xscale[lane] : T(1);
     const T tt = (T) a.t;
     const T sub_c = Sub ? (T) a.sub_coef[lane] : T(0);
@@ -470,7 +489,7 @@ __global__ __launch_bounds__(kThreads) void csr_blocked_kernel(SpmmArgs<T> a,
                 spmm_epilogue<T, NT>(a, r0 + k, lane, acc[k], s, tt, sub_c, sub_s, X, Y, Vout,
                                      Sub, red);
     }
-    spmm_finish<T>(a, red);
+    if (a.red_mode != 0) wg_partial<WPG>(red, a.rb);
 }
 
 // Row-tiled form: one wavefront per tile of R rows, R accumulators per lane.
@@ -1696,7 +1715,7 @@ int csr_blocked_cols(const SpmmArgs<T>& s)
 {
     static int cb = [] {
         const char* e = std::getenv("SLQ_CSR_BLOCK_COLS");
-        return e ? std::atoi(e) : 1024;
+        return e ? std::atoi(e) : 768;
     }();
     if (cb <= 0 || !s.sorted || s.shift_mode == 2 || s.nrows <= 0) return 0;
     if ((double) s.ncols * kBlock * sizeof(T) <= 16.0 * (1 << 20)) return 0;
@@ -1892,18 +1911,27 @@ void launch_spmm_nt(slq_op* op, const SpmmArgs<T>& s, const SpmmIO<T>& io, const
         int grid = grid_for(ngroups, op->spmm_grid);
         // no more workgroups than fit on the chip at once, so that all of
         // them start the column sweep together
-        static const int resident = [] {
+        // 1024-thread workgroups (one per CU): half as many workgroups per
+        // XCD to drift apart as with 512 threads (SLQ_BLK_WAVES=8)
+        static const int wpg = [] {
+            const char* e = std::getenv("SLQ_BLK_WAVES");
+            return e && std::atoi(e) == 8 ? 8 : 16;
+        }();
+        auto kern = wpg == 16 ? csr_blocked_kernel<T, NT, 16> : csr_blocked_kernel<T, NT, 8>;
+        const int threads = wpg * kWave;
+        if (wpg == 16) grid = (int) std::max<int64_t>(kXcds, ((grid / 2) / kXcds) * kXcds);
+        static const int resident = [&] {
             int per_cu = 0, cus = 0, dev = 0;
             if (hipGetDevice(&dev) != hipSuccess ||
                 hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) !=
                     hipSuccess ||
-                hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                    &per_cu, csr_blocked_kernel<T, NT>, kThreads, 0) != hipSuccess)
+                hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, threads, 0) !=
+                    hipSuccess)
                 return 0;
             return ((per_cu * cus) / kXcds) * kXcds;
         }();
         if (resident >= kXcds && grid > resident) grid = resident;
-        hipLaunchKernelGGL((csr_blocked_kernel<T, NT>), dim3(grid), dim3(kThreads), 0,
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(threads), 0,
                            op->stream, s, io.X, io.Y, io.Vout, io.Sub, csr_blocked_cols(s));
         if (s.red_mode)
             hipLaunchKernelGGL((total_kernel<T>), dim3(1), dim3(kTotalWaves * kWave), 0,
