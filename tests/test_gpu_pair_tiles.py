@@ -208,3 +208,24 @@ def test_pair_mixed_precision_close_to_fp64():
     a = numpy.asarray(imate_amd.logdet(Aop, **kw))
     b = numpy.asarray(imate_amd.logdet(Aop, precision='mixed', **kw))
     assert numpy.all(numpy.abs(a - b) <= 1e-5 * numpy.abs(a))
+
+
+def test_gram_general_b_keeps_csr_streams():
+    """Golub-Kahn on A + tB with a general B runs the two CSR streams and
+    their transposes (no union tiles of the transposes); per sample against
+    the oracle."""
+    from imate_amd.sample_matrices import toeplitz
+    A = toeplitz(2.0, 1.0, 60, gram=False)
+    B = scipy.sparse.diags(numpy.linspace(0.5, 1.5, 60)).tocsr()
+    t = [0.3, 1.2]
+    val, info = imate_amd.logdet(imate_amd.AffineMatrixFunction(A, B),
+                                 parameters=t, gram=True, method='slq',
+                                 gpu=True, lanczos_degree=20,
+                                 min_num_samples=16, max_num_samples=16,
+                                 seed=3, return_info=True)
+    ref = oracle.slq(A, func='log', gram=True, parameters=t, B=B,
+                     affine=True, lanczos_degree=20, min_num_samples=16,
+                     max_num_samples=16, seed=3)
+    got = numpy.asarray(info['convergence']['samples'])
+    assert numpy.max(numpy.abs(got - ref['samples']) /
+                     numpy.abs(ref['samples'])) < 1e-10
