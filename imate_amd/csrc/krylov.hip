@@ -1489,6 +1489,69 @@ __global__ __launch_bounds__(kThreads) void gs_sub_kernel(GsArgs a, T* __restric
     if (a.do_norm) wg_partial(red, a.rb);
 }
 
+// The first subtraction and the second product of CGS2 in one pass over the
+// basis: per row r' = r - sum_k h1_k v_k (k order, all K <= C vectors held
+// in registers), stored, and partial_k[g][k] = sum over the rows of
+// workgroup g of v_k r'. Per row this is the arithmetic of gs_sub_kernel
+// followed by gs_dots_kernel, with one read of the basis instead of two.
+// C <= 32: at C = 32 the values, the sums and the addresses take 252 VGPRs
+// (64 FP32 vectors would spill).
+template <typename T, int C>
+__global__ __launch_bounds__(kThreads) void gs_subdots_kernel(GsArgs a, T* __restrict__ R,
+                                                              const T* __restrict__ basis)
+{
+    __shared__ double hs[C][kWave];
+    __shared__ double red[kWavesPerWG][8][kWave];
+    const int lane = threadIdx.x % kWave;
+    const int wave = wave_id();
+    const Sweep sw = sweep_linear(a.nrows, wave);
+    const int64_t stride = a.slot;
+    const int kn = a.K;
+    for (int i = threadIdx.x; i < C * kWave; i += blockDim.x)
+        hs[i / kWave][i % kWave] = i < kn * kWave ? a.h[i] : 0.0;
+    __syncthreads();
+    const T* __restrict__ base = basis + (int64_t) a.last * stride;
+    double acc[C];
+#pragma unroll
+    for (int g = 0; g < C; ++g) acc[g] = 0.0;
+#pragma unroll 1
+    for (int64_t row = sw.first; row < sw.end; row += sw.stride)
+    {
+        // re-read hs from LDS every row rather than hold C coefficients in
+        // registers beside the C values and C sums
+        asm volatile("" ::: "memory");
+        const int64_t o = row * kBlock + lane;
+        T v[C];
+#pragma unroll
+        for (int g = 0; g < C; ++g) v[g] = base[o - (int64_t) (g < kn ? g : kn - 1) * stride];
+        double r = (double) R[o];
+#pragma unroll
+        for (int g = 0; g < C; ++g) r -= hs[g][lane] * (double) v[g];
+        const T rt = (T) r;
+        R[o] = rt;
+        const double rd = (double) rt;
+#pragma unroll
+        for (int g = 0; g < C; ++g) acc[g] += rd * (double) v[g];
+    }
+    // workgroup sums, eight vectors at a time through LDS
+#pragma unroll
+    for (int g0 = 0; g0 < C; g0 += 8)
+    {
+        if (g0 >= kn) break;
+#pragma unroll
+        for (int g = 0; g < 8; ++g) red[wave][g][lane] = acc[g0 + g];
+        __syncthreads();
+        if (wave < 8 && g0 + wave < kn)
+        {
+            double t = 0.0;
+#pragma unroll
+            for (int w = 0; w < kWavesPerWG; ++w) t += red[w][wave][lane];
+            a.partial_k[((int64_t) blockIdx.x * a.K + g0 + wave) * kWave + lane] = t;
+        }
+        __syncthreads();
+    }
+}
+
 // out[k][p] = sum_g partial[g][k][p] in a fixed order; one workgroup per k.
 __global__ __launch_bounds__(kTotalWaves * kWave) void reduce_partials_kernel(
     const double* __restrict__ partial, int G, int K, double* __restrict__ out)
@@ -1995,6 +2058,14 @@ int launch_update(slq_op* op, const UpdateArgs& u, T* R, const T* Vj, const T* V
 // K partial rows.
 int gs_grid(int64_t rows, int cap) { return stream_grid(rows, cap); }
 
+// The fused subtraction + product pass is on unless SLQ_FUSED_GS=0 (read per
+// call, so a process can time both).
+bool fused_gs_enabled()
+{
+    const char* e = std::getenv("SLQ_FUSED_GS");
+    return !(e && std::atoi(e) == 0);
+}
+
 // CGS2 of r against the K newest basis vectors (slots last, last-1, ...);
 // the final pass finishes the norm (beta_j, or alpha_j for Golub-Kahn u).
 template <typename T>
@@ -2021,10 +2092,27 @@ int reorthogonalize(slq_op* op, int64_t nrows, T* R, const T* basis, size_t slot
         hipLaunchKernelGGL(reduce_partials_kernel, dim3(K), tg, 0, op->stream, partial_k, G,
                            K, h1);
         g.h = h1;
-        hipLaunchKernelGGL(gs_sub_kernel<T>, dim3(G), dim3(kThreads), 0, op->stream, g, R,
-                           basis);
-        hipLaunchKernelGGL(gs_dots_kernel<T>, dim3(G), dim3(kThreads), 0, op->stream, g, R,
-                           basis);
+        // r -= V h1 and h2 = V^T r in one pass while the K vectors of a row
+        // fit in registers without spilling (K <= 32), else two passes
+        if (K <= 32 && fused_gs_enabled())
+        {
+            if (K > 16)
+                hipLaunchKernelGGL((gs_subdots_kernel<T, 32>), dim3(G), dim3(kThreads), 0,
+                                   op->stream, g, R, basis);
+            else if (K > 8)
+                hipLaunchKernelGGL((gs_subdots_kernel<T, 16>), dim3(G), dim3(kThreads), 0,
+                                   op->stream, g, R, basis);
+            else
+                hipLaunchKernelGGL((gs_subdots_kernel<T, 8>), dim3(G), dim3(kThreads), 0,
+                                   op->stream, g, R, basis);
+        }
+        else
+        {
+            hipLaunchKernelGGL(gs_sub_kernel<T>, dim3(G), dim3(kThreads), 0, op->stream, g, R,
+                               basis);
+            hipLaunchKernelGGL(gs_dots_kernel<T>, dim3(G), dim3(kThreads), 0, op->stream, g,
+                               R, basis);
+        }
         hipLaunchKernelGGL(reduce_partials_kernel, dim3(K), tg, 0, op->stream, partial_k, G,
                            K, h2);
         SLQ_CHECK(hipGetLastError());
