@@ -1,5 +1,6 @@
 """GPU tests over the grid of imate's own SLQ tests (tests/test_logdet.py:
-142-191 and tests/test_traceinv.py:260-330): the 50 x 50 bidiagonal Toeplitz
+142-191, tests/test_traceinv.py:260-330, tests/test_trace.py:126-170): the
+50 x 50 bidiagonal Toeplitz
 matrix a = 2, b = 1 (its Gramian when gram is False), float32 and float64,
 sparse and dense, gram True and False, exponent p in {0, 1, 2}; SLQ with
 orthogonalize=-1, lanczos_degree=30, 100 to 200 samples and error_rtol=1e-2
@@ -21,7 +22,7 @@ import pytest
 
 import imate_amd
 from imate_amd.sample_matrices import (toeplitz, toeplitz_logdet,
-                                       toeplitz_traceinv)
+                                       toeplitz_trace, toeplitz_traceinv)
 from oracle import oracle
 
 pytestmark = pytest.mark.gpu
@@ -45,6 +46,10 @@ def _exact(func, K, gram, p):
             ld = toeplitz_logdet(A_, B_, SIZE, gram)
             return ld if gram else 2.0 * ld
         return p * float(numpy.sum(numpy.log(lam)))
+    if func == 'identity':
+        if p == 1 and not gram:
+            return toeplitz_trace(A_, B_, SIZE, True)
+        return float(numpy.sum(lam**float(p)))
     if p == 1 and not gram:
         return toeplitz_traceinv(A_, B_, SIZE, True)
     return float(numpy.sum(lam**(-float(p))))
@@ -65,12 +70,16 @@ def _sensitivity(K, func, gram, p, want):
     return float(numpy.max(numpy.abs(b[ok] - want[ok]) / numpy.abs(want[ok])))
 
 
-@pytest.mark.parametrize('func', ['log', 'inv'])
+FUNCS = {'log': imate_amd.logdet, 'inv': imate_amd.traceinv,
+         'identity': imate_amd.trace}
+
+
+@pytest.mark.parametrize('func', ['log', 'inv', 'identity'])
 @pytest.mark.parametrize('dtype,gram,sparse,p', GRID)
 def test_reference_grid(func, dtype, gram, sparse, p):
     K = toeplitz(A_, B_, SIZE, gram=(not gram), dtype=dtype)
     Kin = K if sparse else K.toarray()
-    fn = imate_amd.logdet if func == 'log' else imate_amd.traceinv
+    fn = FUNCS[func]
     val, info = fn(Kin, method='slq', gpu=True, gram=gram, p=p,
                    return_info=True, **OPTS)
     ref = oracle.slq(K.tocsr(), func=func, gram=gram, exponent=p, **OPTS)
