@@ -1493,18 +1493,23 @@ __global__ __launch_bounds__(kThreads) void gs_sub_kernel(GsArgs a, T* __restric
 // basis: per row r' = r - sum_k h1_k v_k (k order, all K <= C vectors held
 // in registers), stored, and partial_k[g][k] = sum over the rows of
 // workgroup g of v_k r'. Per row this is the arithmetic of gs_sub_kernel
-// followed by gs_dots_kernel, with one read of the basis instead of two.
-// C <= 32: at C = 32 the values, the sums and the addresses take 252 VGPRs
-// (64 FP32 vectors would spill).
-template <typename T, int C>
-__global__ __launch_bounds__(kThreads) void gs_subdots_kernel(GsArgs a, T* __restrict__ R,
-                                                              const T* __restrict__ basis)
+// followed by gs_dots_kernel (for K > kGsChunk without the rounding of r to
+// T between chunks), with one read of the basis instead of two.
+// C <= 32 runs in 512-thread workgroups (two wavefronts per SIMD: at C =
+// 32 the values, the sums and the addresses take 252 VGPRs); C = 64 in
+// 256-thread workgroups, one wavefront per SIMD with up to 512 VGPRs.
+template <typename T, int C, int WPG>
+__global__ __launch_bounds__(WPG * kWave) void gs_subdots_kernel(GsArgs a, T* __restrict__ R,
+                                                                 const T* __restrict__ basis)
 {
     __shared__ double hs[C][kWave];
-    __shared__ double red[kWavesPerWG][8][kWave];
+    __shared__ double red[WPG][8][kWave];
     const int lane = threadIdx.x % kWave;
     const int wave = wave_id();
-    const Sweep sw = sweep_linear(a.nrows, wave);
+    Sweep sw;
+    sw.first = (int64_t) blockIdx.x * WPG + wave;
+    sw.stride = (int64_t) gridDim.x * WPG;
+    sw.end = a.nrows;
     const int64_t stride = a.slot;
     const int kn = a.K;
     for (int i = threadIdx.x; i < C * kWave; i += blockDim.x)
@@ -1541,13 +1546,14 @@ __global__ __launch_bounds__(kThreads) void gs_subdots_kernel(GsArgs a, T* __res
 #pragma unroll
         for (int g = 0; g < 8; ++g) red[wave][g][lane] = acc[g0 + g];
         __syncthreads();
-        if (wave < 8 && g0 + wave < kn)
-        {
-            double t = 0.0;
+        for (int q = wave; q < 8; q += WPG)
+            if (g0 + q < kn)
+            {
+                double t = 0.0;
 #pragma unroll
-            for (int w = 0; w < kWavesPerWG; ++w) t += red[w][wave][lane];
-            a.partial_k[((int64_t) blockIdx.x * a.K + g0 + wave) * kWave + lane] = t;
-        }
+                for (int w = 0; w < WPG; ++w) t += red[w][q][lane];
+                a.partial_k[((int64_t) blockIdx.x * a.K + g0 + q) * kWave + lane] = t;
+            }
         __syncthreads();
     }
 }
@@ -2093,18 +2099,21 @@ int reorthogonalize(slq_op* op, int64_t nrows, T* R, const T* basis, size_t slot
                            K, h1);
         g.h = h1;
         // r -= V h1 and h2 = V^T r in one pass while the K vectors of a row
-        // fit in registers without spilling (K <= 32), else two passes
-        if (K <= 32 && fused_gs_enabled())
+        // fit in registers without spilling (K <= 64), else two passes
+        if (K <= 64 && fused_gs_enabled())
         {
-            if (K > 16)
-                hipLaunchKernelGGL((gs_subdots_kernel<T, 32>), dim3(G), dim3(kThreads), 0,
+            if (K > 32)
+                hipLaunchKernelGGL((gs_subdots_kernel<T, 64, 4>), dim3(G), dim3(4 * kWave), 0,
                                    op->stream, g, R, basis);
+            else if (K > 16)
+                hipLaunchKernelGGL((gs_subdots_kernel<T, 32, kWavesPerWG>), dim3(G),
+                                   dim3(kThreads), 0, op->stream, g, R, basis);
             else if (K > 8)
-                hipLaunchKernelGGL((gs_subdots_kernel<T, 16>), dim3(G), dim3(kThreads), 0,
-                                   op->stream, g, R, basis);
+                hipLaunchKernelGGL((gs_subdots_kernel<T, 16, kWavesPerWG>), dim3(G),
+                                   dim3(kThreads), 0, op->stream, g, R, basis);
             else
-                hipLaunchKernelGGL((gs_subdots_kernel<T, 8>), dim3(G), dim3(kThreads), 0,
-                                   op->stream, g, R, basis);
+                hipLaunchKernelGGL((gs_subdots_kernel<T, 8, kWavesPerWG>), dim3(G),
+                                   dim3(kThreads), 0, op->stream, g, R, basis);
         }
         else
         {
