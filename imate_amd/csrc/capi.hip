@@ -277,34 +277,42 @@ bool union_pattern(int64_t nrows, const std::vector<int64_t>& ipa,
 // the host from device copies of both patterns. Leaves op->ab.tiles.R = 0
 // (the two-stream CSR SpMM) when A has no tiled form, a pattern repeats a
 // column or the union is too sparse for tiles.
-int ensure_pair_tiles(slq_op* op, bool need_f32)
+int ensure_pair_tiles(slq_op* op, bool need_f32, bool trans)
 {
-    PairTiles& ab = op->ab;
+    PairTiles& ab = trans ? op->ab_t : op->ab;
     if (!ab.tried)
     {
-        ab.tried = true;
         const MatrixStore& A = *op->A;
         const MatrixStore& B = *op->B;
+        if (trans && !(A.has_trans && B.has_trans))
+        {
+            set_error("internal: the pair tiles of the transposes need both transposes");
+            return SLQ_E_ARG;
+        }
+        ab.tried = true;
         const int R = A.tiles.R;
         if (R == 0 || A.dense || B.dense) return SLQ_OK;
-        const int64_t n = A.csr.nrows;
+        const DevCSR& ca = trans ? A.trans : A.csr;
+        const DevCSR& cb = trans ? B.trans : B.csr;
+        const int64_t n = ca.nrows;
+        SLQ_CHECK(hipStreamSynchronize(op->stream));   // transposes built on it
         std::vector<int64_t> ipa(n + 1), ipb(n + 1), ipu;
-        std::vector<int32_t> ixa(A.csr.nnz > 0 ? A.csr.nnz : 1),
-            ixb(B.csr.nnz > 0 ? B.csr.nnz : 1), ixu, ka, kb;
-        SLQ_CHECK(hipMemcpy(ipa.data(), A.csr.indptr, sizeof(int64_t) * (n + 1),
+        std::vector<int32_t> ixa(ca.nnz > 0 ? ca.nnz : 1), ixb(cb.nnz > 0 ? cb.nnz : 1), ixu,
+            ka, kb;
+        SLQ_CHECK(hipMemcpy(ipa.data(), ca.indptr, sizeof(int64_t) * (n + 1),
                             hipMemcpyDeviceToHost));
-        SLQ_CHECK(hipMemcpy(ipb.data(), B.csr.indptr, sizeof(int64_t) * (n + 1),
+        SLQ_CHECK(hipMemcpy(ipb.data(), cb.indptr, sizeof(int64_t) * (n + 1),
                             hipMemcpyDeviceToHost));
-        if (A.csr.nnz > 0)
-            SLQ_CHECK(hipMemcpy(ixa.data(), A.csr.indices, sizeof(int32_t) * A.csr.nnz,
+        if (ca.nnz > 0)
+            SLQ_CHECK(hipMemcpy(ixa.data(), ca.indices, sizeof(int32_t) * ca.nnz,
                                 hipMemcpyDeviceToHost));
-        if (B.csr.nnz > 0)
-            SLQ_CHECK(hipMemcpy(ixb.data(), B.csr.indices, sizeof(int32_t) * B.csr.nnz,
+        if (cb.nnz > 0)
+            SLQ_CHECK(hipMemcpy(ixb.data(), cb.indices, sizeof(int32_t) * cb.nnz,
                                 hipMemcpyDeviceToHost));
         if (!union_pattern(n, ipa, ixa, ipb, ixb, ipu, ixu, ka, kb)) return SLQ_OK;
         std::vector<int32_t> src;
         DevTiles t;
-        SLQ_TRY(build_tiles(n, n, ipu.data(), ixu.data(), nullptr, A.csr.dtype, R, 0.25, t,
+        SLQ_TRY(build_tiles(n, ca.ncols, ipu.data(), ixu.data(), nullptr, ca.dtype, R, 0.25, t,
                             &src));
         if (t.R == 0) return SLQ_OK;
         if (src.empty())
@@ -991,8 +999,11 @@ int prepare(slq_op* op, int gram, int precision)
             op->B->has_trans = true;
         }
     }
-    else if (op->kind == OP_AFFINE_B)
-        SLQ_TRY(ensure_pair_tiles(op, precision == SLQ_PREC_MIXED));
+    if (op->kind == OP_AFFINE_B)
+    {
+        SLQ_TRY(ensure_pair_tiles(op, precision == SLQ_PREC_MIXED, false));
+        if (gram) SLQ_TRY(ensure_pair_tiles(op, precision == SLQ_PREC_MIXED, true));
+    }
     if (precision == SLQ_PREC_MIXED)
     {
         SLQ_TRY(ensure_f32(op->stream, *op->A, gram != 0));

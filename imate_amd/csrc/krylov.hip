@@ -1849,51 +1849,53 @@ bool pair_tiles_enabled()
     return !(e && std::atoi(e) == 0);
 }
 
-// Tiled form of A + tB at t for SpMMs of value type T, its values gathered
-// on the operator's stream; *out = nullptr keeps the two-stream CSR SpMM.
+// Tiled form of A + tB (with `trans`, of A^T + tB^T) at t for SpMMs of
+// value type T, its values gathered on the operator's stream; *out =
+// nullptr keeps the two-stream CSR SpMM.
 template <typename T>
-int pair_tiles_at(slq_op* op, double t, const DevTiles** out)
+int pair_tiles_at(slq_op* op, double t, const DevTiles** out, bool trans = false)
 {
     *out = nullptr;
-    PairTiles& ab = op->ab;
+    PairTiles& ab = trans ? op->ab_t : op->ab;
     if (op->kind != OP_AFFINE_B || ab.tiles.R == 0 || !pair_tiles_enabled()) return SLQ_OK;
     const bool a64 = op->A->csr.dtype == SLQ_F64;
+    const void* da = trans ? op->A->trans.data : op->A->csr.data;
+    const void* db = trans ? op->B->trans.data : op->B->csr.data;
     if (sizeof(T) == 4 && a64 && !ab.tiles.vals_f32) return SLQ_OK;
     const int64_t count = ab.tiles.nslabs * ab.tiles.R;
     const int64_t g = (count + 255) / 256;
     const dim3 grid((unsigned) (g < 16384 ? g : 16384));
     if (a64 && sizeof(T) == 4)
         hipLaunchKernelGGL((pair_values_kernel<double, float>), grid, dim3(256), 0, op->stream,
-                           ab.src_a, ab.src_b, (const double*) op->A->csr.data,
-                           (const double*) op->B->csr.data, t, ab.tiles.vals_f32, count);
+                           ab.src_a, ab.src_b, (const double*) da, (const double*) db, t,
+                           ab.tiles.vals_f32, count);
     else if (a64)
         hipLaunchKernelGGL((pair_values_kernel<double, double>), grid, dim3(256), 0,
-                           op->stream, ab.src_a, ab.src_b, (const double*) op->A->csr.data,
-                           (const double*) op->B->csr.data, t, (double*) ab.tiles.vals,
-                           count);
+                           op->stream, ab.src_a, ab.src_b, (const double*) da,
+                           (const double*) db, t, (double*) ab.tiles.vals, count);
     else
         hipLaunchKernelGGL((pair_values_kernel<float, float>), grid, dim3(256), 0, op->stream,
-                           ab.src_a, ab.src_b, (const float*) op->A->csr.data,
-                           (const float*) op->B->csr.data, (float) t,
-                           (float*) ab.tiles.vals, count);
+                           ab.src_a, ab.src_b, (const float*) da, (const float*) db,
+                           (float) t, (float*) ab.tiles.vals, count);
     SLQ_CHECK(hipGetLastError());
     *out = &ab.tiles;
     return SLQ_OK;
 }
 
-// Operand tiles of a non-transposed SpMM at t: the pair tiles for A + tB,
-// else tiles_for. *shift_mode is the epilogue's shift (none when the pair
-// tiles already hold A + tB).
+// Operand tiles of an SpMM at t (of the transpose with `trans`): the pair
+// tiles for A + tB, else tiles_for. *shift_mode is the epilogue's shift
+// (none when the pair tiles already hold A + tB).
 template <typename T>
-int operand_tiles(slq_op* op, double t, const DevTiles** out, int* shift_mode)
+int operand_tiles(slq_op* op, double t, const DevTiles** out, int* shift_mode,
+                  bool trans = false)
 {
     *shift_mode = op->kind == OP_AFFINE_I ? 1 : (op->kind == OP_AFFINE_B ? 2 : 0);
     if (op->kind != OP_AFFINE_B)
     {
-        *out = tiles_for<T>(op, false);
+        *out = tiles_for<T>(op, trans);
         return SLQ_OK;
     }
-    SLQ_TRY(pair_tiles_at<T>(op, t, out));
+    SLQ_TRY(pair_tiles_at<T>(op, t, out, trans));
     if (*out) *shift_mode = 0;
     return SLQ_OK;
 }
@@ -2382,8 +2384,11 @@ int golub_kahn(const RecurrenceArgs& ra, Bump& bump)
     const double tol_sqrt_n = ra.tol * std::sqrt((double) nc);
     const int64_t shift_len = nr < nc ? nr : nc;
     const int mode = op->kind == OP_AFFINE_I ? 1 : (op->kind == OP_AFFINE_B ? 2 : 0);
-    const DevTiles* tla = tiles_for<T>(op, false);
-    const DevTiles* tlt = tiles_for<T>(op, true);
+    const DevTiles* tla = nullptr;
+    const DevTiles* tlt = nullptr;
+    int mode_a = mode, mode_t = mode;
+    SLQ_TRY(operand_tiles<T>(op, ra.t, &tla, &mode_a, false));
+    SLQ_TRY(operand_tiles<T>(op, ra.t, &tlt, &mode_t, true));
 
     for (int j = 0; j < m; ++j)
     {
@@ -2391,7 +2396,7 @@ int golub_kahn(const RecurrenceArgs& ra, Bump& bump)
         // (c_golub_kahn_bidiagonalization.cpp:162-195)
         SpmmArgs<T> s = spmm_defaults<T>();
         fill_matrix<T>(s, A, Bm, false, f32);
-        s.shift_mode = mode;
+        s.shift_mode = mode_a;
         s.shift_len = shift_len;
         s.t = ra.t;
         s.xscale = sv + (size_t) j * kBlock;
@@ -2426,7 +2431,7 @@ int golub_kahn(const RecurrenceArgs& ra, Bump& bump)
         // (c_golub_kahn_bidiagonalization.cpp:197-228)
         SpmmArgs<T> t = spmm_defaults<T>();
         fill_matrix<T>(t, A, Bm, true, f32);
-        t.shift_mode = mode;
+        t.shift_mode = mode_t;
         t.shift_len = shift_len;
         t.t = ra.t;
         t.xscale = su + (size_t) j * kBlock;
@@ -2526,8 +2531,8 @@ int run_spmm(slq_op* op, double t, int transpose, const void* X, void* Y,
     MatrixStore* Bm = op->kind == OP_AFFINE_B ? op->B.get() : nullptr;
     const int64_t shift_len = A.csr.nrows < A.csr.ncols ? A.csr.nrows : A.csr.ncols;
     const int mode = op->kind == OP_AFFINE_I ? 1 : (op->kind == OP_AFFINE_B ? 2 : 0);
-    if (op->kind == OP_AFFINE_B && !transpose)
-        SLQ_TRY(ensure_pair_tiles(op, value_dtype == SLQ_F32));
+    if (op->kind == OP_AFFINE_B)
+        SLQ_TRY(ensure_pair_tiles(op, value_dtype == SLQ_F32, transpose != 0));
     // ticket counters of the in-order tile schedule (no reduction here)
     unsigned int* tickets = nullptr;
     SLQ_CHECK(hipMalloc(&tickets, sizeof(unsigned int) * kTicketWords));
@@ -2555,8 +2560,8 @@ int run_spmm(slq_op* op, double t, int transpose, const void* X, void* Y,
         SpmmIO<float> io;
         io.X = (const float*) X;
         io.Y = (float*) Y;
-        const DevTiles* tl = tiles_for<float>(op, transpose != 0);
-        r = transpose ? SLQ_OK : operand_tiles<float>(op, t, &tl, &s.shift_mode);
+        const DevTiles* tl = nullptr;
+        r = operand_tiles<float>(op, t, &tl, &s.shift_mode, transpose != 0);
         if (r == SLQ_OK) r = launch_spmm<float>(op, s, io, tl);
     }
     else
@@ -2571,8 +2576,8 @@ int run_spmm(slq_op* op, double t, int transpose, const void* X, void* Y,
         SpmmIO<double> io;
         io.X = (const double*) X;
         io.Y = (double*) Y;
-        const DevTiles* tl = tiles_for<double>(op, transpose != 0);
-        r = transpose ? SLQ_OK : operand_tiles<double>(op, t, &tl, &s.shift_mode);
+        const DevTiles* tl = nullptr;
+        r = operand_tiles<double>(op, t, &tl, &s.shift_mode, transpose != 0);
         if (r == SLQ_OK) r = launch_spmm<double>(op, s, io, tl);
     }
     const hipError_t e = hipStreamSynchronize(op->stream);

@@ -178,7 +178,8 @@ struct slq_op {
     int device = 0;
     slq::Workspace ws;
     slq::ProfileState prof;
-    slq::PairTiles ab;                      // OP_AFFINE_B only
+    slq::PairTiles ab;                      // OP_AFFINE_B only: A + tB
+    slq::PairTiles ab_t;                    // and A^T + tB^T (Golub-Kahn)
     // launch-grid caps chosen by the autotuner (0 = library default)
     int spmm_grid = 0;
     int stream_grid = 0;
@@ -230,9 +231,10 @@ int run_spmm(slq_op* op, double t, int transpose, const void* X, void* Y,
 // Transpose of a device CSR (stable: rows ascending within a column).
 int build_transpose(hipStream_t s, const DevCSR& A, DevCSR& T);
 int ensure_f32(hipStream_t s, MatrixStore& M, bool need_trans);
-// Builds op->ab for an A + tB operator (once; R = 0 if not profitable) and
-// the FP32 value buffer when `need_f32`.
-int ensure_pair_tiles(slq_op* op, bool need_f32);
+// Builds op->ab (or, with `trans`, op->ab_t from the CSR transposes of A and
+// B, which must exist) for an A + tB operator (once; R = 0 if not
+// profitable) and the FP32 value buffer when `need_f32`.
+int ensure_pair_tiles(slq_op* op, bool need_f32, bool trans = false);
 // Transposed copy of a dense matrix (for Golub-Kahn).
 int dense_transpose(hipStream_t s, MatrixStore& M);
 

@@ -106,8 +106,9 @@ int slq_op_create_dense(int64_t nrows, int64_t ncols, const void* data, int dtyp
  * which the Ritz values of A + tI follow from one Lanczos run by a shift).
  * The new handle references A's (and B's) device arrays; destroy it before
  * them. With a general B (CSR, same shape and dtype) the first estimate tiles
- * the union of both patterns and each t gathers A + tB into those tiles on
- * the device (SLQ_PAIR_TILES=0 keeps two CSR products, A x + t B x). Replaces pycuAffineMatrixFunction / cuCSRAffineMatrixFunction
+ * the union of both patterns (for gram = 1 also of their transposes) and
+ * each t gathers A + tB into those tiles on the device (SLQ_PAIR_TILES=0
+ * keeps two CSR products, A x + t B x). Replaces pycuAffineMatrixFunction / cuCSRAffineMatrixFunction
  * (imate/_cu_linear_operator/py_cu_affine_matrix_function.pyx:31,171;
  * cu_csr_affine_matrix_function.cu:29-211). */
 int slq_op_create_affine(slq_op* A, slq_op* B, slq_op** out);

@@ -84,6 +84,13 @@ def test_pair_spmm_matches_numpy(gpu_lib, monkeypatch, which_b, dtype):
             X.astype(numpy.float64)
         assert numpy.max(numpy.abs(csr - ref)) <= tol * numpy.abs(ref).max()
         assert numpy.max(numpy.abs(pair - ref)) <= tol * numpy.abs(ref).max()
+        # the transposed product runs on the union tiles of A^T and B^T
+        Z = numpy.empty_like(X)
+        _lib.check(gpu_lib.slq_op_spmm(hab, 0.37, 64, 1, _lib.ptr(X),
+                                       _lib.ptr(Z)), 'slq_op_spmm')
+        ref = (A.astype(numpy.float64) + 0.37 * B.astype(numpy.float64)).T @ \
+            X.astype(numpy.float64)
+        assert numpy.max(numpy.abs(Z - ref)) <= tol * numpy.abs(ref).max()
     finally:
         gpu_lib.slq_op_destroy(hab)
         gpu_lib.slq_op_destroy(hb)
@@ -210,10 +217,10 @@ def test_pair_mixed_precision_close_to_fp64():
     assert numpy.all(numpy.abs(a - b) <= 1e-5 * numpy.abs(a))
 
 
-def test_gram_general_b_keeps_csr_streams():
-    """Golub-Kahn on A + tB with a general B runs the two CSR streams and
-    their transposes (no union tiles of the transposes); per sample against
-    the oracle."""
+def test_gram_general_b_sparse_union_keeps_csr_streams():
+    """Golub-Kahn on A + tB where the union is too sparse for tiles (a
+    bidiagonal A): the two CSR streams and their transposes; per sample
+    against the oracle."""
     from imate_amd.sample_matrices import toeplitz
     A = toeplitz(2.0, 1.0, 60, gram=False)
     B = scipy.sparse.diags(numpy.linspace(0.5, 1.5, 60)).tocsr()
@@ -229,3 +236,29 @@ def test_gram_general_b_keeps_csr_streams():
     got = numpy.asarray(info['convergence']['samples'])
     assert numpy.max(numpy.abs(got - ref['samples']) /
                      numpy.abs(ref['samples'])) < 1e-10
+
+
+def test_gram_general_b_on_union_tiles(monkeypatch):
+    """Golub-Kahn on A + tB: u = (A + tB) v on the union tiles of A and B,
+    v = (A + tB)^T u on those of A^T and B^T; per sample against the oracle,
+    and against the two CSR streams (equal to 1e-10, not bit for bit)."""
+    A = sparse_correlation_matrix(2048, nnz_per_row=30, seed=3)
+    B = _spd_other_pattern(2048, 30, 3, 14)
+    t = [0.1, 1.5]
+    kw = dict(parameters=t, gram=True, method='slq', gpu=True,
+              lanczos_degree=16, min_num_samples=32, max_num_samples=32,
+              seed=9, return_info=True)
+    Aop = imate_amd.AffineMatrixFunction(A, B)
+    _, info = imate_amd.logdet(Aop, **kw)
+    ref = oracle.slq(A, func='log', gram=True, parameters=t, B=B, affine=True,
+                     lanczos_degree=16, min_num_samples=32,
+                     max_num_samples=32, seed=9)
+    got = numpy.asarray(info['convergence']['samples'])
+    assert numpy.max(numpy.abs(got - ref['samples']) /
+                     numpy.abs(ref['samples'])) < 1e-10
+    monkeypatch.setenv('SLQ_PAIR_TILES', '0')
+    _, csr = imate_amd.logdet(Aop, **kw)
+    monkeypatch.delenv('SLQ_PAIR_TILES')
+    c = numpy.asarray(csr['convergence']['samples'])
+    assert numpy.max(numpy.abs(got - c) / numpy.abs(c)) < 1e-10
+    assert not numpy.array_equal(got, c)
