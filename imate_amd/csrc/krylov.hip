@@ -938,9 +938,11 @@ __global__ __launch_bounds__(kThreads) void mfma_spmm_kernel(
 // rows of ld values (ld a multiple of 64, zero padding). A workgroup of 8
 // wavefronts takes 128 rows (16 per wavefront, one MFMA row block) and
 // sweeps the columns in chunks of 64: the 64 scaled operand rows of a chunk
-// are staged in LDS once for the workgroup (double-buffered, the next chunk
-// loaded while the current one is multiplied), and each wavefront streams
-// its 16 x 64 slice of A from HBM, four consecutive values per lane. MFMA
+// are staged in LDS once for the workgroup (double-buffered), and each
+// wavefront streams its 16 x 64 slice of A from HBM, four consecutive values
+// per lane; the next chunk's A slice and operand rows are loaded into
+// registers before the current chunk's MFMAs and the rows written to LDS
+// after them, so both loads overlap the matrix-core work. MFMA
 // group (c, g) gives k-slot kq to column 16 c + 4 kq + g of the chunk (A
 // and B alike, so the product is unchanged); the four 16-probe blocks hold
 // probes 4 i + q as in mfma_spmm_kernel, whose epilogue it shares. At 64
@@ -1020,26 +1022,30 @@ __global__ __launch_bounds__(kThreads) void dense_spmm_kernel(SpmmArgs<T> a,
     const int64_t nchunk = (a.ncols + kDenseKC - 1) / kDenseKC;
     const int64_t cps = (nchunk + ksplit - 1) / ksplit;
     const int64_t rblocks = (a.nrows + 127) / 128;
-    auto stage = [&](int64_t ci, int buf) {
+    // operand rows of chunk ci: loaded into registers (xload), written to
+    // LDS later (xstore), so that the loads are in flight during the
+    // current chunk's MFMAs
+    auto xload = [&](int64_t ci, T (&v)[8]) {
         const int64_t k = ci * kDenseKC + kk;
-        T v[8];
         if (k < a.ncols)
         {
             T w[4];
             load_row4(X + k * kBlock + pp, w);
 #pragma unroll
-            for (int u = 0; u < 4; ++u) v[u] = ssc[u] * w[u];
+            for (int u = 0; u < 4; ++u) v[u] = w[u];
             load_row4(X + k * kBlock + pp + 4, w);
 #pragma unroll
-            for (int u = 0; u < 4; ++u) v[4 + u] = ssc[4 + u] * w[u];
+            for (int u = 0; u < 4; ++u) v[4 + u] = w[u];
         }
         else
         {
 #pragma unroll
             for (int u = 0; u < 8; ++u) v[u] = T(0);
         }
+    };
+    auto xstore = [&](const T (&v)[8], int buf) {
 #pragma unroll
-        for (int u = 0; u < 8; ++u) xs[buf][kk][pp + u] = v[u];
+        for (int u = 0; u < 8; ++u) xs[buf][kk][pp + u] = ssc[u] * v[u];
     };
     double red[4] = {0.0, 0.0, 0.0, 0.0};
     for (int64_t it = blockIdx.x; it < rblocks * ksplit; it += gridDim.x)
@@ -1054,15 +1060,31 @@ __global__ __launch_bounds__(kThreads) void dense_spmm_kernel(SpmmArgs<T> a,
         V4 acc[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q) acc[q] = V4{T(0), T(0), T(0), T(0)};
-        if (c0 < c1) stage(c0, 0);
+        T av[4][4];
+        if (c0 < c1)
+        {
+            T xv[8];
+            xload(c0, xv);
+#pragma unroll
+            for (int c = 0; c < 4; ++c) load_row4(arow + c0 * kDenseKC + 16 * c, av[c]);
+            xstore(xv, 0);
+        }
         __syncthreads();
         for (int64_t ci = c0; ci < c1; ++ci)
         {
             const int buf = (int) ((ci - c0) & 1);
-            T av[4][4];
+            const bool more = ci + 1 < c1;
+            // next chunk's A slice and operand rows, in flight during the
+            // MFMAs below
+            T an[4][4];
+            T xv[8];
+            if (more)
+            {
 #pragma unroll
-            for (int c = 0; c < 4; ++c) load_row4(arow + ci * kDenseKC + 16 * c, av[c]);
-            if (ci + 1 < c1) stage(ci + 1, buf ^ 1);
+                for (int c = 0; c < 4; ++c)
+                    load_row4(arow + (ci + 1) * kDenseKC + 16 * c, an[c]);
+                xload(ci + 1, xv);
+            }
 #pragma unroll
             for (int c = 0; c < 4; ++c)
             {
@@ -1075,6 +1097,14 @@ __global__ __launch_bounds__(kThreads) void dense_spmm_kernel(SpmmArgs<T> a,
                     for (int q = 0; q < 4; ++q)
                         acc[q] = Mfma16<T>::op(av[c][g], bv[q], acc[q]);
                 }
+            }
+            if (more)
+            {
+                xstore(xv, buf ^ 1);
+#pragma unroll
+                for (int c = 0; c < 4; ++c)
+#pragma unroll
+                    for (int g = 0; g < 4; ++g) av[c][g] = an[c][g];
             }
             __syncthreads();
         }
