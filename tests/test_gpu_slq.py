@@ -282,3 +282,33 @@ def test_cholesky_accuracy_c1():
                                 seed=0, return_info=True)
     assert abs(ld - exact) <= 0.01 * abs(exact)
     assert abs(ld - exact) <= 3 * info['error']['absolute_error']
+
+
+@pytest.mark.parametrize('m', [50, 70])
+def test_full_reorthogonalization_long_basis(m):
+    """Full re-orthogonalisation past 32 stored vectors: steps with K <= 64
+    take the fused subtraction + product pass (64-vector form above 32), and
+    m = 70 also runs the two-pass form beyond 64."""
+    val, info, ref = _both(c1(), m=m, s=24, ortho=-1)
+    _assert_samples(info, ref, 1e-10)
+
+
+def test_gram_full_reorthogonalization_long_basis():
+    A = _rect()
+    val, info, ref = _both(A, gram=True, m=45, s=24, ortho=-1)
+    _assert_samples(info, ref, 1e-10)
+
+
+def test_fused_gram_schmidt_matches_two_passes(monkeypatch):
+    kw = dict(method='slq', gpu=True, lanczos_degree=60, min_num_samples=64,
+              max_num_samples=64, seed=6, orthogonalize=-1, return_info=True)
+    _, fused = imate_amd.logdet(c1(), **kw)
+    monkeypatch.setenv('SLQ_FUSED_GS', '0')
+    _, plain = imate_amd.logdet(c1(), **kw)
+    monkeypatch.delenv('SLQ_FUSED_GS')
+    a = numpy.asarray(fused['convergence']['samples'])
+    b = numpy.asarray(plain['convergence']['samples'])
+    assert numpy.max(numpy.abs(a - b) / numpy.abs(b)) < 1e-10
+    _, mixed = imate_amd.logdet(c1(), precision='mixed', **kw)
+    c = numpy.asarray(mixed['convergence']['samples'])
+    assert abs(c.mean() - b.mean()) <= 1e-5 * abs(b.mean())
