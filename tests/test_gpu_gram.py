@@ -139,3 +139,20 @@ def test_gram_blocked_transpose_matches_oracle(gpu_lib):
     got = info['convergence']['samples']
     want = ref['samples'][:, 0]
     assert numpy.max(numpy.abs(got - want) / numpy.abs(want)) < 1e-10
+
+
+@pytest.mark.parametrize('nrows', [2051, 20001])
+def test_blocked_spmm_ragged_rows(gpu_lib, nrows):
+    """The column-blocked kernel with a row count that is not a multiple of
+    its eight-row groups, and (20001 rows) more groups than one round of
+    the grid, so the last round has empty groups that still pass every
+    block barrier."""
+    from imate_amd import _lib
+    D = sparse_gaussian_device(nrows, 70000, 0.004, seed=13)
+    A = D.to_csr()
+    X = numpy.random.default_rng(2).standard_normal((70000, 64))
+    Y = numpy.empty((nrows, 64))
+    _lib.check(gpu_lib.slq_op_spmm(D.gpu_Aop.handle, 0.0, 64, 0, _lib.ptr(X),
+                                   _lib.ptr(Y)), 'slq_op_spmm')
+    ref = A @ X
+    assert numpy.max(numpy.abs(Y - ref)) <= 1e-13 * numpy.abs(ref).max()
