@@ -2,7 +2,7 @@
 
 CPU restatement of imate's stochastic Lanczos quadrature (SLQ), used by the
 tests, by ``__graft_entry__.smoke()`` and by ``bench.py``'s ``cpu_baseline``
-leg, and only as the checker (or the timed CPU baseline). The product package
+leg, and only to check results (or as the timed CPU baseline). The product package
 ``imate_amd`` never imports this module.
 
 Split of the work:

@@ -4,7 +4,7 @@
  * CPU restatement of imate's stochastic Lanczos quadrature inner loops
  * (reference: imate 0.25.2, CPU path with USE_CBLAS=0, 32-bit indices).
  * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
- * load this library, and only as the checker or the timed CPU baseline; the
+ * load this library, and only to check results or as the timed CPU baseline; the
  * product library (imate_amd/csrc) never links or calls it.
  *
  * Contents and the reference lines they restate:

@@ -3,7 +3,7 @@ oracle on the same seeded probes (per-sample values first, then the
 convergence / outlier outputs), plus closed forms and size-independent
 properties at larger sizes.
 
-Tolerances: 1e-10 relative per sample for the FP64 forms (the north-star bar;
+Tolerances: 1e-10 relative per sample for the FP64 forms (the parity bar;
 GPU sums run in a different order than the reference's long-double sums);
 re-orthogonalised runs use classical Gram-Schmidt twice on the GPU against
 the reference's modified Gram-Schmidt, also held to 1e-10; FP32 matrices
