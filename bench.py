@@ -58,7 +58,7 @@ def parse():
                    choices=['plain', 'precision', 'batched', 'surrogate',
                             'gram', 'dense', 'general_b'],
                    help='plain: the headline workload (default); the others '
-                        'measure the north-star forms and print their own '
+                        'measure the other SLQ forms and print their own '
                         'JSON lines')
     return p.parse_args()
 

@@ -1,6 +1,6 @@
 """CPU tests of the interpolators' host side: every kind is fitted to exact
 Schatten norms (dense eigenvalues of small matrices, computed here as the
-checker) injected in place of the device estimates, and pinned by what the
+reference values) injected in place of the device estimates, and pinned by what the
 mathematics fixes: interpolation through the points, closed forms a kind
 reproduces exactly (MBF for p = 2; RPF for p = -1 with q + 1 distinct
 eigenvalues), orthonormality of the root-monomial bases, the bounds, and the
