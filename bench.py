@@ -614,7 +614,7 @@ def dense_form(args):
 def main():
     args = parse()
     if args.gpus != 1:
-        print('bench.py runs on one GPU only: this build has no multi-GPU '
+        print('bench.py runs on one GPU only: this package has no multi-GPU '
               'path (probes would shard across GPUs with one gather of the '
               'per-sample results; not built). Use --gpus 1.', flush=True)
         sys.exit(2)

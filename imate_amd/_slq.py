@@ -3,7 +3,7 @@
 Mirrors imate/logdet/_slq_method.pyx:24-44,820-854 (f = Logarithm),
 imate/traceinv/_slq_method.pyx:827 (f = Inverse) and
 imate/trace/_slq_method.pyx:825 (f = Identity): same keyword arguments and
-defaults; ``precision`` and ``block_size`` are options of this build whose
+defaults; ``precision`` and ``block_size`` are options of this package whose
 defaults keep the reference behaviour.
 """
 

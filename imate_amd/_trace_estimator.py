@@ -290,7 +290,7 @@ def trace_estimator(A, parameters, func, gram, exponent, min_num_samples,
     ``func`` is 'log', 'inv', 'identity' or a (name, coefficients) pair
     (see ``function_spec``). ``precision`` ('native' or
     'mixed') and ``block_size`` (probes per device block, <= 64) are options
-    of this build; their defaults reproduce the reference behaviour.
+    of this package; their defaults reproduce the reference behaviour.
     """
     if num_threads < 1:
         import os
@@ -322,7 +322,7 @@ def trace_estimator(A, parameters, func, gram, exponent, min_num_samples,
 
     if plot:
         raise NotImplementedError('plot=True (matplotlib convergence plot) '
-                                  'is not part of this build.')
+                                  'is not part of imate_amd.')
     if not gpu:
         raise NotImplementedError(
             'imate_amd implements the GPU path only: call with gpu=True '

@@ -1,4 +1,4 @@
-"""Batched SLQ over many independent matrices (this build's batched form).
+"""Batched SLQ over many independent matrices (the batched form of this package).
 
 imate estimates one operator per call; a user with many matrices loops over
 ``logdet(A_i, method='slq', gpu=True)``. ``slq_batch`` runs them together:

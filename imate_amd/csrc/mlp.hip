@@ -9,7 +9,7 @@
 // Loss: mean squared error over the batch.
 //
 // No reference counterpart: imate's own surrogates over t are the
-// interpolators (imate/interpolator), which this build does not include.
+// interpolators (imate/interpolator, restated in imate_amd/interpolator).
 
 #include <hip/hip_runtime.h>
 

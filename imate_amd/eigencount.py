@@ -4,7 +4,7 @@
 The number of eigenvalues of A^p in [a, b], tr 1_[a,b](A^p), by stochastic
 Lanczos quadrature with the indicator function (imate/functions/indicator.cpp:
 0 below a or above b, else 1; eigencount/_slq_method.pyx:25-877). Only
-``method='slq'`` is part of this build.
+``method='slq'`` is part of this package.
 """
 
 import numpy

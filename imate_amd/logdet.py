@@ -3,7 +3,7 @@
 logdet(A^p) = p log|det A| by stochastic Lanczos quadrature with
 f = log (imate/functions/logarithm.cpp:42-45); with ``gram=True`` the Gramian
 A^T A is used through Golub-Kahn bidiagonalization. Only ``method='slq'`` is
-part of this build.
+part of this package.
 """
 
 from ._slq import slq_method

@@ -5,7 +5,7 @@
   matrices (imate/sample_matrices/toeplitz.py:25-202,273-635): the upper
   bidiagonal matrix with a on the diagonal and b above it, or its Gramian.
 * ``sparse_correlation_matrix(...)`` — a seeded, spatially ordered sparse
-  correlation matrix (this build's construction of the benchmark workloads).
+  correlation matrix (the construction of the benchmark workloads).
   The reference's own ``correlation_matrix`` is time-seeded with an O(n^2)
   pair loop; this one is seeded, O(n k) with a KD-tree, and positive definite
   by construction:

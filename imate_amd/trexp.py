@@ -3,7 +3,7 @@
 
 tr exp(c A^p) by stochastic Lanczos quadrature with f(x) = exp(c x)
 (imate/functions/exponential.cpp; trexp/_slq_method.pyx:24-852). Only
-``method='slq'`` is part of this build.
+``method='slq'`` is part of this package.
 """
 
 from ._slq import slq_method

@@ -4,7 +4,7 @@
 tr of the linear fractional transformation (a A^p + b I)(c A^p + d I)^-1 by
 stochastic Lanczos quadrature with f(x) = (a x + b) / (c x + d)
 (imate/functions/homographic.cpp; trlinfrac/_slq_method.pyx:25-880). Only
-``method='slq'`` is part of this build.
+``method='slq'`` is part of this package.
 """
 
 import numpy

@@ -217,7 +217,7 @@ int slq_autotune(slq_op* op, int32_t precision, int32_t* spmm_grid,
  * of slq_mlp_num_params() values: W1[128][2], b1[128], W2[128][128],
  * b2[128], W3[128], b3. Inputs x[B][2], targets t[B]; loss = mean squared
  * error. imate's own surrogates over t are its interpolators
- * (imate/interpolator), which this build does not include. */
+ * (imate/interpolator, restated on the host in imate_amd/interpolator). */
 int slq_mlp_num_params(void);
 /* Loss and gradient of all parameters at `params` (one forward + backward). */
 int slq_mlp_grad(const float* params, const float* x, const float* target,
