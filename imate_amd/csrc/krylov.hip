@@ -376,18 +376,19 @@ __global__ __launch_bounds__(kThreads) void spmm_kernel(SpmmArgs<T> a,
 // than the L2 arrive at about half the rate of rows drawn from an
 // L2-resident table (measured: 14-15 vs 27.7 G rows/s for 512 B rows). A
 // wavefront owns kBlkRows rows and sweeps the columns in blocks of `cb`; the
-// waves of a workgroup pass each block together (a barrier per block), the
-// workgroups are as large as allowed (16 wavefronts, one per CU) and the
-// grid holds only as many as fit on the chip, so all of them start the
-// sweep together. The workgroups of an XCD still drift apart by a few
+// waves of a workgroup pass the blocks together (a barrier every `sync`
+// blocks), the workgroups are as large as allowed (16 wavefronts, one per
+// CU) and the grid holds only as many as fit on the chip, so all of them
+// start the sweep together. The workgroups of an XCD still drift apart by a few
 // blocks, so a block must take well under the 4 MB L2: on the 2^20 x 2^18
 // Golub-Kahn case (8-wavefront workgroups) the L2 hit rate was 92 % with
 // 256-column blocks and 13-14 % with 2048 or more; with 16-wavefront
 // workgroups 768 columns (384 KB of operand rows) is fastest, 1.64x the row
 // kernel. Each row keeps a window of 64 (column, value) pairs in registers
 // with a cursor; the pairs of one block are a prefix of what is left of the
-// window (columns ascend), found with a ballot. Per row the products are added in CSR order with the same
-// fma chain as row_gather, so the result equals spmm_kernel's bit for bit.
+// window (columns ascend), found with a ballot. Per row the products are
+// added in CSR order with the same fma chain as row_gather, so the result
+// equals spmm_kernel's bit for bit.
 constexpr int kBlkRows = 8;
 
 template <typename T, bool NT, int WPG>
@@ -396,7 +397,7 @@ __global__ __launch_bounds__(WPG * kWave) void csr_blocked_kernel(SpmmArgs<T> a,
                                                                   T* __restrict__ Y,
                                                                   T* __restrict__ Vout,
                                                                   const T* __restrict__ Sub,
-                                                                  int cb)
+                                                                  int cb, int sync)
 {
     const int lane = threadIdx.x % kWave;
     const int wave = wave_id();
@@ -481,7 +482,11 @@ __global__ __launch_bounds__(WPG * kWave) void csr_blocked_kernel(SpmmArgs<T> a,
                     load_window(k);
                 }
             }
-            __syncthreads();
+            // every `sync` blocks (b is the same in every wave): rows whose
+            // entries per block vary (Poisson counts in A^T) then wait for
+            // the workgroup's slowest wavefront once per `sync` blocks, and
+            // the waves stay within `sync` blocks of each other
+            if ((b + 1) % sync == 0 || b + 1 == nb) __syncthreads();
         }
 #pragma unroll
         for (int k = 0; k < kBlkRows; ++k)
@@ -2032,8 +2037,13 @@ void launch_spmm_nt(slq_op* op, const SpmmArgs<T>& s, const SpmmIO<T>& io, const
             return ((per_cu * cus) / kXcds) * kXcds;
         }();
         if (resident >= kXcds && grid > resident) grid = resident;
-        hipLaunchKernelGGL(kern, dim3(grid), dim3(threads), 0,
-                           op->stream, s, io.X, io.Y, io.Vout, io.Sub, csr_blocked_cols(s));
+        static const int sync = [] {
+            const char* e = std::getenv("SLQ_BLK_SYNC");
+            const int v = e ? std::atoi(e) : 4;
+            return v >= 1 ? v : 1;
+        }();
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(threads), 0, op->stream, s, io.X, io.Y,
+                           io.Vout, io.Sub, csr_blocked_cols(s), sync);
         if (s.red_mode)
             hipLaunchKernelGGL((total_kernel<T>), dim3(1), dim3(kTotalWaves * kWave), 0,
                                op->stream, s, grid);
