@@ -383,8 +383,8 @@ __global__ __launch_bounds__(kThreads) void spmm_kernel(SpmmArgs<T> a,
 // blocks, so a block must take well under the 4 MB L2: on the 2^20 x 2^18
 // Golub-Kahn case (8-wavefront workgroups) the L2 hit rate was 92 % with
 // 256-column blocks and 13-14 % with 2048 or more; with 16-wavefront
-// workgroups 768 columns (384 KB of operand rows) is fastest, 1.64x the row
-// kernel. Each row keeps a window of 64 (column, value) pairs in registers
+// workgroups and a barrier every four blocks 1024 columns (512 KB of
+// operand rows) is fastest, 1.7x the row kernel. Each row keeps a window of 64 (column, value) pairs in registers
 // with a cursor; the pairs of one block are a prefix of what is left of the
 // window (columns ascend), found with a ballot. Per row the products are
 // added in CSR order with the same fma chain as row_gather, so the result
@@ -1819,7 +1819,7 @@ int csr_blocked_cols(const SpmmArgs<T>& s)
 {
     static int cb = [] {
         const char* e = std::getenv("SLQ_CSR_BLOCK_COLS");
-        return e ? std::atoi(e) : 768;
+        return e ? std::atoi(e) : 1024;
     }();
     if (cb <= 0 || !s.sorted || s.shift_mode == 2 || s.nrows <= 0) return 0;
     if ((double) s.ncols * kBlock * sizeof(T) <= 16.0 * (1 << 20)) return 0;
