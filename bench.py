@@ -116,6 +116,24 @@ def cpu_baseline(A, m, threads, seed, per_thread=8):
     return steps / dt, dt, steps
 
 
+def cpu_calibration():
+    """The port's speed relative to imate's own CPU Lanczos loop (imate's
+    sources compiled into oracle/_ref), measured on the same cores in the
+    build container by oracle/calibrate_cpu_baseline.py."""
+    path = os.path.join(HERE, 'oracle', 'cpu_calibration.json')
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        c = json.load(f)
+    return {'port_over_imate_cpu': c['port_over_reference'],
+            'imate_cpu_steps_per_s': c['reference_steps_per_s'],
+            'port_steps_per_s': c['port_steps_per_s'],
+            'threads': c['threads'], 'host': c['host'],
+            'workload': 'n=%d, nnz=%d, lanczos_degree %d, %d probes per thread'
+                        % (c['n'], c['nnz'], c['lanczos_degree'],
+                           c['probes_per_thread'])}
+
+
 def precision_form(args):
     """Precision form: n = 65536 correlation matrix with kernel scale 0.01
     (~247 non-zeros per row, cond ~1e4), lanczos_degree 60, full
@@ -684,10 +702,14 @@ def main():
     _lib.check(lib.slq_profile_read(handle, ctypes.byref(spmm_ms),
                                     ctypes.byref(spmm_n), ctypes.byref(upd_ms),
                                     ctypes.byref(upd_n)), 'slq_profile_read')
+    done = ctypes.c_int64(0)
+    _lib.check(lib.slq_profile_steps(handle, ctypes.byref(done)),
+               'slq_profile_steps')
     lib.slq_profile_enable(handle, 0)
 
-    steps_per_call = s * m
-    value = steps_per_call * args.steps / elapsed
+    # completed Lanczos steps of the processed probes (a probe whose
+    # recurrence breaks down early counts only the steps it ran)
+    value = done.value / elapsed
     ms_per_step = elapsed / args.steps * 1e3
     avg_spmm = spmm_ms.value / max(spmm_n.value, 1)
     avg_upd = upd_ms.value / max(upd_n.value, 1)
@@ -716,6 +738,9 @@ def main():
                'sample': '%d probes x %d Lanczos steps (eight probes per '
                          'thread) on the same n=%d matrix: %d steps in %.1fs'
                          % (8 * threads, m, n, steps, dt)}
+        cal = cpu_calibration()
+        if cal is not None:
+            cpu['calibration'] = cal
         log('cpu baseline: %.1f matvec-steps/s on %d threads' % (rate, threads))
 
     out = {
@@ -734,7 +759,8 @@ def main():
         'config': {'workload': 'slq-%s-correlation-n%d-nnz%.0f-m%d-s%d-b64'
                                % (args.func, n, k, m, s),
                    'n': n, 'nnz': int(A.nnz), 'lanczos_degree': m,
-                   'samples': s, 'block': 64, 'parallelism': 'single-gpu'},
+                   'samples': s, 'block': 64, 'parallelism': 'single-gpu',
+                   'completed_steps_per_call': done.value / args.steps},
         'roofline': {'bound': 'hbm', 'achieved': achieved,
                      'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
                      'frac': achieved / HBM_PEAK_GBS,

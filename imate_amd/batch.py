@@ -32,7 +32,11 @@ def slq_batch(matrices, func='log', gram=False, p=1.0, lanczos_degree=20,
 
     Returns ``(trace, info)`` with ``trace[i]`` and ``info`` holding
     ``error``, ``samples`` ([count, num_samples]), ``run_time`` (launch to
-    completion, seconds), ``setup_time`` and the autotuned grids.
+    completion, seconds), ``setup_time`` and the autotuned grids (empty
+    when nothing was tuned: Golub-Kahn, a rectangular matrix or
+    ``autotune=False``). Tuned grids are used by this batch only; later
+    estimates on the same matrices run with the default grids, so their
+    results do not change bit for bit after a batch.
     """
     if func not in _FUNCS:
         raise ValueError('"func" should be one of %s.' % list(_FUNCS))
@@ -62,7 +66,10 @@ def slq_batch(matrices, func='log', gram=False, p=1.0, lanczos_degree=20,
     if len(set(h.value for h in handles)) != count:
         raise ValueError('each problem needs its own Matrix object.')
     grids = []
-    if autotune:
+    # the autotuner times Lanczos steps, so Golub-Kahn runs (gram) and
+    # rectangular matrices keep the default grids
+    square = all(M.get_num_rows() == M.get_num_columns() for M in ops)
+    if autotune and not gram and square:
         for h in handles:
             sg = ctypes.c_int32(0)
             ug = ctypes.c_int32(0)

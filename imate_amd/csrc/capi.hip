@@ -705,8 +705,18 @@ static int build_transpose_chunked(hipStream_t s, const DevCSR& A, DevCSR& T)
 int build_transpose(hipStream_t s, const DevCSR& A, DevCSR& T)
 {
     const int64_t nnz = A.nnz;
-    if (nnz >= (int64_t) 1 << 31 || std::getenv("SLQ_TRANSPOSE_CHUNK"))
-        return build_transpose_chunked(s, A, T);
+    // The chunked transpose finds each row's run of a column range by binary
+    // search, so it needs the column indices sorted within every row. An
+    // unsorted CSR takes the one-sort path where the sort can count it, and
+    // is refused above that.
+    const bool chunked = nnz >= (int64_t) 1 << 31 || std::getenv("SLQ_TRANSPOSE_CHUNK");
+    if (chunked && A.sorted) return build_transpose_chunked(s, A, T);
+    if (nnz >= (int64_t) 1 << 31)
+    {
+        set_error("transpose: a matrix with 2^31 or more non-zeros needs its column "
+                  "indices sorted within every row");
+        return SLQ_E_ARG;
+    }
     const size_t vb = A.dtype == SLQ_F64 ? 8 : 4;
     T.nrows = A.ncols;
     T.ncols = A.nrows;
@@ -854,7 +864,6 @@ void average_estimates(const double* samples, int q, double confidence,
     for (int j = 0; j < q; ++j)
     {
         const int N = num_used[j];
-        std::vector<char> out(N > 0 ? N : 1, 0);
         num_outliers[j] = 0;
         double s = 0.0;
         for (int i = 0; i < N; ++i) s += samples[(size_t) processed[i] * q + j];
@@ -872,26 +881,25 @@ void average_estimates(const double* samples, int q, double confidence,
         }
         else
             sd = INFINITY;
+        // outlier test against the first pass's mean and deviation; it is
+        // re-evaluated in the passes below instead of being stored
         const double half = oz * sd;
+        const double mean0 = mean;
+        auto outlier_at = [&](int i) {
+            return std::abs(samples[(size_t) processed[i] * q + j] - mean0) > half;
+        };
         for (int i = 0; i < N; ++i)
-        {
-            const double d = samples[(size_t) processed[i] * q + j] - mean;
-            if (std::abs(d) > half)
-            {
-                out[i] = 1;
-                num_outliers[j] += 1;
-            }
-        }
+            if (outlier_at(i)) num_outliers[j] += 1;
         const int no = num_outliers[j];
         s = 0.0;
         for (int i = 0; i < N; ++i)
-            if (!out[i]) s += samples[(size_t) processed[i] * q + j];
+            if (!outlier_at(i)) s += samples[(size_t) processed[i] * q + j];
         mean = s / (N - no);
         if (N > 1 + no)
         {
             s = 0.0;
             for (int i = 0; i < N; ++i)
-                if (!out[i])
+                if (!outlier_at(i))
                 {
                     const double d = samples[(size_t) processed[i] * q + j] - mean;
                     s += d * d;
@@ -923,6 +931,7 @@ struct Plan {
     size_t off_words = 0, off_X0 = 0, off_alpha = 0, off_beta = 0, off_sizes = 0,
            off_samples = 0, off_params = 0, off_scratch = 0, total = 0;
     size_t scratch_bytes = 0;
+    char* base = nullptr;   // device workspace the offsets are taken from
 };
 
 size_t align256(size_t b) { return (b + 255) & ~(size_t) 255; }
@@ -951,7 +960,7 @@ Plan make_plan(slq_op* op, int gram, int m, int ortho, int q, int vbytes)
 }
 
 template <typename U>
-U* at(slq_op* op, size_t off) { return (U*) ((char*) op->ws.base + off); }
+U* at(const Plan& p, size_t off) { return (U*) (p.base + off); }
 
 int value_dtype(const slq_op* op, int precision)
 {
@@ -1016,9 +1025,9 @@ int prepare(slq_op* op, int gram, int precision)
 int make_probes(slq_op* op, const Plan& p, const uint64_t base[4], int64_t first,
                 int count, int vdtype)
 {
-    SLQ_TRY(rng_words(op->stream, base, p.W, first, count, at<uint64_t>(op, p.off_words)));
-    SLQ_TRY(rng_expand(op->stream, at<uint64_t>(op, p.off_words), p.W, p.len, count,
-                       vdtype, at<void>(op, p.off_X0)));
+    SLQ_TRY(rng_words(op->stream, base, p.W, first, count, at<uint64_t>(p, p.off_words)));
+    SLQ_TRY(rng_expand(op->stream, at<uint64_t>(p, p.off_words), p.W, p.len, count,
+                       vdtype, at<void>(p, p.off_X0)));
     return SLQ_OK;
 }
 
@@ -1033,11 +1042,11 @@ int recurrence_block(slq_op* op, const Plan& p, double t, int gram, double tol,
     ra.tol = tol;
     ra.ortho = ortho;
     ra.precision = precision;
-    ra.X0 = at<void>(op, p.off_X0);
-    ra.alpha = at<double>(op, p.off_alpha);
-    ra.beta = at<double>(op, p.off_beta);
-    ra.sizes = at<int32_t>(op, p.off_sizes);
-    return run_recurrence(ra, at<void>(op, p.off_scratch), p.scratch_bytes);
+    ra.X0 = at<void>(p, p.off_X0);
+    ra.alpha = at<double>(p, p.off_alpha);
+    ra.beta = at<double>(p, p.off_beta);
+    ra.sizes = at<int32_t>(p, p.off_sizes);
+    return run_recurrence(ra, at<void>(p, p.off_scratch), p.scratch_bytes);
 }
 
 struct DeviceGuard {
@@ -1120,6 +1129,7 @@ double time_block(slq_op* op, int steps, int precision)
     const int vdt = value_dtype(op, precision);
     Plan p = make_plan(op, 0, steps, 0, 1, vdt == SLQ_F64 ? 8 : 4);
     if (op->ws.reserve(p.total) != SLQ_OK) return -1.0;
+    p.base = (char*) op->ws.base;
     uint64_t base[4];
     thread_state(1, 0, base);
     if (make_probes(op, p, base, 0, kBlock, vdt) != SLQ_OK) return -1.0;
@@ -1160,8 +1170,12 @@ struct TuneKey {
     }
 };
 
+struct TuneValue {
+    int spmm_grid, stream_grid, spmm_split;
+};
+
 std::mutex g_tune_mu;
-std::map<TuneKey, std::pair<int, int>> g_tune_cache;
+std::map<TuneKey, TuneValue> g_tune_cache;
 
 }  // namespace
 
@@ -1187,25 +1201,33 @@ extern "C" int slq_autotune(slq_op* op, int32_t precision, int32_t* spmm_grid,
         auto it = g_tune_cache.find(key);
         if (it != g_tune_cache.end())
         {
-            op->spmm_grid = it->second.first;
-            op->stream_grid = it->second.second;
-            if (spmm_grid) *spmm_grid = op->spmm_grid;
-            if (stream_grid) *stream_grid = op->stream_grid;
+            op->tuned_spmm_grid = it->second.spmm_grid;
+            op->tuned_stream_grid = it->second.stream_grid;
+            op->tuned_spmm_split = it->second.spmm_split;
+            if (spmm_grid) *spmm_grid = op->tuned_spmm_grid;
+            if (stream_grid) *stream_grid = op->tuned_stream_grid;
             return SLQ_OK;
         }
     }
-    // candidate caps (multiples of 8 up to the natural grid of the problem)
+    // candidate caps (multiples of 8 up to the natural grid of the problem),
+    // timed on the handle's own grid fields, which are restored afterwards
     const int cands[] = {8, 16, 32, 64, 128, 256, 512, 1024};
     const int steps = 4;
+    const int keep_s = op->spmm_grid, keep_u = op->stream_grid;
     int best_s = 0, best_u = 0;
     double best_t = 1e300;
+    int r = SLQ_OK;
     op->stream_grid = 0;
     for (int cap : cands)
     {
         if (cap > grid_for(c.nrows, kMaxGrid)) break;
         op->spmm_grid = cap;
         const double t = time_block(op, steps, precision);
-        if (t < 0) return SLQ_E_HIP;
+        if (t < 0)
+        {
+            r = SLQ_E_HIP;
+            break;
+        }
         if (t < best_t)
         {
             best_t = t;
@@ -1216,20 +1238,54 @@ extern "C" int slq_autotune(slq_op* op, int32_t precision, int32_t* spmm_grid,
     best_t = 1e300;
     for (int cap : cands)
     {
-        if (cap > grid_for(c.nrows, kMaxGrid)) break;
+        if (r != SLQ_OK || cap > grid_for(c.nrows, kMaxGrid)) break;
         op->stream_grid = cap;
         const double t = time_block(op, steps, precision);
-        if (t < 0) return SLQ_E_HIP;
+        if (t < 0)
+        {
+            r = SLQ_E_HIP;
+            break;
+        }
         if (t < best_t)
         {
             best_t = t;
             best_u = cap;
         }
     }
-    op->stream_grid = best_u;
+    // wavefronts per tile of the matrix-core SpMM, at the chosen caps
+    int best_k = 0;
+    const int keep_k = op->spmm_split;
+    if (r == SLQ_OK && op->A->tiles.R == 16)
+    {
+        op->spmm_grid = best_s;
+        op->stream_grid = best_u;
+        best_t = 1e300;
+        for (int k : {1, 2, 4})
+        {
+            op->spmm_split = k;
+            const double t = time_block(op, steps, precision);
+            if (t < 0)
+            {
+                r = SLQ_E_HIP;
+                break;
+            }
+            if (t < best_t)
+            {
+                best_t = t;
+                best_k = k;
+            }
+        }
+    }
+    op->spmm_grid = keep_s;
+    op->stream_grid = keep_u;
+    op->spmm_split = keep_k;
+    if (r != SLQ_OK) return r;
+    op->tuned_spmm_grid = best_s;
+    op->tuned_stream_grid = best_u;
+    op->tuned_spmm_split = best_k;
     {
         std::lock_guard<std::mutex> lock(g_tune_mu);
-        g_tune_cache[key] = {best_s, best_u};
+        g_tune_cache[key] = {best_s, best_u, best_k};
     }
     if (spmm_grid) *spmm_grid = best_s;
     if (stream_grid) *stream_grid = best_u;
@@ -1237,6 +1293,36 @@ extern "C" int slq_autotune(slq_op* op, int32_t precision, int32_t* spmm_grid,
 }
 
 // ---------------------------------------------------------------- batch
+
+// Swaps the autotuned launch settings into the handles of a batch and back.
+struct TunedLaunch {
+    struct Saved {
+        int spmm_grid, stream_grid, spmm_split;
+    };
+    slq_op** ops;
+    int count;
+    std::vector<Saved> saved;
+    TunedLaunch(slq_op** o, int c) : ops(o), count(c), saved(c)
+    {
+        for (int i = 0; i < count; ++i)
+        {
+            slq_op* op = ops[i];
+            saved[i] = {op->spmm_grid, op->stream_grid, op->spmm_split};
+            if (op->tuned_spmm_grid > 0) op->spmm_grid = op->tuned_spmm_grid;
+            if (op->tuned_stream_grid > 0) op->stream_grid = op->tuned_stream_grid;
+            if (op->tuned_spmm_split > 0) op->spmm_split = op->tuned_spmm_split;
+        }
+    }
+    ~TunedLaunch()
+    {
+        for (int i = 0; i < count; ++i)
+        {
+            ops[i]->spmm_grid = saved[i].spmm_grid;
+            ops[i]->stream_grid = saved[i].stream_grid;
+            ops[i]->spmm_split = saved[i].spmm_split;
+        }
+    }
+};
 
 // A valid matrix function: a known code, with coefficients where the code
 // takes them (a positive sigma for the Gaussian).
@@ -1283,6 +1369,9 @@ extern "C" int slq_estimate_batch(slq_op** ops, int32_t count, int32_t func,
             }
     }
     DeviceGuard guard(ops[0]->device);
+    // the autotuned launch settings of each handle apply for this call only
+    // (plans, captures and launches); the handle's own are restored on exit
+    TunedLaunch tuned(ops, count);
     if (seed < 0) seed = clock_seed();
     uint64_t base[4];
     thread_state(seed, 0, base);
@@ -1300,26 +1389,28 @@ extern "C" int slq_estimate_batch(slq_op** ops, int32_t count, int32_t func,
         plans[i] = make_plan(op, gram, m, orthogonalize, 1, vdt == SLQ_F64 ? 8 : 4);
         sample_off[i] = plans[i].total;
         SLQ_TRY(op->ws.reserve(plans[i].total + align256(sizeof(double) * num_samples)));
+        plans[i].base = (char*) op->ws.base;
     }
     SLQ_CHECK(hipDeviceSynchronize());
     // every block of problem i on its own stream
-    auto enqueue = [&](int i) -> int {
+    auto enqueue_blocks = [&](int i) -> int {
         slq_op* op = ops[i];
         const Plan& p = plans[i];
         const int vdt = value_dtype(op, precision);
-        double* dsamp = at<double>(op, sample_off[i]);
+        double* dsamp = at<double>(p, sample_off[i]);
         for (int64_t s0 = 0; s0 < num_samples; s0 += kBlock)
         {
             const int cnt = (int) std::min<int64_t>(kBlock, num_samples - s0);
             SLQ_TRY(make_probes(op, p, base, s0, cnt, vdt));
             SLQ_TRY(recurrence_block(op, p, 0.0, gram, tol, orthogonalize, precision));
-            SLQ_TRY(run_quadrature(op->stream, at<double>(op, p.off_alpha),
-                                   at<double>(op, p.off_beta), at<int32_t>(op, p.off_sizes),
+            SLQ_TRY(run_quadrature(op->stream, at<double>(p, p.off_alpha),
+                                   at<double>(p, p.off_beta), at<int32_t>(p, p.off_sizes),
                                    cnt, m, gram, func, func_params, exponent, (double) p.len,
                                    nullptr, 1, 0, 0, 1, dsamp + s0, nullptr, nullptr));
         }
         return SLQ_OK;
     };
+    auto enqueue = [&](int i) -> int { return enqueue_blocks(i); };
     double t_run = 0.0;
     if (use_graph)
     {
@@ -1400,7 +1491,7 @@ extern "C" int slq_estimate_batch(slq_op** ops, int32_t count, int32_t func,
     for (int k = 0; k < num_samples; ++k) processed[k] = k;
     for (int i = 0; i < count; ++i)
     {
-        SLQ_CHECK(hipMemcpy(hs.data(), at<double>(ops[i], sample_off[i]),
+        SLQ_CHECK(hipMemcpy(hs.data(), at<double>(plans[i], sample_off[i]),
                             sizeof(double) * num_samples, hipMemcpyDeviceToHost));
         if (samples) std::memcpy(samples + (size_t) i * num_samples, hs.data(),
                                  sizeof(double) * num_samples);
@@ -1912,6 +2003,161 @@ int slq_op_destroy(slq_op* op)
     return SLQ_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// Arguments of one estimate (slq_estimate).
+struct EstimateArgs {
+    const double* params;
+    int32_t q, func;
+    const double* func_params;
+    int32_t gram;
+    double exponent;
+    int32_t orthogonalize;
+    int64_t seed;
+    int32_t m;
+    double tol;
+    int32_t min_s, max_s;
+    double atol, rtol, confidence, outlier;
+    int32_t block_size, precision;
+    double *trace, *error, *samples;
+    int32_t *processed, *num_used, *num_outliers, *converged, *all_converged;
+    double* alg_wall_time;
+};
+
+int check_estimate(const char* who, const slq_op* op, const EstimateArgs& a)
+{
+    if (!op || a.q < 1 || a.m < 1 || a.min_s < 1 || a.max_s < a.min_s || !a.trace ||
+        !a.error || !a.samples || !a.processed || !a.num_used || !a.num_outliers ||
+        !a.converged || !func_ok(a.func, a.func_params) ||
+        (a.precision != SLQ_PREC_NATIVE && a.precision != SLQ_PREC_MIXED))
+    {
+        set_error("%s: invalid argument", who);
+        return SLQ_E_ARG;
+    }
+    if (op->kind != OP_MATRIX && !a.params)
+    {
+        set_error("%s: an affine operator needs parameters", who);
+        return SLQ_E_ARG;
+    }
+    if (op->kind == OP_MATRIX && a.q != 1)
+    {
+        set_error("%s: a matrix operator has exactly one inquiry", who);
+        return SLQ_E_ARG;
+    }
+    if (!a.gram && op->A->csr.nrows != op->A->csr.ncols)
+    {
+        set_error("%s: Lanczos needs a square operator", who);
+        return SLQ_E_ARG;
+    }
+    return SLQ_OK;
+}
+
+// The sampling loop of one estimate on the workspace p.base (sized by p),
+// following cu_trace_estimator.cu:197-310 with one device: probes in blocks
+// of 64 in sample order, one host synchronisation per block (the block's
+// samples are copied straight into the caller's `samples` rows), then the
+// reference's per-sample convergence test. Allocates no device memory.
+int estimate_loop(slq_op* op, const Plan& p, EstimateArgs a)
+{
+    const int q = a.q;
+    const int bs = (a.block_size <= 0 || a.block_size > kBlock) ? kBlock : a.block_size;
+    const bool relation = op->kind == OP_AFFINE_I;
+    const int required = relation ? 1 : q;
+    const int vdt = value_dtype(op, a.precision);
+    if (a.seed < 0) a.seed = clock_seed();
+    double* params_dev = at<double>(p, p.off_params);
+    if (op->kind != OP_MATRIX)
+        SLQ_CHECK(hipMemcpyAsync(params_dev, a.params, sizeof(double) * q,
+                                 hipMemcpyHostToDevice, op->stream));
+    uint64_t base[4];
+    thread_state(a.seed, 0, base);
+    const double nq = (double) p.len;
+    const double nan = std::numeric_limits<double>::quiet_NaN();
+    double* samples = a.samples;
+    int32_t* converged = a.converged;
+
+    for (int64_t i = 0; i < (int64_t) a.max_s * q; ++i) samples[i] = nan;
+    for (int j = 0; j < q; ++j)
+    {
+        converged[j] = 0;
+        a.num_used[j] = 0;
+        a.num_outliers[j] = 0;
+    }
+    // completed recurrence steps per block, kept only while profiling
+    const bool count_steps = op->prof.enabled;
+    std::vector<int32_t> block_sizes(count_steps ? (size_t) kBlock * required : 0);
+    int nproc = 0;
+    int all_conv = 0;
+
+    SLQ_CHECK(hipStreamSynchronize(op->stream));
+    const auto t0 = std::chrono::steady_clock::now();
+    for (int64_t s0 = 0; s0 < a.max_s && !all_conv; s0 += bs)
+    {
+        const int cnt = (int) std::min<int64_t>(bs, a.max_s - s0);
+        SLQ_TRY(make_probes(op, p, base, s0, cnt, vdt));
+        std::vector<char> ran(count_steps ? required : 0, 0);
+        for (int j = 0; j < required; ++j)
+        {
+            if (!relation && converged[j]) continue;   // masked below anyway
+            const double t = op->kind != OP_MATRIX ? a.params[j] : 0.0;
+            SLQ_TRY(recurrence_block(op, p, t, a.gram, a.tol, a.orthogonalize, a.precision));
+            SLQ_TRY(run_quadrature(op->stream, at<double>(p, p.off_alpha),
+                                   at<double>(p, p.off_beta), at<int32_t>(p, p.off_sizes),
+                                   cnt, a.m, a.gram, a.func, a.func_params, a.exponent, nq,
+                                   relation ? params_dev : nullptr, q, relation ? 1 : 0, j,
+                                   q, at<double>(p, p.off_samples), nullptr, nullptr));
+            if (count_steps)
+            {
+                SLQ_CHECK(hipMemcpyAsync(block_sizes.data() + (size_t) j * kBlock,
+                                         at<int32_t>(p, p.off_sizes), sizeof(int32_t) * kBlock,
+                                         hipMemcpyDeviceToHost, op->stream));
+                ran[j] = 1;
+            }
+        }
+        SLQ_CHECK(hipMemcpyAsync(samples + s0 * q, at<double>(p, p.off_samples),
+                                 sizeof(double) * cnt * q, hipMemcpyDeviceToHost, op->stream));
+        SLQ_CHECK(hipStreamSynchronize(op->stream));
+        for (int pi = 0; pi < cnt; ++pi)
+        {
+            const int64_t i = s0 + pi;
+            if (all_conv)
+            {
+                // past the stopping sample: not processed
+                for (int j = 0; j < q; ++j) samples[i * q + j] = nan;
+                continue;
+            }
+            if (relation && q > 1)
+                for (int j = 0; j < q; ++j) converged[j] = 0;
+            for (int j = 0; j < q; ++j)
+                if (converged[j]) samples[i * q + j] = nan;
+            if (count_steps)
+                for (int j = 0; j < required; ++j)
+                    if (ran[j])
+                    {
+                        const int32_t sz = block_sizes[(size_t) j * kBlock + pi];
+                        op->prof.steps += sz > 0 ? sz : a.m;
+                    }
+            a.processed[nproc++] = (int32_t) i;
+            all_conv = check_convergence(samples, q, a.min_s, a.processed, nproc, a.confidence,
+                                         a.atol, a.rtol, a.error, a.num_used, converged);
+        }
+    }
+    const auto t1 = std::chrono::steady_clock::now();
+    if (a.alg_wall_time) *a.alg_wall_time = std::chrono::duration<double>(t1 - t0).count();
+    for (int i = nproc; i < a.max_s; ++i) a.processed[i] = 0;
+    average_estimates(samples, q, a.confidence, a.outlier, a.num_used, a.processed,
+                      a.num_outliers, a.trace, a.error);
+    if (a.all_converged) *a.all_converged = all_conv;
+    prof_collect(op);
+    return SLQ_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
 int slq_estimate(slq_op* op, const double* params, int32_t q, int32_t func,
                  const double* func_params, int32_t gram, double exponent,
                  int32_t orthogonalize,
@@ -1923,100 +2169,18 @@ int slq_estimate(slq_op* op, const double* params, int32_t q, int32_t func,
                  int32_t* converged, int32_t* all_converged,
                  double* alg_wall_time)
 {
-    if (!op || q < 1 || m < 1 || min_s < 1 || max_s < min_s || !trace || !error ||
-        !samples || !processed || !num_used || !num_outliers || !converged ||
-        !func_ok(func, func_params) ||
-        (precision != SLQ_PREC_NATIVE && precision != SLQ_PREC_MIXED))
-    {
-        set_error("slq_estimate: invalid argument");
-        return SLQ_E_ARG;
-    }
-    if (op->kind != OP_MATRIX && !params)
-    {
-        set_error("slq_estimate: an affine operator needs parameters");
-        return SLQ_E_ARG;
-    }
-    if (op->kind == OP_MATRIX && q != 1)
-    {
-        set_error("slq_estimate: a matrix operator has exactly one inquiry");
-        return SLQ_E_ARG;
-    }
-    if (!gram && op->A->csr.nrows != op->A->csr.ncols)
-    {
-        set_error("slq_estimate: Lanczos needs a square operator");
-        return SLQ_E_ARG;
-    }
+    const EstimateArgs a{params, q, func, func_params, gram, exponent, orthogonalize, seed, m,
+                         tol, min_s, max_s, atol, rtol, confidence, outlier, block_size,
+                         precision, trace, error, samples, processed, num_used, num_outliers,
+                         converged, all_converged, alg_wall_time};
+    SLQ_TRY(check_estimate("slq_estimate", op, a));
     DeviceGuard guard(op->device);
-    const int bs = (block_size <= 0 || block_size > kBlock) ? kBlock : block_size;
-    const bool relation = op->kind == OP_AFFINE_I;
-    const int required = relation ? 1 : q;
     const int vdt = value_dtype(op, precision);
-    if (seed < 0) seed = clock_seed();
-
     SLQ_TRY(prepare(op, gram, precision));
     Plan p = make_plan(op, gram, m, orthogonalize, q, vdt == SLQ_F64 ? 8 : 4);
     SLQ_TRY(op->ws.reserve(p.total));
-    double* params_dev = at<double>(op, p.off_params);
-    if (op->kind != OP_MATRIX)
-        SLQ_CHECK(hipMemcpyAsync(params_dev, params, sizeof(double) * q,
-                                 hipMemcpyHostToDevice, op->stream));
-    uint64_t base[4];
-    thread_state(seed, 0, base);
-    const double nq = (double) p.len;
-
-    for (int64_t i = 0; i < (int64_t) max_s * q; ++i)
-        samples[i] = std::numeric_limits<double>::quiet_NaN();
-    for (int j = 0; j < q; ++j)
-    {
-        converged[j] = 0;
-        num_used[j] = 0;
-        num_outliers[j] = 0;
-    }
-    std::vector<double> block_out((size_t) kBlock * q);
-    int nproc = 0;
-    int all_conv = 0;
-
-    SLQ_CHECK(hipStreamSynchronize(op->stream));
-    const auto t0 = std::chrono::steady_clock::now();
-    for (int64_t s0 = 0; s0 < max_s && !all_conv; s0 += bs)
-    {
-        const int cnt = (int) std::min<int64_t>(bs, max_s - s0);
-        SLQ_TRY(make_probes(op, p, base, s0, cnt, vdt));
-        for (int j = 0; j < required; ++j)
-        {
-            if (!relation && converged[j]) continue;   // masked below anyway
-            const double t = op->kind != OP_MATRIX ? params[j] : 0.0;
-            SLQ_TRY(recurrence_block(op, p, t, gram, tol, orthogonalize, precision));
-            SLQ_TRY(run_quadrature(op->stream, at<double>(op, p.off_alpha),
-                                   at<double>(op, p.off_beta), at<int32_t>(op, p.off_sizes),
-                                   cnt, m, gram, func, func_params, exponent, nq,
-                                   relation ? params_dev : nullptr, q, relation ? 1 : 0, j,
-                                   q, at<double>(op, p.off_samples), nullptr, nullptr));
-        }
-        SLQ_CHECK(hipMemcpyAsync(block_out.data(), at<double>(op, p.off_samples),
-                                 sizeof(double) * cnt * q, hipMemcpyDeviceToHost,
-                                 op->stream));
-        SLQ_CHECK(hipStreamSynchronize(op->stream));
-        for (int pi = 0; pi < cnt && !all_conv; ++pi)
-        {
-            const int64_t i = s0 + pi;
-            if (relation && q > 1)
-                for (int j = 0; j < q; ++j) converged[j] = 0;
-            for (int j = 0; j < q; ++j)
-                if (!converged[j]) samples[i * q + j] = block_out[(size_t) pi * q + j];
-            processed[nproc++] = (int32_t) i;
-            all_conv = check_convergence(samples, q, min_s, processed, nproc, confidence,
-                                         atol, rtol, error, num_used, converged);
-        }
-    }
-    const auto t1 = std::chrono::steady_clock::now();
-    if (alg_wall_time) *alg_wall_time = std::chrono::duration<double>(t1 - t0).count();
-    for (int i = nproc; i < max_s; ++i) processed[i] = 0;
-    average_estimates(samples, q, confidence, outlier, num_used, processed, num_outliers,
-                      trace, error);
-    if (all_converged) *all_converged = all_conv;
-    prof_collect(op);
-    return SLQ_OK;
+    p.base = (char*) op->ws.base;
+    return estimate_loop(op, p, a);
 }
 
 int slq_rademacher(int64_t seed, int64_t n, int64_t first_sample, int32_t count,
@@ -2120,17 +2284,18 @@ int slq_recurrences(slq_op* op, double t, int32_t gram, int64_t seed,
     SLQ_TRY(prepare(op, gram, precision));
     Plan p = make_plan(op, gram, m, ortho, 1, vdt == SLQ_F64 ? 8 : 4);
     SLQ_TRY(op->ws.reserve(p.total));
+    p.base = (char*) op->ws.base;
     uint64_t base[4];
     thread_state(seed, 0, base);
     SLQ_TRY(make_probes(op, p, base, first_sample, count, vdt));
     SLQ_TRY(recurrence_block(op, p, t, gram, tol, ortho, precision));
     std::vector<double> ha((size_t) m * kBlock), hb((size_t) m * kBlock);
     std::vector<int32_t> hs(kBlock);
-    SLQ_CHECK(hipMemcpyAsync(ha.data(), at<double>(op, p.off_alpha), sizeof(double) * m * kBlock,
+    SLQ_CHECK(hipMemcpyAsync(ha.data(), at<double>(p, p.off_alpha), sizeof(double) * m * kBlock,
                              hipMemcpyDeviceToHost, op->stream));
-    SLQ_CHECK(hipMemcpyAsync(hb.data(), at<double>(op, p.off_beta), sizeof(double) * m * kBlock,
+    SLQ_CHECK(hipMemcpyAsync(hb.data(), at<double>(p, p.off_beta), sizeof(double) * m * kBlock,
                              hipMemcpyDeviceToHost, op->stream));
-    SLQ_CHECK(hipMemcpyAsync(hs.data(), at<int32_t>(op, p.off_sizes), sizeof(int32_t) * kBlock,
+    SLQ_CHECK(hipMemcpyAsync(hs.data(), at<int32_t>(p, p.off_sizes), sizeof(int32_t) * kBlock,
                              hipMemcpyDeviceToHost, op->stream));
     SLQ_CHECK(hipStreamSynchronize(op->stream));
     prof_collect(op);
@@ -2216,6 +2381,41 @@ int slq_profile_enable(slq_op* op, int32_t enable)
     op->prof.spmm_launches = 0;
     op->prof.update_ms = 0.0;
     op->prof.update_launches = 0;
+    op->prof.steps = 0;
+    return SLQ_OK;
+}
+
+int slq_op_set_launch(slq_op* op, int32_t spmm_grid, int32_t stream_grid,
+                      int32_t spmm_split)
+{
+    if (!op || spmm_grid < 0 || spmm_grid > kMaxGrid || stream_grid < 0 ||
+        stream_grid > kMaxGrid || !(spmm_split == 0 || spmm_split == 1 || spmm_split == 2 ||
+                                    spmm_split == 4))
+    {
+        set_error("slq_op_set_launch: grid caps must be 0..%d and the split 0, 1, 2 or 4",
+                  kMaxGrid);
+        return SLQ_E_ARG;
+    }
+    op->spmm_grid = spmm_grid;
+    op->stream_grid = stream_grid;
+    op->spmm_split = spmm_split;
+    return SLQ_OK;
+}
+
+int slq_op_get_launch(slq_op* op, int32_t* spmm_grid, int32_t* stream_grid,
+                      int32_t* spmm_split)
+{
+    if (!op) return SLQ_E_ARG;
+    if (spmm_grid) *spmm_grid = op->spmm_grid;
+    if (stream_grid) *stream_grid = op->stream_grid;
+    if (spmm_split) *spmm_split = op->spmm_split;
+    return SLQ_OK;
+}
+
+int slq_profile_steps(slq_op* op, int64_t* steps)
+{
+    if (!op || !steps) return SLQ_E_ARG;
+    *steps = op->prof.steps;
     return SLQ_OK;
 }
 

@@ -939,6 +939,171 @@ __global__ __launch_bounds__(kThreads) void mfma_spmm_kernel(
     }
 }
 
+// The matrix-core SpMM with S wavefronts per tile (S = 2 or 4): the waves
+// of a team split the tile's slab list into S contiguous ranges of eight-slab
+// pairs, so a workgroup holds 8 / S tiles instead of 8 and the operand rows
+// gathered by the tiles in flight on an XCD take 1 / S of the L2 that one
+// wave per tile needs, while every slab value and every gathered row is
+// still loaded once. Each wave writes the partial products of the rows it
+// does not finish to LDS; after a workgroup barrier wave j of a team adds
+// the S partials of row slices m = j (4 / S) .. (j + 1) (4 / S) - 1 in the
+// fixed order of the waves and runs the epilogue on them, writing the
+// reduction term of its rows as partial row t S + j (tile_sum_kernel adds
+// them in order, so results are deterministic for a given S). Tiles are
+// fetched per workgroup (8 / S at a time) from the XCD's in-order ticket, so
+// every wave of a workgroup runs the same number of iterations and reaches
+// the same barriers.
+template <typename T, bool NT, int S>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4)))
+void mfma_spmm_split_kernel(
+    SpmmArgs<T> a, const T* __restrict__ X, T* __restrict__ Y, T* __restrict__ Vout,
+    const T* __restrict__ Sub, int64_t ntiles, const int64_t* __restrict__ tptr,
+    const int32_t* __restrict__ tcols, const T* __restrict__ tvals)
+{
+    static_assert(S == 2 || S == 4, "two or four wavefronts per tile");
+    using V4 = typename Mfma16<T>::V4;
+    constexpr int kTeams = kWavesPerWG / S;
+    constexpr int MS = 4 / S;   // row slices finished per wave
+    // per team: for every slice m and every wave j not finishing it, the four
+    // values q of each lane
+    __shared__ T part[kTeams][4 * (S - 1)][4][kWave];
+    __shared__ int64_t s_first;
+    const int lane = threadIdx.x % kWave;
+    const int wave = threadIdx.x / kWave;
+    const int team = wave / S;
+    const int j = wave % S;
+    const int i0 = lane % 16;
+    const int kq = lane / 16;
+    const int V = gridDim.x < (unsigned int) kXcds ? (int) gridDim.x : kXcds;
+    const int x = blockIdx.x % V;
+    const int64_t span = (ntiles + V - 1) / V;
+    const int64_t b = x * span;
+    const int64_t e = b + span < ntiles ? b + span : ntiles;
+    const unsigned int waves = (unsigned int) ((gridDim.x - x + V - 1) / V) * kWavesPerWG;
+    unsigned int* ticket = a.rb.tickets + 64 * x;
+    unsigned int* done = ticket + 32;
+    T sq[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) sq[q] = a.xscale ? (T) a.xscale[4 * i0 + q] : T(1);
+    const T tt = (T) a.t;
+    auto slot = [](int m, int src) { return m * (S - 1) + (src < m / MS ? src : src - 1); };
+    for (;;)
+    {
+        if (threadIdx.x == 0) s_first = b + (int64_t) atomicAdd(ticket, (unsigned int) kTeams);
+        __syncthreads();
+        const int64_t first = s_first;
+        if (first >= e) break;   // the same value in every wave of the workgroup
+        const int64_t t = first + team;
+        const bool valid = t < e;
+        V4 acc[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) acc[q] = V4{T(0), T(0), T(0), T(0)};
+        if (valid)
+        {
+            const int64_t beg0 = tptr[t];
+            const int64_t npt = (tptr[t + 1] - beg0) / 8;
+            const int64_t beg = beg0 + 8 * ((npt * j) / S);
+            const int64_t end = beg0 + 8 * ((npt * (j + 1)) / S);
+            for (int64_t base = beg; base < end; base += kWave)
+            {
+                const int64_t rem = end - base;
+                const int cnt = rem < kWave ? (int) rem : kWave;   // multiple of 8
+                const int cl = lane < cnt ? tcols[base + lane] : 0;
+                auto col_of = [&](int g) {
+                    const int c0 = readlane(cl, 4 * g);
+                    const int c1 = readlane(cl, 4 * g + 1);
+                    const int c2 = readlane(cl, 4 * g + 2);
+                    const int c3 = readlane(cl, 4 * g + 3);
+                    return (int64_t) (kq == 0 ? c0 : (kq == 1 ? c1 : (kq == 2 ? c2 : c3)));
+                };
+                const int np = cnt / 8;
+                const Pair2<T>* __restrict__ ap2 =
+                    reinterpret_cast<const Pair2<T>*>(tvals + base * 16) + lane;
+                Pair2<T> ap = ap2[0], an;
+                T b0[4], b1[4];
+                load_row4(X + col_of(0) * kBlock + 4 * i0, b0);
+                for (int P = 0; P < np; ++P)
+                {
+                    load_row4(X + col_of(2 * P + 1) * kBlock + 4 * i0, b1);
+#pragma unroll
+                    for (int q = 0; q < 4; ++q)
+                        acc[q] = Mfma16<T>::op(ap.x, sq[q] * b0[q], acc[q]);
+                    if (P + 1 < np)
+                    {
+                        an = ap2[(P + 1) * kWave];
+                        load_row4(X + col_of(2 * P + 2) * kBlock + 4 * i0, b0);
+                    }
+#pragma unroll
+                    for (int q = 0; q < 4; ++q)
+                        acc[q] = Mfma16<T>::op(ap.y, sq[q] * b1[q], acc[q]);
+                    ap = an;
+                }
+            }
+        }
+        // partial products of the slices other waves finish
+#pragma unroll
+        for (int m = 0; m < 4; ++m)
+            if (m / MS != j)
+            {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) part[team][slot(m, j)][q][lane] = acc[q][m];
+            }
+        __syncthreads();
+        if (valid)
+        {
+            double red[4] = {0.0, 0.0, 0.0, 0.0};
+            T sub_c[4], sub_s[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+            {
+                sub_c[q] = Sub ? (T) a.sub_coef[4 * i0 + q] : T(0);
+                sub_s[q] = Sub ? (T) a.sub_scale[4 * i0 + q] : T(0);
+            }
+#pragma unroll
+            for (int mm = 0; mm < MS; ++mm)
+            {
+                const int m = j * MS + mm;
+                T y[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                {
+                    // waves 0 .. S-1 in order, this wave's own partial in place
+                    T v = j == 0 ? acc[q][m] : part[team][slot(m, 0)][q][lane];
+#pragma unroll
+                    for (int src = 1; src < S; ++src)
+                        v += src == j ? acc[q][m] : part[team][slot(m, src)][q][lane];
+                    y[q] = v;
+                }
+                const int64_t row = t * 16 +
+                                    (Mfma16<T>::kRowStride == 4 ? kq + 4 * m : 4 * kq + m);
+                if (row < a.nrows)
+                    spmm_epilogue4<T, NT>(a, row, i0, y, sq, tt, sub_c, sub_s, X, Y, Vout,
+                                          Sub, red);
+            }
+            if (a.red_mode)
+            {
+                double mine = 0.0;
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                {
+                    const double r1 = red[q] + __shfl_xor(red[q], 16);
+                    const double r2 = r1 + __shfl_xor(r1, 32);
+                    if (q == kq) mine = r2;
+                }
+                a.rb.tile_partial[(t * S + j) * kWave + 4 * i0 + kq] = mine;
+            }
+        }
+    }
+    unsigned int d = 0;
+    if (lane == 0) d = atomicAdd(done, 1u);
+    d = __builtin_amdgcn_readfirstlane(d);
+    if (d == waves - 1 && lane == 0)
+    {
+        atomicExch(ticket, 0u);
+        atomicExch(done, 0u);
+    }
+}
+
 // Dense SpMM on the matrix cores: Y = A (s X) for a row-major A padded to
 // rows of ld values (ld a multiple of 64, zero padding). A workgroup of 8
 // wavefronts takes 128 rows (16 per wavefront, one MFMA row block) and
@@ -1710,8 +1875,27 @@ struct Bump {
     }
 };
 
+// Wavefronts per tile of the matrix-core SpMM: SLQ_SPMM_SPLIT = 1 (default),
+// 2 or 4 (mfma_spmm_split_kernel), unless the handle sets its own.
+int spmm_split_default()
+{
+    static const int v = [] {
+        const char* e = std::getenv("SLQ_SPMM_SPLIT");
+        const int s = e ? std::atoi(e) : 1;
+        return (s == 2 || s == 4) ? s : 1;
+    }();
+    return v;
+}
+
+// The handle's setting (slq_op_set_launch) or the default.
+int spmm_split(const slq_op* op)
+{
+    return op->spmm_split > 0 ? op->spmm_split : spmm_split_default();
+}
+
 // Tiles of the largest tiled operand a recurrence on `op` may multiply by
-// (A and, for Golub-Kahn, A^T): the length of the per-tile partial rows.
+// (A and, for Golub-Kahn, A^T), times the partial rows per tile of the
+// matrix-core SpMM: the length of the per-tile partial rows.
 int64_t sched_tiles(const slq_op* op)
 {
     const MatrixStore& A = *op->A;
@@ -1726,7 +1910,7 @@ int64_t sched_tiles(const slq_op* op)
     if (A.tiles.R == 0) return 0;
     const int64_t R = A.tiles.R;
     const int64_t a = (A.csr.nrows + R - 1) / R, b = (A.csr.ncols + R - 1) / R;
-    return a > b ? a : b;
+    return (a > b ? a : b) * (R == 16 ? spmm_split(op) : 1);
 }
 
 RedBuf take_redbuf(Bump& bump, int64_t tiles)
@@ -1983,7 +2167,16 @@ void launch_spmm_nt(slq_op* op, const SpmmArgs<T>& s, const SpmmIO<T>& io, const
                                            (s.red_mode == 0 || s.rb.tile_partial));
         if (sched)
         {
-            if (tl->R == 16)
+            const int split = tl->R == 16 ? spmm_split(op) : 1;
+            if (tl->R == 16 && split == 2)
+                hipLaunchKernelGGL((mfma_spmm_split_kernel<T, NT, 2>), dim3(grid),
+                                   dim3(kThreads), 0, op->stream, s, io.X, io.Y, io.Vout,
+                                   io.Sub, tl->ntiles, tl->ptr, tl->cols, tv);
+            else if (tl->R == 16 && split == 4)
+                hipLaunchKernelGGL((mfma_spmm_split_kernel<T, NT, 4>), dim3(grid),
+                                   dim3(kThreads), 0, op->stream, s, io.X, io.Y, io.Vout,
+                                   io.Sub, tl->ntiles, tl->ptr, tl->cols, tv);
+            else if (tl->R == 16)
                 hipLaunchKernelGGL((mfma_spmm_kernel<T, NT>), dim3(grid), dim3(kThreads), 0,
                                    op->stream, s, io.X, io.Y, io.Vout, io.Sub, tl->ntiles,
                                    tl->ptr, tl->cols, tv);
@@ -1993,11 +2186,11 @@ void launch_spmm_nt(slq_op* op, const SpmmArgs<T>& s, const SpmmIO<T>& io, const
                                    io.Sub, tl->ntiles, tl->ptr, tl->cols, tv);
             if (s.red_mode)
             {
-                int rg = (int) ((tl->ntiles + 63) / 64);
+                const int64_t prows = tl->ntiles * split;
+                int rg = (int) ((prows + 63) / 64);
                 rg = rg < 1 ? 1 : (rg > kTileSumGrid ? kTileSumGrid : rg);
                 hipLaunchKernelGGL(tile_sum_kernel, dim3(rg), dim3(kThreads), 0, op->stream,
-                                   (const double*) s.rb.tile_partial, tl->ntiles,
-                                   s.rb.partial);
+                                   (const double*) s.rb.tile_partial, prows, s.rb.partial);
                 hipLaunchKernelGGL((total_kernel<T>), dim3(1), dim3(kTotalWaves * kWave), 0,
                                    op->stream, s, rg);
             }

@@ -90,7 +90,18 @@ std::vector<BitMatrix>& jump_tables()
     return tables;
 }
 
-// One device copy of the tables per device.
+}  // namespace
+}  // namespace slq
+
+// Device copy of the 64 jump matrices (512 KiB), one per device in a static
+// array, so that probe generation never allocates.
+static __device__ __align__(256) uint64_t jump_tab[64 * 256 * 4];
+
+namespace slq {
+namespace {
+
+// Device address of the tables on the current device, filled from the host
+// tables on first use there (looked up with that device current).
 uint64_t* device_tables()
 {
     static std::mutex mu;
@@ -102,15 +113,13 @@ uint64_t* device_tables()
     if (!per_device[dev])
     {
         auto& t = jump_tables();
-        size_t bytes = sizeof(BitMatrix) * t.size();
-        uint64_t* d = nullptr;
-        if (hipMalloc(&d, bytes) != hipSuccess) return nullptr;
-        if (hipMemcpy(d, t.data(), bytes, hipMemcpyHostToDevice) != hipSuccess)
-        {
-            (void) hipFree(d);
+        static_assert(sizeof(BitMatrix) * 64 == sizeof(jump_tab), "jump table size");
+        void* d = nullptr;
+        if (hipGetSymbolAddress(&d, HIP_SYMBOL(jump_tab)) != hipSuccess || !d) return nullptr;
+        if (hipMemcpy(d, t.data(), sizeof(BitMatrix) * t.size(), hipMemcpyHostToDevice) !=
+            hipSuccess)
             return nullptr;
-        }
-        per_device[dev] = d;
+        per_device[dev] = (uint64_t*) d;
     }
     return per_device[dev];
 }

@@ -147,6 +147,9 @@ struct ProfileState {
     int64_t spmm_launches = 0;
     double update_ms = 0.0;
     int64_t update_launches = 0;
+    // completed Lanczos / Golub-Kahn steps of the processed samples (summed
+    // over the recurrences each sample ran) of slq_estimate calls
+    int64_t steps = 0;
     std::vector<hipEvent_t> pool;            // free events
     std::vector<std::pair<hipEvent_t, hipEvent_t>> spmm_pending;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> update_pending;
@@ -180,9 +183,17 @@ struct slq_op {
     slq::ProfileState prof;
     slq::PairTiles ab;                      // OP_AFFINE_B only: A + tB
     slq::PairTiles ab_t;                    // and A^T + tB^T (Golub-Kahn)
-    // launch-grid caps chosen by the autotuner (0 = library default)
+    // launch-grid caps of the kernels (0 = library default)
     int spmm_grid = 0;
     int stream_grid = 0;
+    // caps chosen by the autotuner (0 = not tuned); slq_estimate_batch runs
+    // with them, every other entry point with the caps above, so a tuned
+    // handle gives slq_estimate the same reduction order as before
+    int tuned_spmm_grid = 0;
+    int tuned_stream_grid = 0;
+    // wavefronts per tile of the matrix-core SpMM (1, 2, 4; 0 = default)
+    int spmm_split = 0;
+    int tuned_spmm_split = 0;
 };
 
 namespace slq {

@@ -204,12 +204,29 @@ int slq_estimate_batch(slq_op** ops, int32_t count, int32_t func,
                        double* samples, double* run_time, double* setup_time);
 
 /* Autotuner: times one 64-probe block of Lanczos steps over candidate launch
- * grids for the SpMM and for the streaming kernels, keeps the fastest in the
- * handle (used by every later call on it) and caches it per (rows, non-zeros
- * per row, value type, tile rows). Outputs the chosen caps when non-NULL.
+ * grids for the SpMM and for the streaming kernels, then (16-row tiles) over
+ * 1, 2 and 4 wavefronts per tile of the matrix-core SpMM, keeps the fastest
+ * in the handle for slq_estimate_batch and caches it per (rows, non-zeros
+ * per row, value type, tile rows). Outputs the chosen caps when non-NULL. The other
+ * entry points keep the library's default grids on a tuned handle, so their
+ * reduction order, and with it every result bit, does not depend on tuning.
+ * Square matrix or A + tI handles only (Lanczos is what is timed).
  * No reference counterpart (imate has no tunable kernels). */
 int slq_autotune(slq_op* op, int32_t precision, int32_t* spmm_grid,
                  int32_t* stream_grid);
+
+/* Launch configuration of a handle's kernels: grid caps of the SpMM and of
+ * the streaming kernels (0 .. 1024) and wavefronts per tile of the
+ * matrix-core SpMM (1, 2 or 4); 0 keeps the library default (environment
+ * SLQ_GRID / SLQ_STREAM_GRID / SLQ_SPMM_SPLIT, else the built-in). Every
+ * setting gives results within rounding of every other; for a fixed
+ * setting they are reproducible bit for bit (the reduction order follows
+ * the grid and the split). No reference counterpart (imate has no tunable
+ * kernels). */
+int slq_op_set_launch(slq_op* op, int32_t spmm_grid, int32_t stream_grid,
+                      int32_t spmm_split);
+int slq_op_get_launch(slq_op* op, int32_t* spmm_grid, int32_t* stream_grid,
+                      int32_t* spmm_split);
 
 /* ---- learned surrogate (no reference counterpart) ----
  * A 2 -> 128 -> 128 -> 1 SiLU MLP used as a surrogate of SLQ estimates
@@ -287,6 +304,12 @@ int slq_quadrature(const double* alpha, const double* beta,
 int slq_profile_enable(slq_op* op, int32_t enable);
 int slq_profile_read(slq_op* op, double* spmm_ms, int64_t* spmm_launches,
                      double* update_ms, int64_t* update_launches);
+/* Completed Lanczos / Golub-Kahn steps of the samples processed by
+ * slq_estimate since profiling was enabled (each sample's recurrence length,
+ * shorter than lanczos_degree where it broke down; summed over the
+ * recurrences a sample ran). Like imate's info['convergence'], no reference
+ * counterpart for the count itself. */
+int slq_profile_steps(slq_op* op, int64_t* steps);
 
 /* Device copy of `bytes` bytes (a plain streaming kernel) timed with HIP
  * events over `iters` launches; used to calibrate rocprof byte counters and

@@ -611,3 +611,31 @@ def slq(A, func='log', gram=False, exponent=1.0, parameters=None, B=None,
                 processed_samples_indices=processed,
                 num_samples_used=num_used, num_outliers=num_outliers,
                 converged=converged, all_converged=all_conv, sizes=sizes)
+
+
+def first_samples(A, func='log', lanczos_degree=20, count=16, seed=0,
+                  gram=False, exponent=1.0, orthogonalize=0, dtype=None,
+                  threads=16):
+    """Per-sample estimates of the first ``count`` samples of the one-thread
+    (reproducible) stream, i.e. ``slq(...)['samples'][:count, 0]`` of a plain
+    matrix, with the samples' recurrences spread over ``threads`` host
+    threads (each runs ``sample_estimates`` on its probe; the C calls release
+    the interpreter lock). For sizes where the one-thread driver would take
+    minutes."""
+    from concurrent.futures import ThreadPoolExecutor
+    op = Operator(A, dtype=dtype)
+    dt = op.dtype
+    n = op.shape[1] if gram else op.shape[0]
+    tol = float(numpy.finfo(dt).eps)
+    probes = rademacher(seed, n, count, dtype=dt)
+
+    def one(k):
+        est, sizes = sample_estimates(op, probes[k], func, gram, exponent,
+                                      int(lanczos_degree), tol,
+                                      int(orthogonalize))
+        return est[0], sizes[0]
+
+    with ThreadPoolExecutor(max_workers=max(1, int(threads))) as ex:
+        res = list(ex.map(one, range(count)))
+    return (numpy.array([r[0] for r in res]),
+            numpy.array([r[1] for r in res], dtype=numpy.int32))

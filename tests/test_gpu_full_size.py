@@ -1,0 +1,119 @@
+"""The measured configurations at their full sizes.
+
+* n = 2^20 correlation matrix (53.3 M non-zeros, FP64), lanczos_degree 30,
+  256 probes: logdet against the sparse Cholesky value recorded for this
+  exact matrix (tests/golden/cholesky_logdet.json) within 1 % and three
+  95 % Monte-Carlo errors; logdet and traceinv per sample against the
+  oracle for the first 16 probes (the oracle spreads them over host
+  threads), at 1e-10.
+* Golub-Kahn on the 2^20 x 2^18 sparse Gaussian matrix (2.75e9 non-zeros:
+  64-bit offsets, transpose built in 2^30-entry chunks), 64 probes: logdet
+  of A^T A against the Wishart mean of a dense Gaussian matrix with the same
+  entry variance; the sparse matrix matches it up to O(1/column degree) =
+  1e-4, so the bar is 2e-4 relative plus three Monte-Carlo errors.
+* 32 FP32 correlation matrices with sizes log-uniform in [1e3, 2e5] in one
+  hipGraph launch: the first four probes of every problem against the
+  oracle's FP32 and FP64 runs on the same matrix at 2e-5 relative (the
+  oracle's own FP32 run is within 3.4e-6 of its FP64 run on these
+  matrices).
+"""
+
+import json
+import os
+
+import numpy
+import pytest
+
+import imate_amd
+from imate_amd.sample_matrices import sparse_correlation_matrix
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def _cholesky_logdet(n, k, seed):
+    with open(os.path.join(HERE, 'golden', 'cholesky_logdet.json')) as f:
+        for row in json.load(f)['entries']:
+            if row['n'] == n and row['nnz_per_row'] == k and row['seed'] == seed:
+                return row['logdet']
+    raise KeyError((n, k, seed))
+
+
+@pytest.fixture(scope='module')
+def c2():
+    n = 1 << 20
+    A = sparse_correlation_matrix(n, nnz_per_row=50.0, seed=0)
+    M = imate_amd.Matrix(A)
+    M.get_linear_operator(gpu=True)
+    return A, M
+
+
+def test_c2_logdet_vs_cholesky_and_oracle(c2):
+    A, M = c2
+    n = A.shape[0]
+    ld, info = imate_amd.logdet(M, method='slq', gpu=True, lanczos_degree=30,
+                                min_num_samples=256, max_num_samples=256,
+                                seed=0, return_info=True)
+    exact = _cholesky_logdet(n, 50.0, 0)
+    err = info['error']['absolute_error']
+    assert abs(ld - exact) <= 0.01 * abs(exact)
+    assert abs(ld - exact) <= 3 * err, (ld, exact, err)
+    want, sizes = oracle.first_samples(A, 'log', 30, 16, seed=0, threads=16)
+    got = info['convergence']['samples'][:16]
+    assert numpy.all(sizes == 30)
+    assert numpy.max(numpy.abs(got - want) / numpy.abs(want)) < 1e-10
+
+
+def test_c2_traceinv_vs_oracle(c2):
+    A, M = c2
+    n = A.shape[0]
+    ti, info = imate_amd.traceinv(M, method='slq', gpu=True, lanczos_degree=30,
+                                  min_num_samples=256, max_num_samples=256,
+                                  seed=1, return_info=True)
+    # tr(A^-1) >= n^2 / tr(A) (harmonic <= arithmetic mean of eigenvalues)
+    assert ti >= n * n / A.diagonal().sum()
+    want, _ = oracle.first_samples(A, 'inv', 30, 16, seed=1, threads=16)
+    got = info['convergence']['samples'][:16]
+    assert numpy.max(numpy.abs(got - want) / numpy.abs(want)) < 1e-10
+
+
+def test_gram_2p20_x_2p18_vs_wishart_mean():
+    from scipy.special import digamma
+    from imate_amd.sample_matrices import sparse_gaussian_device
+    N, p, density = 1 << 20, 1 << 18, 0.01
+    D = sparse_gaussian_device(N, p, density, seed=0)
+    assert D.get_nnz() >= 1 << 31
+    val, info = imate_amd.logdet(D, gram=True, method='slq', gpu=True,
+                                 lanczos_degree=30, min_num_samples=64,
+                                 max_num_samples=64, seed=0,
+                                 return_info=True)
+    k = int(round(density * p))
+    s2 = k / p
+    i = numpy.arange(p, dtype=numpy.float64)
+    mean = p * numpy.log(s2) + float(numpy.sum(digamma((N - i) / 2.0) +
+                                              numpy.log(2.0)))
+    err = info['error']['absolute_error']
+    assert abs(val - mean) <= 2e-4 * abs(mean) + 3 * err, (val, mean, err)
+    del D
+
+
+def test_batch_32_fp32_matrices_vs_oracle():
+    from imate_amd.batch import slq_batch
+    rng = numpy.random.default_rng(0)
+    sizes = numpy.round(numpy.exp(rng.uniform(numpy.log(1e3), numpy.log(2e5),
+                                              32))).astype(int)
+    mats = [sparse_correlation_matrix(int(k), nnz_per_row=30, seed=i,
+                                      dtype='float32')
+            for i, k in enumerate(sizes)]
+    m, s, first = 30, 64, 4
+    tr, info = slq_batch(mats, lanczos_degree=m, num_samples=s, seed=0,
+                         use_graph=True)
+    for i, A in enumerate(mats):
+        got = info['samples'][i][:first]
+        w32, _ = oracle.first_samples(A, 'log', m, first, seed=0, threads=16)
+        w64, _ = oracle.first_samples(A.astype(numpy.float64), 'log', m,
+                                      first, seed=0, threads=16)
+        assert numpy.max(numpy.abs(got - w32) / numpy.abs(w32)) < 2e-5, i
+        assert numpy.max(numpy.abs(got - w64) / numpy.abs(w64)) < 2e-5, i

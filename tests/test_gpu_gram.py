@@ -156,3 +156,61 @@ def test_blocked_spmm_ragged_rows(gpu_lib, nrows):
                                    _lib.ptr(Y)), 'slq_op_spmm')
     ref = A @ X
     assert numpy.max(numpy.abs(Y - ref)) <= 1e-13 * numpy.abs(ref).max()
+
+
+def test_unsorted_csr_transpose_with_chunking_set(gpu_lib, monkeypatch):
+    """A CSR given through the C ABI with every row's columns reversed
+    (unsorted) while the chunked transpose is requested: the chunked path
+    needs sorted rows, so the one-sort path must build A^T, and the
+    transposed product and Golub-Kahn must equal those of the sorted
+    matrix."""
+    import ctypes
+    from imate_amd import _lib
+    monkeypatch.setenv('SLQ_TRANSPOSE_CHUNK', '777')
+    # singular values near 2: the recurrences move by ~1e-15 under a
+    # relative perturbation of 1e-15 of the values (checked on the oracle),
+    # so a different summation order stays inside the 1e-10 bar
+    G = scipy.sparse.random(700, 300, density=0.06, random_state=21,
+                            format='csr',
+                            data_rvs=numpy.random.default_rng(1).standard_normal)
+    A = (2.0 * scipy.sparse.eye(700, 300, format='csr') + 0.3 * G).tocsr()
+    A.sort_indices()
+    indptr = A.indptr.astype(numpy.int64)
+    ix = A.indices.astype(numpy.int32).copy()
+    v = A.data.copy()
+    for r in range(A.shape[0]):
+        a, b = indptr[r], indptr[r + 1]
+        ix[a:b] = ix[a:b][::-1].copy()
+        v[a:b] = v[a:b][::-1].copy()
+    h = ctypes.c_void_p()
+    _lib.check(gpu_lib.slq_op_create_csr(700, 300, A.nnz, _lib.ptr(indptr),
+                                         _lib.ptr(ix), _lib.ptr(v),
+                                         _lib.SLQ_F64, ctypes.byref(h)),
+               'slq_op_create_csr')
+    try:
+        U = numpy.random.default_rng(3).standard_normal((700, 64))
+        Z = numpy.empty((300, 64))
+        _lib.check(gpu_lib.slq_op_spmm(h, 0.0, 64, 1, _lib.ptr(U),
+                                       _lib.ptr(Z)), 'slq_op_spmm')
+        refz = A.T @ U
+        assert numpy.max(numpy.abs(Z - refz)) <= 1e-13 * numpy.abs(refz).max()
+        m, s = 12, 40
+        alpha = numpy.zeros((s, m))
+        beta = numpy.zeros((s, m))
+        sizes = numpy.zeros(s, dtype=numpy.int32)
+        _lib.check(gpu_lib.slq_recurrences(h, 0.0, 1, 6, 0, s, m,
+                                           float(numpy.finfo(float).eps), 0,
+                                           _lib.PREC_NATIVE, _lib.ptr(alpha),
+                                           _lib.ptr(beta), _lib.ptr(sizes)),
+                   'slq_recurrences')
+    finally:
+        gpu_lib.slq_op_destroy(h)
+    op = oracle.Operator(A)
+    a2, b2, sz2 = oracle.recurrences(op, 6, s, m,
+                                     float(numpy.finfo(float).eps), 0,
+                                     gram=True)
+    assert numpy.array_equal(sizes, sz2)
+    for k in range(s):
+        n_k = sizes[k]
+        assert numpy.allclose(alpha[k, :n_k], a2[k, :n_k], rtol=1e-10, atol=0)
+        assert numpy.allclose(beta[k, :n_k], b2[k, :n_k], rtol=1e-10, atol=0)

@@ -1,0 +1,102 @@
+"""The precision form (precision='mixed': FP32 SpMM, probes, basis and
+Gram-Schmidt; FP64 alpha/beta, eigensolve and quadrature) against the
+oracle's FP64 samples and, at its full configuration, against exact values.
+
+Bars:
+* per sample, relative 1e-5 of the oracle's FP64 value. The oracle's own
+  FP32 run differs from its FP64 run by at most 3.4e-6 on C1 for every
+  (lanczos_degree, orthogonalize) below, so 1e-5 is about three times the
+  FP32 rounding of the whole recurrence; a probe, step or sign error moves a
+  sample by far more.
+* full configuration (n = 65536, kernel scale 0.01, lanczos_degree 60, full
+  re-orthogonalisation, 256 probes): logdet and Schatten-2 within 1 % and
+  within three times the 95 % Monte-Carlo error of the dense FP64 Cholesky
+  values in tests/golden/exact_precision_form.json (Gauss quadrature is
+  exact for f(x) = x^2 at this degree, so Schatten-2 has Monte-Carlo error
+  only). traceinv: Gauss quadrature underestimates x^-1 on an SPD matrix
+  (the even derivatives of 1/x are positive), by 8.2e-3 of the exact value
+  at lanczos_degree 60 on this matrix; the bar is an error below 1 % with
+  the estimate not above the exact value by more than three Monte-Carlo
+  errors.
+"""
+
+import json
+import os
+
+import numpy
+import pytest
+
+import imate_amd
+from imate_amd.sample_matrices import sparse_correlation_matrix
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def _c1():
+    return sparse_correlation_matrix(4096, nnz_per_row=40, seed=0)
+
+
+@pytest.mark.parametrize('ortho', [0, -1])
+@pytest.mark.parametrize('m', [20, 30, 60])
+def test_mixed_per_sample_vs_oracle_fp64(m, ortho):
+    A = _c1()
+    s = 64
+    val, info = imate_amd.logdet(A, method='slq', gpu=True, lanczos_degree=m,
+                                 min_num_samples=s, max_num_samples=s,
+                                 orthogonalize=ortho, seed=5,
+                                 precision='mixed', return_info=True)
+    ref = oracle.slq(A, func='log', lanczos_degree=m, min_num_samples=s,
+                     max_num_samples=s, seed=5, orthogonalize=ortho)
+    got = info['convergence']['samples']
+    want = ref['samples'][:, 0]
+    rel = numpy.abs(got - want) / numpy.abs(want)
+    assert rel.max() < 1e-5, rel.max()
+    assert abs(val - ref['trace'][0]) <= 1e-5 * abs(ref['trace'][0])
+
+
+@pytest.mark.parametrize('func', ['inv', 'identity'])
+def test_mixed_reorth_other_functions_vs_oracle(func):
+    A = _c1()
+    fn = {'inv': imate_amd.traceinv, 'identity': imate_amd.trace}[func]
+    kw = dict(method='slq', gpu=True, lanczos_degree=60, min_num_samples=64,
+              max_num_samples=64, orthogonalize=-1, seed=8,
+              precision='mixed', return_info=True)
+    if func == 'identity':
+        kw['p'] = 2
+    val, info = fn(A, **kw)
+    ref = oracle.slq(A, func=func, lanczos_degree=60, min_num_samples=64,
+                     max_num_samples=64, seed=8, orthogonalize=-1,
+                     exponent=2.0 if func == 'identity' else 1.0)
+    got = info['convergence']['samples']
+    want = ref['samples'][:, 0]
+    assert (numpy.abs(got - want) / numpy.abs(want)).max() < 1e-5
+
+
+def test_precision_form_full_config_vs_exact():
+    with open(os.path.join(HERE, 'golden', 'exact_precision_form.json')) as f:
+        exact = json.load(f)
+    n, m, s = exact['n'], 60, 256
+    A = sparse_correlation_matrix(n, scale=exact['scale'], seed=exact['seed'])
+    assert A.nnz == exact['nnz']
+    M = imate_amd.Matrix(A)
+    for prec in ('mixed', 'native'):
+        kw = dict(method='slq', gpu=True, lanczos_degree=m, min_num_samples=s,
+                  max_num_samples=s, orthogonalize=-1, seed=0,
+                  precision=prec, return_info=True)
+        ld, i1 = imate_amd.logdet(M, **kw)
+        e1 = i1['error']['absolute_error']
+        assert abs(ld - exact['logdet']) <= 0.01 * abs(exact['logdet'])
+        assert abs(ld - exact['logdet']) <= 3 * e1, (prec, ld, e1)
+        t2, i2 = imate_amd.trace(M, p=2, **kw)
+        e2 = i2['error']['absolute_error']
+        assert abs(t2 - exact['trace_A2']) <= 0.01 * exact['trace_A2']
+        assert abs(t2 - exact['trace_A2']) <= 3 * e2, (prec, t2, e2)
+        sch = (t2 / n) ** 0.5
+        assert abs(sch - exact['schatten2']) <= 0.01 * exact['schatten2']
+        ti, i3 = imate_amd.traceinv(M, **kw)
+        e3 = i3['error']['absolute_error']
+        assert abs(ti - exact['traceinv']) <= 0.01 * exact['traceinv']
+        assert ti - exact['traceinv'] <= 3 * e3, (prec, ti, e3)

@@ -102,10 +102,14 @@ def test_convergence_tools_match_reference(gold):
     assert nout.sum() >= 1   # the planted outliers were found
 
 
-@pytest.mark.skipif(oracle.ref_lib() is None, reason='reference not built')
 def test_oracle_equals_reference_build_sweep():
-    """Wider sweep straight against the reference build (where present)."""
+    """Wider sweep straight against the reference build (where present).
+
+    The build is looked up inside the test, so that collecting the suite
+    (``-m gpu`` included) never loads it."""
     import ctypes
+    if oracle.ref_lib() is None:
+        pytest.skip('reference not built')
     sys_path_gold = os.path.join(HERE, 'golden')
     import sys
     sys.path.insert(0, sys_path_gold)
@@ -286,3 +290,47 @@ def test_cholesky_logdet_c1():
                    max_num_samples=100, seed=0)
     assert abs(r['trace'][0] - exact) <= 0.01 * abs(exact)
     assert abs(r['trace'][0] - exact) <= 3 * r['error'][0]
+
+
+# ---------------------------------------------------------------- drivers
+
+def test_first_samples_equals_one_thread_driver():
+    """The threaded per-sample helper gives the one-thread driver's samples
+    (same thread-0 probes, same recurrences) bit for bit."""
+    from imate_amd.sample_matrices import sparse_correlation_matrix
+    A = sparse_correlation_matrix(2000, nnz_per_row=20, seed=3)
+    for func, gram, ortho in (('log', False, 0), ('inv', False, -1),
+                              ('log', True, 0)):
+        est, sizes = oracle.first_samples(A, func, 15, 10, seed=4,
+                                          gram=gram, orthogonalize=ortho,
+                                          threads=4)
+        ref = oracle.slq(A, func=func, gram=gram, lanczos_degree=15,
+                         min_num_samples=10, max_num_samples=10, seed=4,
+                         orthogonalize=ortho)
+        assert numpy.array_equal(est, ref['samples'][:, 0])
+
+
+@pytest.mark.parametrize('func', ['log', 'inv'])
+def test_ritz_shift_equals_direct_runs(func):
+    """A + tI with several t: one recurrence per probe and the Ritz values
+    shifted by t_j - t_0 (c_affine_matrix_function.cpp:97-113) against a
+    separate one-thread run on the explicit matrix A + t_j I for every j.
+    Lanczos on A + tI has the same basis as on A (alpha moves by t, beta is
+    unchanged), so the two agree to rounding."""
+    from imate_amd.sample_matrices import sparse_correlation_matrix
+    A = sparse_correlation_matrix(1500, nnz_per_row=25, seed=9)
+    ts = [0.0, 1e-3, 0.2, 3.0, 50.0]
+    kw = dict(func=func, lanczos_degree=20, min_num_samples=30,
+              max_num_samples=30, seed=2)
+    sh = oracle.slq(A, parameters=ts, affine=True, **kw)
+    for j, t in enumerate(ts):
+        At = (A + t * scipy.sparse.eye(A.shape[0], format='csr')).tocsr()
+        direct = oracle.slq(At, **kw)
+        a = sh['samples'][:, j]
+        b = direct['samples'][:, 0]
+        assert numpy.max(numpy.abs(a - b) / numpy.abs(b)) < 1e-9, (t, func)
+    # the bar separates neighbouring t: the shifted values of t = 1e-3
+    # are not those of a direct run at t = 0 (a dropped shift would pass)
+    d0 = oracle.slq(A, **kw)['samples'][:, 0]
+    assert numpy.max(numpy.abs(sh['samples'][:, 1] - d0) /
+                     numpy.abs(d0)) > 1e-6
