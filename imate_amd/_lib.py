@@ -61,6 +61,10 @@ SIGNATURES = {
         _P, _P, _i32, _i32, _P, _i32, _f64, _i32, _i64, _i32, _f64, _i32, _i32,
         _f64, _f64, _f64, _f64, _i32, _i32, _P, _P, _P, _P, _P, _P, _P, _P,
         _P]),
+    'slq_estimate_noalloc': (ctypes.c_int, [
+        _P, _P, _i32, _i32, _P, _i32, _f64, _i32, _i64, _i32, _f64, _i32, _i32,
+        _f64, _f64, _f64, _f64, _i32, _i32, _P, _P, _P, _P, _P, _P, _P, _P,
+        _P]),
     'slq_rademacher': (ctypes.c_int, [_i64, _i64, _i64, _i32, ctypes.c_int,
                                       _P, _P]),
     'slq_op_spmm': (ctypes.c_int, [_P, _f64, _i32, _i32, _P, _P]),

@@ -2007,7 +2007,7 @@ int slq_op_destroy(slq_op* op)
 
 namespace {
 
-// Arguments of one estimate (slq_estimate).
+// Arguments of one estimate, shared by slq_estimate and slq_estimate_noalloc.
 struct EstimateArgs {
     const double* params;
     int32_t q, func;
@@ -2154,6 +2154,85 @@ int estimate_loop(slq_op* op, const Plan& p, EstimateArgs a)
     return SLQ_OK;
 }
 
+// True when every derived form an estimate of this kind reads already exists
+// on the handle, i.e. prepare() would allocate nothing. Mirrors prepare().
+bool prepared(const slq_op* op, int gram, int precision)
+{
+    const MatrixStore& A = *op->A;
+    const bool mixed = precision == SLQ_PREC_MIXED;
+    if (A.dense)
+    {
+        if (gram && !A.dn.trans) return false;
+        if (mixed && A.csr.dtype == SLQ_F64 &&
+            (!A.dn.data_f32 || (gram && !A.dn.trans_f32)))
+            return false;
+        return true;
+    }
+    auto f32_ready = [gram](const MatrixStore& M) {
+        if (M.csr.dtype != SLQ_F64) return true;
+        if (M.tiles.R > 0 && !M.tiles.vals_f32) return false;
+        if (gram && M.trans_tiles.R > 0 && !M.trans_tiles.vals_f32) return false;
+        if (!M.data_f32) return false;
+        if (gram && M.has_trans && !M.trans_data_f32) return false;
+        return true;
+    };
+    if (gram)
+    {
+        if (!A.has_trans) return false;
+        if (A.tiles.R > 0 && !A.trans_tiles_tried) return false;
+        if (op->kind == OP_AFFINE_B && !op->B->has_trans) return false;
+    }
+    if (op->kind == OP_AFFINE_B)
+    {
+        if (!op->ab.tried || (gram && !op->ab_t.tried)) return false;
+        if (mixed && A.csr.dtype == SLQ_F64 &&
+            ((op->ab.tiles.R > 0 && !op->ab.tiles.vals_f32) ||
+             (gram && op->ab_t.tiles.R > 0 && !op->ab_t.tiles.vals_f32)))
+            return false;
+    }
+    if (mixed)
+    {
+        if (!f32_ready(A)) return false;
+        if (op->kind == OP_AFFINE_B && !f32_ready(*op->B)) return false;
+    }
+    return true;
+}
+
+}  // namespace
+
+// The no-allocation entry point's workspace: one array per device, reached
+// through the address hipGetSymbolAddress gives on that device (the array's
+// name in host code is only a host-side stand-in). Every call on a device
+// shares it, so calls that overlap on one device race.
+static constexpr size_t kNoallocScratchBytes = (size_t) 256 << 20;
+static __device__ __align__(256) unsigned char scratch[kNoallocScratchBytes];
+
+namespace {
+
+std::mutex g_scratch_mu;
+std::vector<void*> g_scratch_ptr;   // per device index
+
+// Device address of `scratch` on the current device (looked up once per
+// device, with that device current).
+int scratch_address(void** out)
+{
+    int dev = 0;
+    SLQ_CHECK(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if ((int) g_scratch_ptr.size() <= dev) g_scratch_ptr.resize(dev + 1, nullptr);
+    if (!g_scratch_ptr[dev])
+    {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return hip_status(e != hipSuccess ? e : hipErrorInvalidSymbol,
+                              "hipGetSymbolAddress(scratch)", __FILE__, __LINE__);
+        g_scratch_ptr[dev] = ptr;
+    }
+    *out = g_scratch_ptr[dev];
+    return SLQ_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -2180,6 +2259,49 @@ int slq_estimate(slq_op* op, const double* params, int32_t q, int32_t func,
     Plan p = make_plan(op, gram, m, orthogonalize, q, vdt == SLQ_F64 ? 8 : 4);
     SLQ_TRY(op->ws.reserve(p.total));
     p.base = (char*) op->ws.base;
+    return estimate_loop(op, p, a);
+}
+
+int slq_estimate_noalloc(slq_op* op, const double* params, int32_t q, int32_t func,
+                         const double* func_params, int32_t gram, double exponent,
+                         int32_t orthogonalize, int64_t seed, int32_t m, double tol,
+                         int32_t min_s, int32_t max_s, double atol, double rtol,
+                         double confidence, double outlier, int32_t block_size,
+                         int32_t precision, double* trace, double* error, double* samples,
+                         int32_t* processed, int32_t* num_used, int32_t* num_outliers,
+                         int32_t* converged, int32_t* all_converged, double* alg_wall_time)
+{
+    const EstimateArgs a{params, q, func, func_params, gram, exponent, orthogonalize, seed, m,
+                         tol, min_s, max_s, atol, rtol, confidence, outlier, block_size,
+                         precision, trace, error, samples, processed, num_used, num_outliers,
+                         converged, all_converged, alg_wall_time};
+    SLQ_TRY(check_estimate("slq_estimate_noalloc", op, a));
+    if (op->prof.enabled)
+    {
+        set_error("slq_estimate_noalloc: profiling is enabled on this handle (its events "
+                  "are created on demand); disable it first");
+        return SLQ_E_ARG;
+    }
+    DeviceGuard guard(op->device);
+    if (!prepared(op, gram, precision))
+    {
+        set_error("slq_estimate_noalloc: the handle lacks a derived form this estimate "
+                  "reads (transpose, FP32 copy or A + tB tiles); run slq_estimate of the "
+                  "same form on it once first");
+        return SLQ_E_ARG;
+    }
+    const int vdt = value_dtype(op, precision);
+    Plan p = make_plan(op, gram, m, orthogonalize, q, vdt == SLQ_F64 ? 8 : 4);
+    if (p.total > kNoallocScratchBytes)
+    {
+        set_error("slq_estimate_noalloc: this estimate needs %zu bytes of workspace, more "
+                  "than the %zu of the fixed scratch array; use slq_estimate",
+                  p.total, kNoallocScratchBytes);
+        return SLQ_E_ARG;
+    }
+    void* ptr = nullptr;
+    SLQ_TRY(scratch_address(&ptr));
+    p.base = (char*) ptr;
     return estimate_loop(op, p, a);
 }
 

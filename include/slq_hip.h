@@ -184,6 +184,34 @@ int slq_estimate(slq_op* op, const double* params, int32_t num_inquiries,
                  int32_t* num_outliers, int32_t* converged,
                  int32_t* all_converged, double* alg_wall_time);
 
+/* slq_estimate without device allocation: same arguments, outputs and
+ * kernels (results equal slq_estimate's bit for bit), with the per-call
+ * workspace taken from a fixed 256 MiB array in device memory, one per device
+ * for the life of the process, instead of the handle's allocated workspace.
+ * No counterpart in the reference: it is for callers that cannot allocate
+ * device memory (serving loops, code that forbids hipMalloc). Limits:
+ *   - the workspace of the estimate must fit 256 MiB (plain FP64 Lanczos up
+ *     to about 120k rows at 64 probes per block; larger needs, SLQ_E_ARG);
+ *   - the derived forms an estimate reads (the transpose for gram, FP32
+ *     copies for SLQ_PREC_MIXED, A + tB tiles) must already exist on the
+ *     handle, built by an earlier slq_estimate of the same form (else
+ *     SLQ_E_ARG); a plain CSR, dense or A + tI handle needs none;
+ *   - profiling must be off on the handle (else SLQ_E_ARG).
+ * Every call on one device shares the array: calls that overlap on one
+ * device (from several host threads) race. */
+int slq_estimate_noalloc(slq_op* op, const double* params, int32_t num_inquiries,
+                         int32_t func, const double* func_params, int32_t gram,
+                         double exponent, int32_t orthogonalize, int64_t seed,
+                         int32_t lanczos_degree, double lanczos_tol,
+                         int32_t min_num_samples, int32_t max_num_samples,
+                         double error_atol, double error_rtol,
+                         double confidence_level, double outlier_significance_level,
+                         int32_t block_size, int32_t precision,
+                         double* trace, double* error, double* samples,
+                         int32_t* processed_samples_indices, int32_t* num_samples_used,
+                         int32_t* num_outliers, int32_t* converged,
+                         int32_t* all_converged, double* alg_wall_time);
+
 /* Batched SLQ over `count` independent matrix handles (distinct handles, one
  * device), each with a fixed number of samples (no early stop): per problem
  * trace[i], error[i] (after the outlier pass) and samples[i][num_samples]
