@@ -48,6 +48,9 @@ def parse():
     p.add_argument('--func', default='logdet', choices=['logdet', 'traceinv'])
     p.add_argument('--no-cpu', action='store_true',
                    help='skip the CPU baseline leg')
+    p.add_argument('--no-forms', action='store_true',
+                   help='skip the compact precision / batched / surrogate '
+                        'legs of the default run')
     p.add_argument('--cpu-threads', type=int, default=0)
     p.add_argument('--copy-calibration', action='store_true',
                    help='only run the plain copy kernel (rocprof calibration)')
@@ -134,13 +137,14 @@ def cpu_calibration():
                            c['probes_per_thread'])}
 
 
-def precision_form(args):
+def precision_form(args, compact=False):
     """Precision form: n = 65536 correlation matrix with kernel scale 0.01
     (~247 non-zeros per row, cond ~1e4), lanczos_degree 60, full
     re-orthogonalisation, 64-probe blocks. FP32 SpMM / basis / Gram-Schmidt
     with FP64 recurrences and eigensolve ('mixed') against the plain FP64 GPU
     path; errors of logdet, traceinv and Schatten p=2 against exact values
-    (dense FP64 Cholesky, tests/golden/exact_precision_form.json)."""
+    (dense FP64 Cholesky, tests/golden/exact_precision_form.json).
+    compact: logdet only, full re-orthogonalisation, both precisions."""
     import imate_amd
     from imate_amd.sample_matrices import sparse_correlation_matrix
     n, m, s = 65536, 60, args.samples
@@ -154,44 +158,60 @@ def precision_form(args):
     Aop.get_linear_operator(gpu=True)
     rows = []
     for prec in ('native', 'mixed'):
-        for ortho in (-1, 0):
+        for ortho in ((-1,) if compact else (-1, 0)):
             kw = dict(method='slq', gpu=True, lanczos_degree=m,
                       min_num_samples=s, max_num_samples=s, seed=args.seed,
                       orthogonalize=ortho, precision=prec, return_info=True)
             imate_amd.logdet(Aop, **kw)   # warm-up (workspace, FP32 copies)
-            res = {}
             t0 = time.perf_counter()
             ld, i1 = imate_amd.logdet(Aop, **kw)
-            ti, i2 = imate_amd.traceinv(Aop, **kw)
-            t2, i3 = imate_amd.trace(Aop, p=2, **kw)
-            el = time.perf_counter() - t0
-            sch = (t2 / n) ** 0.5
+            el1 = time.perf_counter() - t0
             res = {'precision': prec, 'orthogonalize': ortho,
-                   'seconds_3_calls': el,
-                   'matvec_steps_per_s': 3 * s * m / el,
-                   'logdet': ld, 'logdet_mc_err': float(i1['error']['absolute_error']),
-                   'traceinv': ti, 'traceinv_mc_err': float(i2['error']['absolute_error']),
-                   'schatten2': sch}
+                   'logdet_matvec_steps_per_s': s * m / el1,
+                   'logdet': ld,
+                   'logdet_mc_err': float(i1['error']['absolute_error'])}
             if exact:
                 res['logdet_rel_err'] = abs(ld - exact['logdet']) / abs(exact['logdet'])
-                res['traceinv_rel_err'] = abs(ti - exact['traceinv']) / abs(exact['traceinv'])
-                res['schatten2_rel_err'] = abs(sch - exact['schatten2']) / exact['schatten2']
+            if not compact:
+                t0 = time.perf_counter()
+                ti, i2 = imate_amd.traceinv(Aop, **kw)
+                t2, i3 = imate_amd.trace(Aop, p=2, **kw)
+                el = time.perf_counter() - t0 + el1
+                sch = (t2 / n) ** 0.5
+                res.update({'seconds_3_calls': el,
+                            'matvec_steps_per_s': 3 * s * m / el,
+                            'traceinv': ti,
+                            'traceinv_mc_err': float(i2['error']['absolute_error']),
+                            'schatten2': sch})
+                if exact:
+                    res['traceinv_rel_err'] = abs(ti - exact['traceinv']) / abs(exact['traceinv'])
+                    res['schatten2_rel_err'] = abs(sch - exact['schatten2']) / exact['schatten2']
             log('precision form: %s' % json.dumps(res))
             rows.append(res)
-    print(json.dumps({'form': 'precision', 'n': n, 'nnz': int(A.nnz),
-                      'lanczos_degree': m, 'samples': s, 'exact': exact,
-                      'runs': rows}), flush=True)
+    out = {'form': 'precision', 'n': n, 'nnz': int(A.nnz),
+           'lanczos_degree': m, 'samples': s, 'runs': rows}
+    if not compact:
+        out['exact'] = exact
+    else:
+        nat, mix = rows[0], rows[1]
+        out['mixed_over_native_speed'] = (mix['logdet_matvec_steps_per_s'] /
+                                          nat['logdet_matvec_steps_per_s'])
+    return out
 
 
-def batched_form(args):
+def batched_form(args, compact=False):
     """Batched form. (i) A + tI with 256 log-spaced t in [1e-4, 1e2] on an
     n = 65536 correlation matrix (~50 non-zeros per row, FP64): one Lanczos
-    run per probe with Ritz shifts for every t, against a loop of one SLQ run
-    per t. (ii) 32 correlation matrices with sizes log-uniform in [1e3, 2e5],
-    ~30 non-zeros per row, FP32: one hipGraph launch over all problems (with
-    autotuned launch grids) against a per-problem launch loop. (iii) A + tB
-    with a general B, 8 t, n = 262144: the tiled union-pattern SpMM against
-    the two-stream CSR SpMM (general_b_leg)."""
+    run per probe with Ritz shifts for every t, against a measured loop of
+    one SLQ run per t (all 256). (ii) 32 correlation matrices with sizes
+    log-uniform in [1e3, 2e5], ~30 non-zeros per row, FP32: one hipGraph
+    launch over all problems (with autotuned launch settings) against a
+    per-problem launch loop of the same kernels and against plain FP64 GPU
+    SLQ (one logdet call per matrix, the same probes: the FP32 values'
+    distance from the FP64 ones is reported in units of the Monte-Carlo
+    error, i.e. the accuracy is matched when it is far below 1). (iii, not
+    compact) A + tB with a general B, 8 t, n = 262144: the tiled
+    union-pattern SpMM against the two-stream CSR SpMM (general_b_leg)."""
     import imate_amd
     from imate_amd.batch import slq_batch
     from imate_amd.sample_matrices import sparse_correlation_matrix
@@ -208,25 +228,26 @@ def batched_form(args):
     t0 = time.perf_counter()
     vals, info = imate_amd.logdet(Aop, parameters=ts, **kw)
     t_shift = time.perf_counter() - t0
-    loop_ts = ts[::16]
+    imate_amd.logdet(Aop, parameters=float(ts[0]), **kw)
     t0 = time.perf_counter()
     direct = []
-    for t in loop_ts:
+    for t in ts:
         v, _ = imate_amd.logdet(Aop, parameters=float(t), **kw)
         direct.append(v)
-    t_loop = (time.perf_counter() - t0) * (len(ts) / len(loop_ts))
-    rel = numpy.max(numpy.abs(numpy.array(direct) - vals[::16]) /
-                    numpy.abs(numpy.array(direct)))
+    t_loop = time.perf_counter() - t0
+    direct = numpy.array(direct)
+    rel = numpy.max(numpy.abs(direct - vals) / numpy.abs(direct))
     out['affine_t256'] = {
         'n': n, 'nnz': int(A.nnz), 'num_t': len(ts),
-        'shift_seconds': t_shift, 'per_t_loop_seconds_extrapolated': t_loop,
+        'shift_seconds': t_shift, 'per_t_loop_seconds': t_loop,
         'speedup': t_loop / t_shift,
         't_evaluations_per_s': len(ts) * s / t_shift,
         'max_rel_diff_shift_vs_direct': float(rel)}
     log('affine: %s' % json.dumps(out['affine_t256']))
-    # (iii) many t of A + tB, general B: one recurrence per t on the tiles of
-    # the union pattern, against the two-stream CSR SpMM (SLQ_PAIR_TILES=0)
-    out['affine_general_b'] = general_b_leg(args.seed, m, s)
+    if not compact:
+        # (iii) many t of A + tB, general B: one recurrence per t on the
+        # tiles of the union pattern, against the two-stream CSR SpMM
+        out['affine_general_b'] = general_b_leg(args.seed, m, s)
     # (ii) 32 uneven matrices
     rng = numpy.random.default_rng(args.seed)
     sizes = numpy.round(numpy.exp(rng.uniform(numpy.log(1e3), numpy.log(2e5),
@@ -242,6 +263,18 @@ def batched_form(args):
                        use_graph=True)
     tl, il = slq_batch(ig['operators'], lanczos_degree=m, num_samples=s,
                        seed=args.seed, use_graph=False, autotune=False)
+    # plain FP64 GPU SLQ, one call per matrix (uploaded beforehand)
+    m64 = [imate_amd.Matrix(M.astype(numpy.float64)) for M in mats]
+    for M in m64:
+        M.get_linear_operator(gpu=True)
+    kw64 = dict(method='slq', gpu=True, lanczos_degree=m, min_num_samples=s,
+                max_num_samples=s, seed=args.seed, return_info=True)
+    imate_amd.logdet(m64[0], **kw64)
+    t0 = time.perf_counter()
+    r64 = [imate_amd.logdet(M, **kw64) for M in m64]
+    t64 = time.perf_counter() - t0
+    v64 = numpy.array([r[0] for r in r64])
+    e64 = numpy.array([float(r[1]['error']['absolute_error']) for r in r64])
     steps = 32 * s * m
     out['matrices32'] = {
         'sizes': [int(k) for k in sizes], 'total_rows': int(sizes.sum()),
@@ -251,12 +284,19 @@ def batched_form(args):
         'speedup': il['run_time'] / ig['run_time'],
         'graph_matvec_steps_per_s': steps / ig['run_time'],
         'loop_matvec_steps_per_s': steps / il['run_time'],
+        'plain_fp64_gpu_seconds': t64,
+        'speedup_vs_plain_fp64_gpu': t64 / ig['run_time'],
+        'max_abs_diff_fp32_vs_fp64_over_mc_error':
+            float(numpy.max(numpy.abs(tg - v64) / e64)),
         'samples_identical_graph_vs_loop':
             bool(numpy.array_equal(ig['samples'], il['samples'])),
         'autotuned_grids': ig['grids']}
     log('matrices: %s' % json.dumps({k: v for k, v in out['matrices32'].items()
                                      if k not in ('sizes', 'autotuned_grids')}))
-    print(json.dumps(out), flush=True)
+    if compact:
+        out['matrices32'] = {k: v for k, v in out['matrices32'].items()
+                             if k not in ('sizes', 'autotuned_grids')}
+    return out
 
 
 def general_b_leg(seed, m, s, n=262144, num_t=8):
@@ -354,7 +394,7 @@ def interpolator_baseline(fam, ls, ts, t_range, m, samples, slq, ci, pred,
     return res
 
 
-def surrogate_form(args):
+def surrogate_form(args, compact=False):
     """Learned form. Family: n = 65536 correlation matrices K(l) over the
     length scale l in [0.01, 0.1] with the support radius fixed at
     0.01 ln(1/0.03) (one pattern, ~247 non-zeros per row), queries
@@ -502,7 +542,27 @@ def surrogate_form(args):
         'bf16_vs_f32_max_abs_diff_per_n':
             float(numpy.max(numpy.abs(pred_all - p32)) / n)}
     log('sweep: %s' % json.dumps(out['sweep']))
-    print(json.dumps(out), flush=True)
+    if compact:
+        h = out['held_out']
+        sw = out['sweep']
+        keep = {'n': n, 'lanczos_degree': m, 'samples': 128,
+                'train': out['train'],
+                'held_out': {k: h[k] for k in (
+                    'points', 'surrogate_inside_ci', 'fallback_fraction',
+                    'median_abs_err_per_n_surrogate_vs_slq') if k in h},
+                'sweep': {k: sw[k] for k in (
+                    'queries', 'fallback_fraction', 'surrogate_queries_per_s',
+                    'plain_slq_queries_per_s', 'gain_vs_plain_slq',
+                    'inference_queries_per_s_kernel')}}
+        if 'vs_cholesky' in h:
+            keep['held_out']['vs_cholesky'] = {
+                k: h['vs_cholesky'][k] for k in (
+                    'count', 'median_abs_err_per_n_surrogate',
+                    'median_abs_err_per_n_slq')}
+        if 'interpolator' in h:
+            keep['held_out']['interpolator_inside_ci'] = h['interpolator']['inside_ci']
+        return keep
+    return out
 
 
 def gram_form(args):
@@ -553,7 +613,7 @@ def gram_form(args):
            'marchenko_pastur_wishart_mean': mp,
            'rel_diff_vs_wishart_mean': abs(float(val) - mp) / abs(mp)}
     log('gram: %s' % json.dumps(out))
-    print(json.dumps(out), flush=True)
+    return out
 
 
 def dense_form(args):
@@ -626,7 +686,7 @@ def dense_form(args):
     out['csr_tiles_8192'] = run(Mc, n, 'csr (16-row tiles)')
     out['exact_logdet_8192'] = float(ld_exact)
     out['rel_err_8192'] = abs(out['dense_8192']['logdet'] - ld_exact) / abs(ld_exact)
-    print(json.dumps(out), flush=True)
+    return out
 
 
 def main():
@@ -650,16 +710,12 @@ def main():
                           'GBps': 2 * nbytes / ms.value / 1e6}), flush=True)
         return
 
-    if args.form == 'precision':
-        return precision_form(args)
-    if args.form == 'batched':
-        return batched_form(args)
-    if args.form == 'surrogate':
-        return surrogate_form(args)
-    if args.form == 'gram':
-        return gram_form(args)
-    if args.form == 'dense':
-        return dense_form(args)
+    forms = {'precision': precision_form, 'batched': batched_form,
+             'surrogate': surrogate_form, 'gram': gram_form,
+             'dense': dense_form}
+    if args.form in forms:
+        print(json.dumps(forms[args.form](args)), flush=True)
+        return
     if args.form == 'general_b':
         print(json.dumps({'form': 'general_b', 'lanczos_degree': 30,
                           'samples': 64,
@@ -774,6 +830,19 @@ def main():
         'accuracy': accuracy,
         'cpu_baseline': cpu,
     }
+    if not args.no_forms:
+        # the other forms, compact, each against plain FP64 GPU SLQ
+        t0 = time.perf_counter()
+        forms = {}
+        for name, fn in (('precision', precision_form),
+                         ('batched', batched_form),
+                         ('surrogate', surrogate_form)):
+            t1 = time.perf_counter()
+            forms[name] = fn(args, compact=True)
+            forms[name]['leg_seconds'] = time.perf_counter() - t1
+            log('%s leg done in %.1fs' % (name, forms[name]['leg_seconds']))
+        forms['seconds'] = time.perf_counter() - t0
+        out['forms'] = forms
     print(json.dumps(out), flush=True)
 
 

@@ -53,6 +53,8 @@ SIGNATURES = {
     'slq_op_create_dense': (ctypes.c_int, [_i64, _i64, _P, ctypes.c_int, _P]),
     'slq_op_destroy': (ctypes.c_int, [_P]),
     'slq_op_set_values': (ctypes.c_int, [_P, _P]),
+    'slq_op_family_attach': (ctypes.c_int, [_P, _P, _P, _P]),
+    'slq_op_family_values': (ctypes.c_int, [_P, _f64]),
     'slq_op_create_sparse_gaussian': (ctypes.c_int, [_i64, _i64, _i32, _i64,
                                                      ctypes.c_int, _P]),
     'slq_op_get_csr': (ctypes.c_int, [_P, _P, _P, _P]),

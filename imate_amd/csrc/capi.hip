@@ -85,6 +85,7 @@ MatrixStore::~MatrixStore()
     if (trans.data) (void) hipFree(trans.data);
     if (data_f32) (void) hipFree(data_f32);
     if (trans_data_f32) (void) hipFree(trans_data_f32);
+    if (family) (void) hipFree(family);
 }
 
 Workspace::~Workspace()
@@ -1929,6 +1930,65 @@ int slq_op_create_dense(int64_t nrows, int64_t ncols, const void* data, int dtyp
     return SLQ_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// value_k = w_k exp(-d_k / scale) + c_k, rounded once to the matrix type.
+template <typename T>
+__global__ __launch_bounds__(256) void family_kernel(const double* __restrict__ f, int64_t nnz,
+                                                     double scale, T* __restrict__ out)
+{
+    const int64_t stride = (int64_t) gridDim.x * blockDim.x;
+    for (int64_t k = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; k < nnz; k += stride)
+        out[k] = (T) (f[nnz + k] * exp(-f[k] / scale) + f[2 * nnz + k]);
+}
+
+int family_fill(hipStream_t s, const double* f, int64_t nnz, double scale, int dtype,
+                void* out)
+{
+    if (nnz <= 0) return SLQ_OK;
+    const int64_t g = (nnz + 255) / 256;
+    const dim3 grid((unsigned) (g < 8192 ? g : 8192));
+    if (dtype == SLQ_F64)
+        hipLaunchKernelGGL(family_kernel<double>, grid, dim3(256), 0, s, f, nnz, scale,
+                           (double*) out);
+    else
+        hipLaunchKernelGGL(family_kernel<float>, grid, dim3(256), 0, s, f, nnz, scale,
+                           (float*) out);
+    SLQ_CHECK(hipGetLastError());
+    return SLQ_OK;
+}
+
+// After new CSR values of a matrix handle: the tiled layout is refilled on
+// the device and the derived forms are rebuilt from the new values on
+// their next use.
+int values_changed(slq_op* op)
+{
+    MatrixStore& M = *op->A;
+    SLQ_TRY(tiles_refill(op->stream, M.tiles, M.csr.data, M.csr.dtype));
+    if (M.data_f32) (void) hipFree(M.data_f32);
+    M.data_f32 = nullptr;
+    if (M.trans_data_f32) (void) hipFree(M.trans_data_f32);
+    M.trans_data_f32 = nullptr;
+    if (M.has_trans)
+    {
+        (void) hipFree(M.trans.indptr);
+        (void) hipFree(M.trans.indices);
+        (void) hipFree(M.trans.data);
+        M.trans = DevCSR();
+        M.has_trans = false;
+    }
+    free_tiles(M.trans_tiles);
+    M.trans_tiles_tried = false;
+    SLQ_CHECK(hipStreamSynchronize(op->stream));
+    return SLQ_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
 int slq_op_set_values(slq_op* op, const void* data)
 {
     if (!op || op->kind != OP_MATRIX || (!data && op->A->csr.nnz > 0))
@@ -1972,24 +2032,53 @@ int slq_op_set_values(slq_op* op, const void* data)
     }
     else if (M.csr.nnz > 0)
         SLQ_CHECK(hipMemcpy(M.csr.data, data, vb * M.csr.nnz, hipMemcpyHostToDevice));
-    SLQ_TRY(tiles_refill(op->stream, M.tiles, M.csr.data, M.csr.dtype));
-    // derived forms are rebuilt from the new values on their next use
-    if (M.data_f32) (void) hipFree(M.data_f32);
-    M.data_f32 = nullptr;
-    if (M.trans_data_f32) (void) hipFree(M.trans_data_f32);
-    M.trans_data_f32 = nullptr;
-    if (M.has_trans)
+    return values_changed(op);
+}
+
+int slq_op_family_attach(slq_op* op, const double* d, const double* w, const double* c)
+{
+    if (!op || op->kind != OP_MATRIX || op->A->dense || !op->A->csc_order.empty() ||
+        (op->A->csr.nnz > 0 && (!d || !w || !c)))
     {
-        (void) hipFree(M.trans.indptr);
-        (void) hipFree(M.trans.indices);
-        (void) hipFree(M.trans.data);
-        M.trans = DevCSR();
-        M.has_trans = false;
+        set_error("slq_op_family_attach: a CSR matrix handle and its d, w, c arrays are "
+                  "required");
+        return SLQ_E_ARG;
     }
-    free_tiles(M.trans_tiles);
-    M.trans_tiles_tried = false;
-    SLQ_CHECK(hipStreamSynchronize(op->stream));
+    MatrixStore& M = *op->A;
+    if (M.tiles.R > 0 && !M.tiles.src)
+    {
+        set_error("slq_op_family_attach: the matrix has duplicate entries or too many "
+                  "non-zeros for an in-place value refresh");
+        return SLQ_E_ARG;
+    }
+    DeviceGuard guard(op->device);
+    const int64_t nnz = M.csr.nnz;
+    if (!M.family)
+        SLQ_CHECK(hipMalloc(&M.family, sizeof(double) * 3 * (size_t) (nnz > 0 ? nnz : 1)));
+    if (nnz > 0)
+    {
+        SLQ_CHECK(hipMemcpy(M.family, d, sizeof(double) * nnz, hipMemcpyHostToDevice));
+        SLQ_CHECK(hipMemcpy(M.family + nnz, w, sizeof(double) * nnz, hipMemcpyHostToDevice));
+        SLQ_CHECK(hipMemcpy(M.family + 2 * nnz, c, sizeof(double) * nnz,
+                            hipMemcpyHostToDevice));
+    }
     return SLQ_OK;
+}
+
+int slq_op_family_values(slq_op* op, double scale)
+{
+    if (!op || op->kind != OP_MATRIX || !op->A->family || !(scale > 0.0))
+    {
+        set_error("slq_op_family_values: a matrix handle with an attached family and a "
+                  "positive scale are required");
+        return SLQ_E_ARG;
+    }
+    DeviceGuard guard(op->device);
+    MatrixStore& M = *op->A;
+    // work on other handles sharing the matrix (A + tI) may still read it
+    SLQ_CHECK(hipDeviceSynchronize());
+    SLQ_TRY(family_fill(op->stream, M.family, M.csr.nnz, scale, M.csr.dtype, M.csr.data));
+    return values_changed(op);
 }
 
 int slq_op_destroy(slq_op* op)

@@ -941,18 +941,21 @@ __global__ __launch_bounds__(kThreads) void mfma_spmm_kernel(
 
 // The matrix-core SpMM with S wavefronts per tile (S = 2 or 4): the waves
 // of a team split the tile's slab list into S contiguous ranges of eight-slab
-// pairs, so a workgroup holds 8 / S tiles instead of 8 and the operand rows
-// gathered by the tiles in flight on an XCD take 1 / S of the L2 that one
-// wave per tile needs, while every slab value and every gathered row is
-// still loaded once. Each wave writes the partial products of the rows it
-// does not finish to LDS; after a workgroup barrier wave j of a team adds
-// the S partials of row slices m = j (4 / S) .. (j + 1) (4 / S) - 1 in the
-// fixed order of the waves and runs the epilogue on them, writing the
-// reduction term of its rows as partial row t S + j (tile_sum_kernel adds
-// them in order, so results are deterministic for a given S). Tiles are
-// fetched per workgroup (8 / S at a time) from the XCD's in-order ticket, so
-// every wave of a workgroup runs the same number of iterations and reaches
-// the same barriers.
+// pairs, so the tiles in flight on an XCD are 1 / S as many as with one wave
+// per tile and the operand rows they gather take 1 / S of the L2 (measured at
+// n = 2^20: HBM reads per launch 2.96 GB with one wave, 2.02 GB with two,
+// 1.78 GB with four), while every slab value and every gathered row is still
+// loaded once. Each wave writes the partial products of the row slices it
+// does not finish to LDS; wave j of a team then adds the S partials of
+// slices m = j (4 / S) .. (j + 1) (4 / S) - 1 in the fixed order of the
+// waves and runs the epilogue on them, writing the reduction term of its
+// rows as partial row t S + j (tile_sum_kernel adds them in order, so
+// results are deterministic for a given S). The waves of a team meet at an
+// LDS arrival counter rather than a workgroup barrier, so teams never wait
+// for each other: wave 0 of a team takes the team's next tile from the
+// XCD's in-order ticket one tile ahead and publishes it with its partials,
+// and a second counter of waves done reading a tile's partials keeps a wave
+// from overwriting them before its whole team has read them.
 template <typename T, bool NT, int S>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4)))
 void mfma_spmm_split_kernel(
@@ -964,10 +967,12 @@ void mfma_spmm_split_kernel(
     using V4 = typename Mfma16<T>::V4;
     constexpr int kTeams = kWavesPerWG / S;
     constexpr int MS = 4 / S;   // row slices finished per wave
-    // per team: for every slice m and every wave j not finishing it, the four
-    // values q of each lane
+    // per team: for every slice m and every wave j not finishing it, the
+    // four values q of each lane
     __shared__ T part[kTeams][4 * (S - 1)][4][kWave];
-    __shared__ int64_t s_first;
+    __shared__ int64_t s_tile[kTeams][2];
+    __shared__ unsigned int s_arrive[kTeams];
+    __shared__ unsigned int s_read[kTeams];
     const int lane = threadIdx.x % kWave;
     const int wave = threadIdx.x / kWave;
     const int team = wave / S;
@@ -987,18 +992,25 @@ void mfma_spmm_split_kernel(
     for (int q = 0; q < 4; ++q) sq[q] = a.xscale ? (T) a.xscale[4 * i0 + q] : T(1);
     const T tt = (T) a.t;
     auto slot = [](int m, int src) { return m * (S - 1) + (src < m / MS ? src : src - 1); };
-    for (;;)
+    if (j == 0 && lane == 0)
     {
-        if (threadIdx.x == 0) s_first = b + (int64_t) atomicAdd(ticket, (unsigned int) kTeams);
-        __syncthreads();
-        const int64_t first = s_first;
-        if (first >= e) break;   // the same value in every wave of the workgroup
-        const int64_t t = first + team;
-        const bool valid = t < e;
+        s_arrive[team] = 0u;
+        s_read[team] = 0u;
+        s_tile[team][0] = b + (int64_t) atomicAdd(ticket, 1u);
+    }
+    __syncthreads();   // the only workgroup barrier: counters and first tiles
+    for (unsigned int k = 0;; ++k)
+    {
+        const int par = (int) (k & 1u);
+        const int64_t t = s_tile[team][par];
+        if (t >= e) break;   // the same value in every wave of the team
+        // wave 0 of the team asks for the team's next tile now; the answer
+        // is published with its partials below
+        unsigned int nxt = 0;
+        if (j == 0 && lane == 0) nxt = atomicAdd(ticket, 1u);
         V4 acc[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q) acc[q] = V4{T(0), T(0), T(0), T(0)};
-        if (valid)
         {
             const int64_t beg0 = tptr[t];
             const int64_t npt = (tptr[t + 1] - beg0) / 8;
@@ -1040,7 +1052,14 @@ void mfma_spmm_split_kernel(
                 }
             }
         }
-        // partial products of the slices other waves finish
+        // the team has read the partials of tile k - 1 (its waves run the
+        // same code, so this wait ends)
+        const unsigned int read_target = (unsigned int) S * k;
+        while (__hip_atomic_load(&s_read[team], __ATOMIC_ACQUIRE,
+                                 __HIP_MEMORY_SCOPE_WORKGROUP) < read_target)
+            __builtin_amdgcn_s_sleep(1);
+        // partial products of the slices other waves finish, the next tile,
+        // then the arrival of this wave at tile k
 #pragma unroll
         for (int m = 0; m < 4; ++m)
             if (m / MS != j)
@@ -1048,50 +1067,65 @@ void mfma_spmm_split_kernel(
 #pragma unroll
                 for (int q = 0; q < 4; ++q) part[team][slot(m, j)][q][lane] = acc[q][m];
             }
-        __syncthreads();
-        if (valid)
+        if (j == 0 && lane == 0) s_tile[team][par ^ 1] = b + (int64_t) nxt;
+        if (lane == 0)
+            __hip_atomic_fetch_add(&s_arrive[team], 1u, __ATOMIC_RELEASE,
+                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+        // every wave of the team has written its partials of tile k (the
+        // other waves of the team run the same code, so this wait ends)
+        const unsigned int target = (unsigned int) S * (k + 1u);
+        while (__hip_atomic_load(&s_arrive[team], __ATOMIC_ACQUIRE,
+                                 __HIP_MEMORY_SCOPE_WORKGROUP) < target)
+            __builtin_amdgcn_s_sleep(1);
+        // the sums of this wave's slices, waves 0 .. S-1 in order (its own
+        // partial in place), then the partials are released to the team
+        T y[MS][4];
+#pragma unroll
+        for (int mm = 0; mm < MS; ++mm)
         {
-            double red[4] = {0.0, 0.0, 0.0, 0.0};
-            T sub_c[4], sub_s[4];
+            const int m = j * MS + mm;
 #pragma unroll
             for (int q = 0; q < 4; ++q)
             {
-                sub_c[q] = Sub ? (T) a.sub_coef[4 * i0 + q] : T(0);
-                sub_s[q] = Sub ? (T) a.sub_scale[4 * i0 + q] : T(0);
-            }
+                T v = j == 0 ? acc[q][m] : part[team][slot(m, 0)][q][lane];
 #pragma unroll
-            for (int mm = 0; mm < MS; ++mm)
+                for (int src = 1; src < S; ++src)
+                    v += src == j ? acc[q][m] : part[team][slot(m, src)][q][lane];
+                y[mm][q] = v;
+            }
+        }
+        if (lane == 0)
+            __hip_atomic_fetch_add(&s_read[team], 1u, __ATOMIC_RELEASE,
+                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+        double red[4] = {0.0, 0.0, 0.0, 0.0};
+        T sub_c[4], sub_s[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+        {
+            sub_c[q] = Sub ? (T) a.sub_coef[4 * i0 + q] : T(0);
+            sub_s[q] = Sub ? (T) a.sub_scale[4 * i0 + q] : T(0);
+        }
+#pragma unroll
+        for (int mm = 0; mm < MS; ++mm)
+        {
+            const int m = j * MS + mm;
+            const int64_t row = t * 16 +
+                                (Mfma16<T>::kRowStride == 4 ? kq + 4 * m : 4 * kq + m);
+            if (row < a.nrows)
+                spmm_epilogue4<T, NT>(a, row, i0, y[mm], sq, tt, sub_c, sub_s, X, Y, Vout,
+                                      Sub, red);
+        }
+        if (a.red_mode)
+        {
+            double mine = 0.0;
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
             {
-                const int m = j * MS + mm;
-                T y[4];
-#pragma unroll
-                for (int q = 0; q < 4; ++q)
-                {
-                    // waves 0 .. S-1 in order, this wave's own partial in place
-                    T v = j == 0 ? acc[q][m] : part[team][slot(m, 0)][q][lane];
-#pragma unroll
-                    for (int src = 1; src < S; ++src)
-                        v += src == j ? acc[q][m] : part[team][slot(m, src)][q][lane];
-                    y[q] = v;
-                }
-                const int64_t row = t * 16 +
-                                    (Mfma16<T>::kRowStride == 4 ? kq + 4 * m : 4 * kq + m);
-                if (row < a.nrows)
-                    spmm_epilogue4<T, NT>(a, row, i0, y, sq, tt, sub_c, sub_s, X, Y, Vout,
-                                          Sub, red);
+                const double r1 = red[q] + __shfl_xor(red[q], 16);
+                const double r2 = r1 + __shfl_xor(r1, 32);
+                if (q == kq) mine = r2;
             }
-            if (a.red_mode)
-            {
-                double mine = 0.0;
-#pragma unroll
-                for (int q = 0; q < 4; ++q)
-                {
-                    const double r1 = red[q] + __shfl_xor(red[q], 16);
-                    const double r2 = r1 + __shfl_xor(r1, 32);
-                    if (q == kq) mine = r2;
-                }
-                a.rb.tile_partial[(t * S + j) * kWave + 4 * i0 + kq] = mine;
-            }
+            a.rb.tile_partial[(t * S + j) * kWave + 4 * i0 + kq] = mine;
         }
     }
     unsigned int d = 0;

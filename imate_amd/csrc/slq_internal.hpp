@@ -115,6 +115,9 @@ struct MatrixStore {
     // made from CSC arrays: CSC position of each CSR entry (host), so that
     // new values can be given in CSC order; empty otherwise
     std::vector<int64_t> csc_order;
+    // value family (slq_op_family_attach): per-entry d, w, c in CSR order,
+    // FP64, [3][nnz]; values w exp(-d / scale) + c are made on the device
+    double* family = nullptr;
     ~MatrixStore();
 };
 

@@ -87,6 +87,21 @@ class DeviceOperator(object):
         _lib.check(lib.slq_op_set_values(self.handle, _lib.ptr(data)),
                    'slq_op_set_values')
 
+    def family_attach(self, d, w, c):
+        """Per-entry coefficients of a value family (CSR value order)."""
+        lib = _lib.load()
+        arrs = [numpy.ascontiguousarray(x, dtype=numpy.float64)
+                for x in (d, w, c)]
+        _lib.check(lib.slq_op_family_attach(self.handle, *[_lib.ptr(x)
+                                                           for x in arrs]),
+                   'slq_op_family_attach')
+
+    def family_values(self, scale):
+        """Values w exp(-d / scale) + c made on the device."""
+        lib = _lib.load()
+        _lib.check(lib.slq_op_family_values(self.handle, float(scale)),
+                   'slq_op_family_values')
+
     def get_device_properties(self):
         return _lib.device_properties()
 
@@ -277,6 +292,23 @@ class AffineMatrixFunction(LinearOperator):
         self.A, data = _new_values(self.A, data)
         if self.initialized_on_gpu:
             self._A_dev.set_values(data)
+
+    def attach_family(self, d, w, c):
+        """A value family of A (one pattern; per-entry coefficients d, w, c
+        in A's CSR value order): ``set_family_scale(l)`` then makes the
+        values w exp(-d / l) + c on the device, with no host values and no
+        upload. No counterpart in imate."""
+        if not (scipy.sparse.isspmatrix_csr(self.A) and
+                self.A.has_sorted_indices):
+            raise ValueError('a value family needs A as a CSR matrix with '
+                             'sorted indices.')
+        self.initialize(True, 0)
+        self._A_dev.family_attach(d, w, c)
+
+    def set_family_scale(self, scale):
+        """New device values of A from its family at ``scale``. The host
+        copy of A keeps its old values (the device copy is the operator)."""
+        self._A_dev.family_values(scale)
 
     def get_num_rows(self):
         return self.A.shape[0]

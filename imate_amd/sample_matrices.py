@@ -195,6 +195,15 @@ class CorrelationFamily(object):
         self._w = (1.0 - r)**4 * (4.0 * r + 1.0)
         self.nnz = len(self.indices)
 
+    def coefficients(self):
+        """(d, w, c) in CSR value order with every value equal to
+        w exp(-d / l) + c: d the distance and w the Wendland factor off the
+        diagonal, c = 1 + jitter on it (d = w = 0 there); for a device value
+        family (``AffineMatrixFunction.attach_family``)."""
+        w = numpy.where(self._diag, 0.0, self._w)
+        c = numpy.where(self._diag, 1.0 + self.jitter, 0.0)
+        return self._d, w, c
+
     def values(self, scale):
         v = numpy.exp(-self._d / scale) * self._w
         v[self._diag] = 1.0 + self.jitter

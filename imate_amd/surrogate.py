@@ -157,11 +157,12 @@ class LogdetSurrogate(object):
     # ------------------------------------------------------------ SLQ
 
     def _operator(self, scale):
-        vals = self.family.values(scale)
+        # one upload of the pattern and the family's coefficients; every
+        # member's values are then made on the device
         if self._op is None:
             self._op = AffineMatrixFunction(self.family.matrix(scale))
-        else:
-            self._op.set_values(vals)
+            self._op.attach_family(*self.family.coefficients())
+        self._op.set_family_scale(scale)
         return self._op
 
     def solve(self, scales, t, seed=None):

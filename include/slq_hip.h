@@ -132,6 +132,17 @@ int slq_op_set_values(slq_op* op, const void* data);
  * in the reference (imate's sample matrices are host-built,
  * imate/sample_matrices/toeplitz.py, correlation_matrix.py): a 2^20 x 2^18
  * matrix at 1 % density has 2.75e9 entries, too many to build on the host. */
+/* Value family of a CSR matrix handle: per-entry FP64 coefficients d, w, c
+ * (in the CSR value order of slq_op_create_csr) are uploaded once; then
+ * slq_op_family_values(op, scale) sets every value to
+ * w_k exp(-d_k / scale) + c_k on the device (rounded once to the matrix
+ * type) and refreshes the handle as slq_op_set_values does, with no host
+ * values and no upload. For the kernel-matrix families of the learned
+ * surrogate (values exp(-d / l) w(d) of one pattern over the length scale
+ * l). No reference counterpart: imate builds every member on the host. */
+int slq_op_family_attach(slq_op* op, const double* d, const double* w, const double* c);
+int slq_op_family_values(slq_op* op, double scale);
+
 int slq_op_create_sparse_gaussian(int64_t nrows, int64_t ncols, int32_t per_row,
                                   int64_t seed, int dtype, slq_op** out);
 

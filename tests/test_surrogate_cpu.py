@@ -66,3 +66,16 @@ def test_stratified_queries_cover_every_slice():
     v = (numpy.log(ts) - numpy.log(1e-4)) / (numpy.log(1e2) - numpy.log(1e-4))
     for row in v:
         assert sorted(numpy.floor(row * 10).astype(int)) == list(range(10))
+
+
+def test_family_coefficients_reproduce_values():
+    """The device family's per-entry (d, w, c) give the host values by the
+    kernel's formula w exp(-d / l) + c, diagonal included."""
+    import numpy
+    from imate_amd.sample_matrices import CorrelationFamily
+    fam = CorrelationFamily(1500, support=0.07, seed=4)
+    d, w, c = fam.coefficients()
+    assert d.shape == w.shape == c.shape == (fam.nnz,)
+    for scale in (0.01, 0.033, 0.1):
+        v = w * numpy.exp(-d / scale) + c
+        assert numpy.array_equal(v, fam.values(scale))
