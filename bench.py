@@ -322,11 +322,16 @@ def general_b_leg(seed, m, s, n=262144, num_t=8):
               min_num_samples=s, max_num_samples=s, seed=seed)
     res = {'n': n, 'nnz_a': int(A.nnz), 'nnz_b': int(B.nnz), 'num_t': num_t}
     vals = {}
-    for path, env in (('pair_tiles', None), ('csr', '0')):
-        if env is None:
-            os.environ.pop('SLQ_PAIR_TILES', None)
-        else:
-            os.environ['SLQ_PAIR_TILES'] = env
+    # union tiles with the t side by side (default), one t after the other,
+    # and the two-stream CSR SpMM
+    for path, pair, conc in (('pair_tiles', None, None),
+                             ('pair_tiles_sequential_t', None, '0'),
+                             ('csr', '0', '0')):
+        for var, val in (('SLQ_PAIR_TILES', pair), ('SLQ_CONCURRENT_T', conc)):
+            if val is None:
+                os.environ.pop(var, None)
+            else:
+                os.environ[var] = val
         imate_amd.logdet(Aop, **kw)            # builds, warms up
         t0 = time.perf_counter()
         vals[path] = numpy.asarray(imate_amd.logdet(Aop, **kw))
@@ -334,7 +339,12 @@ def general_b_leg(seed, m, s, n=262144, num_t=8):
         res[path] = {'seconds': sec,
                      'matvec_steps_per_s': num_t * s * m / sec}
     os.environ.pop('SLQ_PAIR_TILES', None)
+    os.environ.pop('SLQ_CONCURRENT_T', None)
     res['speedup'] = res['csr']['seconds'] / res['pair_tiles']['seconds']
+    res['speedup_t_side_by_side'] = (res['pair_tiles_sequential_t']['seconds'] /
+                                     res['pair_tiles']['seconds'])
+    res['t_side_by_side_bitwise_equal'] = bool(numpy.array_equal(
+        vals['pair_tiles'], vals['pair_tiles_sequential_t']))
     res['max_rel_diff'] = float(numpy.max(numpy.abs(vals['pair_tiles'] -
                                                     vals['csr']) /
                                           numpy.abs(vals['csr'])))
@@ -719,7 +729,9 @@ def main():
     if args.form == 'general_b':
         print(json.dumps({'form': 'general_b', 'lanczos_degree': 30,
                           'samples': 64,
-                          'affine_general_b': general_b_leg(args.seed, 30, 64)}),
+                          'affine_general_b': general_b_leg(args.seed, 30, 64),
+                          'affine_general_b_small': general_b_leg(
+                              args.seed, 30, 64, n=16384, num_t=16)}),
               flush=True)
         return
 

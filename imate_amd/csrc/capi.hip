@@ -217,6 +217,7 @@ int build_tiles(int64_t nrows, int64_t ncols, const int64_t* indptr,
 
 PairTiles::~PairTiles()
 {
+    if (borrowed) return;
     free_tiles(tiles);
     if (src_a) (void) hipFree(src_a);
     if (src_b) (void) hipFree(src_b);
@@ -933,6 +934,13 @@ struct Plan {
            off_samples = 0, off_params = 0, off_scratch = 0, total = 0;
     size_t scratch_bytes = 0;
     char* base = nullptr;   // device workspace the offsets are taken from
+    // recurrences of several t of A + tB side by side: per lane alpha, beta,
+    // sizes, recurrence scratch and the pair-tile values of its t (both
+    // directions for Golub-Kahn), lane c at off_lanes + c * lane_stride
+    int lanes = 0;
+    size_t off_lanes = 0, lane_stride = 0;
+    size_t lane_alpha = 0, lane_beta = 0, lane_sizes = 0, lane_vals = 0, lane_vals_t = 0,
+           lane_scratch = 0;
 };
 
 size_t align256(size_t b) { return (b + 255) & ~(size_t) 255; }
@@ -958,6 +966,47 @@ Plan make_plan(slq_op* op, int gram, int m, int ortho, int q, int vbytes)
     o += p.scratch_bytes;
     p.total = o;
     return p;
+}
+
+// Lanes for the t of an A + tB estimate with q inquiries: 0 (one recurrence
+// after the other) unless the union tiles exist in every direction the
+// recurrence multiplies by, q > 1 and SLQ_CONCURRENT_T is not 0; at most 8,
+// and no more than fit 8 GB of lane workspace.
+int t_lanes(const slq_op* op, int gram, int q, const Plan& p)
+{
+    const char* env = std::getenv("SLQ_CONCURRENT_T");   // read per estimate
+    const bool on = !(env && std::atoi(env) == 0);
+    if (!on || op->kind != OP_AFFINE_B || q < 2 || op->ab.tiles.R == 0 ||
+        (gram && op->ab_t.tiles.R == 0))
+        return 0;
+    const size_t per = 2 * align256(sizeof(double) * kBlock * p.m) +
+                       align256(sizeof(int32_t) * kBlock) +
+                       align256((size_t) p.vbytes * op->ab.tiles.nslabs * op->ab.tiles.R) +
+                       (gram ? align256((size_t) p.vbytes * op->ab_t.tiles.nslabs *
+                                        op->ab_t.tiles.R) : 0) +
+                       p.scratch_bytes;
+    int lanes = q < 8 ? q : 8;
+    while (lanes > 1 && per * (size_t) lanes > ((size_t) 8 << 30)) --lanes;
+    return lanes > 1 ? lanes : 0;
+}
+
+// Adds the lane regions to a plan.
+void plan_lanes(const slq_op* op, int gram, int lanes, Plan& p)
+{
+    p.lanes = lanes;
+    if (lanes <= 0) return;
+    size_t o = 0;
+    p.lane_alpha = o;   o += align256(sizeof(double) * kBlock * p.m);
+    p.lane_beta = o;    o += align256(sizeof(double) * kBlock * p.m);
+    p.lane_sizes = o;   o += align256(sizeof(int32_t) * kBlock);
+    p.lane_vals = o;
+    o += align256((size_t) p.vbytes * op->ab.tiles.nslabs * op->ab.tiles.R);
+    p.lane_vals_t = o;
+    if (gram) o += align256((size_t) p.vbytes * op->ab_t.tiles.nslabs * op->ab_t.tiles.R);
+    p.lane_scratch = o; o += p.scratch_bytes;
+    p.lane_stride = o;
+    p.off_lanes = p.total;
+    p.total += o * (size_t) lanes;
 }
 
 template <typename U>
@@ -1048,6 +1097,57 @@ int recurrence_block(slq_op* op, const Plan& p, double t, int gram, double tol,
     ra.beta = at<double>(p, p.off_beta);
     ra.sizes = at<int32_t>(p, p.off_sizes);
     return run_recurrence(ra, at<void>(p, p.off_scratch), p.scratch_bytes);
+}
+
+// A view of an A + tB handle for one lane: the same matrices and launch
+// settings, the lane's stream, and the handle's pair tiles with the lane's
+// own value buffers (its t's values are gathered there), so the recurrences
+// of several t run side by side without sharing writable state.
+struct LaneOp {
+    slq_op op;
+    LaneOp(const slq_op* parent, hipStream_t s, void* vals, void* vals_t)
+    {
+        op.kind = parent->kind;
+        op.A = parent->A;
+        op.B = parent->B;
+        op.stream = s;
+        op.device = parent->device;
+        op.spmm_grid = parent->spmm_grid;
+        op.stream_grid = parent->stream_grid;
+        op.spmm_split = parent->spmm_split;
+        borrow(op.ab, parent->ab, vals);
+        borrow(op.ab_t, parent->ab_t, vals_t);
+    }
+    static void borrow(PairTiles& dst, const PairTiles& src, void* vals)
+    {
+        dst.tiles = src.tiles;
+        dst.tiles.vals = vals;
+        dst.tiles.vals_f32 = (float*) vals;
+        dst.src_a = src.src_a;
+        dst.src_b = src.src_b;
+        dst.tried = src.tried;
+        dst.borrowed = true;
+    }
+};
+
+// The recurrence of lane c of a plan (on the lane view's stream).
+int recurrence_lane(slq_op* lop, const Plan& p, int c, double t, int gram, double tol,
+                    int ortho, int precision)
+{
+    const size_t o = p.off_lanes + (size_t) c * p.lane_stride;
+    RecurrenceArgs ra;
+    ra.op = lop;
+    ra.t = t;
+    ra.gram = gram;
+    ra.m = p.m;
+    ra.tol = tol;
+    ra.ortho = ortho;
+    ra.precision = precision;
+    ra.X0 = at<void>(p, p.off_X0);
+    ra.alpha = at<double>(p, o + p.lane_alpha);
+    ra.beta = at<double>(p, o + p.lane_beta);
+    ra.sizes = at<int32_t>(p, o + p.lane_sizes);
+    return run_recurrence(ra, at<void>(p, o + p.lane_scratch), p.scratch_bytes);
 }
 
 struct DeviceGuard {
@@ -2085,9 +2185,11 @@ int slq_op_destroy(slq_op* op)
 {
     if (!op) return SLQ_OK;
     if (op->stream) (void) hipStreamSynchronize(op->stream);
+    for (hipStream_t ls : op->lane_streams) (void) hipStreamSynchronize(ls);
     prof_collect(op);
     for (hipEvent_t e : op->prof.pool) (void) hipEventDestroy(e);
     if (op->stream) (void) hipStreamDestroy(op->stream);
+    for (hipStream_t ls : op->lane_streams) (void) hipStreamDestroy(ls);
     delete op;
     return SLQ_OK;
 }
@@ -2143,6 +2245,83 @@ int check_estimate(const char* who, const slq_op* op, const EstimateArgs& a)
     return SLQ_OK;
 }
 
+// The recurrences and quadratures of one probe block for every unconverged
+// t of an A + tB estimate, p.lanes at a time side by side: each t of a group
+// runs on its own lane (stream, buffers, pair-tile values) after the probes
+// are made on the handle's stream, and the handle's stream waits for the
+// whole group before the next one and before the samples are read. Every
+// t's kernels and inputs are those of the one-after-the-other path, so the
+// samples are the same bit for bit.
+int lanes_block(slq_op* op, const Plan& p, const EstimateArgs& a, int cnt, double nq,
+                bool count_steps, std::vector<int32_t>& block_sizes, std::vector<char>& ran)
+{
+    while ((int) op->lane_streams.size() < p.lanes)
+    {
+        hipStream_t ls = nullptr;
+        SLQ_CHECK(hipStreamCreateWithFlags(&ls, hipStreamNonBlocking));
+        op->lane_streams.push_back(ls);
+    }
+    std::vector<int> todo;
+    for (int j = 0; j < a.q; ++j)
+        if (!a.converged[j]) todo.push_back(j);
+    hipEvent_t fork = nullptr;
+    std::vector<hipEvent_t> join(p.lanes, nullptr);
+    int r = SLQ_OK;
+    hipError_t ee = hipEventCreateWithFlags(&fork, hipEventDisableTiming);
+    for (auto& e : join)
+        if (ee == hipSuccess) ee = hipEventCreateWithFlags(&e, hipEventDisableTiming);
+    if (ee != hipSuccess) r = hip_status(ee, "hipEventCreate", __FILE__, __LINE__);
+    double* samples_dev = at<double>(p, p.off_samples);
+    for (size_t g = 0; r == SLQ_OK && g < todo.size(); g += p.lanes)
+    {
+        const int n_in = (int) std::min<size_t>(p.lanes, todo.size() - g);
+        if (hipEventRecord(fork, op->stream) != hipSuccess)
+        {
+            r = SLQ_E_HIP;
+            break;
+        }
+        for (int c = 0; c < n_in && r == SLQ_OK; ++c)
+        {
+            const int j = todo[g + c];
+            hipStream_t ls = op->lane_streams[c];
+            const size_t o = p.off_lanes + (size_t) c * p.lane_stride;
+            LaneOp lane(op, ls, at<void>(p, o + p.lane_vals), at<void>(p, o + p.lane_vals_t));
+            if (hipStreamWaitEvent(ls, fork, 0) != hipSuccess) r = SLQ_E_HIP;
+            if (r == SLQ_OK)
+                r = recurrence_lane(&lane.op, p, c, a.params[j], a.gram, a.tol,
+                                    a.orthogonalize, a.precision);
+            if (r == SLQ_OK)
+                r = run_quadrature(ls, at<double>(p, o + p.lane_alpha),
+                                   at<double>(p, o + p.lane_beta),
+                                   at<int32_t>(p, o + p.lane_sizes), cnt, a.m, a.gram, a.func,
+                                   a.func_params, a.exponent, nq, nullptr, a.q, 0, j, a.q,
+                                   samples_dev, nullptr, nullptr);
+            if (r == SLQ_OK && count_steps)
+            {
+                if (hipMemcpyAsync(block_sizes.data() + (size_t) j * kBlock,
+                                   at<int32_t>(p, o + p.lane_sizes), sizeof(int32_t) * kBlock,
+                                   hipMemcpyDeviceToHost, ls) != hipSuccess)
+                    r = SLQ_E_HIP;
+                ran[j] = 1;
+            }
+            if (r == SLQ_OK && (hipEventRecord(join[c], ls) != hipSuccess ||
+                                hipStreamWaitEvent(op->stream, join[c], 0) != hipSuccess))
+                r = SLQ_E_HIP;
+        }
+    }
+    if (r != SLQ_OK)
+    {
+        if (g_last_error.empty()) set_error("A + tB lanes: HIP call failed");
+        // no lane may still write the workspace when the caller returns
+        for (hipStream_t ls : op->lane_streams) (void) hipStreamSynchronize(ls);
+    }
+    // otherwise the handle's stream has waited for every lane that ran
+    if (fork) (void) hipEventDestroy(fork);
+    for (auto& e : join)
+        if (e) (void) hipEventDestroy(e);
+    return r;
+}
+
 // The sampling loop of one estimate on the workspace p.base (sized by p),
 // following cu_trace_estimator.cu:197-310 with one device: probes in blocks
 // of 64 in sample order, one host synchronisation per block (the block's
@@ -2187,24 +2366,29 @@ int estimate_loop(slq_op* op, const Plan& p, EstimateArgs a)
         const int cnt = (int) std::min<int64_t>(bs, a.max_s - s0);
         SLQ_TRY(make_probes(op, p, base, s0, cnt, vdt));
         std::vector<char> ran(count_steps ? required : 0, 0);
-        for (int j = 0; j < required; ++j)
-        {
-            if (!relation && converged[j]) continue;   // masked below anyway
-            const double t = op->kind != OP_MATRIX ? a.params[j] : 0.0;
-            SLQ_TRY(recurrence_block(op, p, t, a.gram, a.tol, a.orthogonalize, a.precision));
-            SLQ_TRY(run_quadrature(op->stream, at<double>(p, p.off_alpha),
-                                   at<double>(p, p.off_beta), at<int32_t>(p, p.off_sizes),
-                                   cnt, a.m, a.gram, a.func, a.func_params, a.exponent, nq,
-                                   relation ? params_dev : nullptr, q, relation ? 1 : 0, j,
-                                   q, at<double>(p, p.off_samples), nullptr, nullptr));
-            if (count_steps)
+        if (p.lanes > 0)
+            SLQ_TRY(lanes_block(op, p, a, cnt, nq, count_steps, block_sizes, ran));
+        else
+            for (int j = 0; j < required; ++j)
             {
-                SLQ_CHECK(hipMemcpyAsync(block_sizes.data() + (size_t) j * kBlock,
-                                         at<int32_t>(p, p.off_sizes), sizeof(int32_t) * kBlock,
-                                         hipMemcpyDeviceToHost, op->stream));
-                ran[j] = 1;
+                if (!relation && converged[j]) continue;   // masked below anyway
+                const double t = op->kind != OP_MATRIX ? a.params[j] : 0.0;
+                SLQ_TRY(recurrence_block(op, p, t, a.gram, a.tol, a.orthogonalize,
+                                         a.precision));
+                SLQ_TRY(run_quadrature(op->stream, at<double>(p, p.off_alpha),
+                                       at<double>(p, p.off_beta), at<int32_t>(p, p.off_sizes),
+                                       cnt, a.m, a.gram, a.func, a.func_params, a.exponent, nq,
+                                       relation ? params_dev : nullptr, q, relation ? 1 : 0, j,
+                                       q, at<double>(p, p.off_samples), nullptr, nullptr));
+                if (count_steps)
+                {
+                    SLQ_CHECK(hipMemcpyAsync(block_sizes.data() + (size_t) j * kBlock,
+                                             at<int32_t>(p, p.off_sizes),
+                                             sizeof(int32_t) * kBlock, hipMemcpyDeviceToHost,
+                                             op->stream));
+                    ran[j] = 1;
+                }
             }
-        }
         SLQ_CHECK(hipMemcpyAsync(samples + s0 * q, at<double>(p, p.off_samples),
                                  sizeof(double) * cnt * q, hipMemcpyDeviceToHost, op->stream));
         SLQ_CHECK(hipStreamSynchronize(op->stream));
@@ -2346,6 +2530,7 @@ int slq_estimate(slq_op* op, const double* params, int32_t q, int32_t func,
     const int vdt = value_dtype(op, precision);
     SLQ_TRY(prepare(op, gram, precision));
     Plan p = make_plan(op, gram, m, orthogonalize, q, vdt == SLQ_F64 ? 8 : 4);
+    plan_lanes(op, gram, t_lanes(op, gram, q, p), p);
     SLQ_TRY(op->ws.reserve(p.total));
     p.base = (char*) op->ws.base;
     return estimate_loop(op, p, a);

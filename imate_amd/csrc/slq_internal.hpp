@@ -169,6 +169,9 @@ struct PairTiles {
     int32_t* src_a = nullptr;  // [nslabs * R]
     int32_t* src_b = nullptr;  // [nslabs * R]
     bool tried = false;
+    // a view of another handle's pair tiles (with its own value buffer):
+    // owns nothing
+    bool borrowed = false;
     ~PairTiles();
 };
 
@@ -181,6 +184,9 @@ struct slq_op {
     std::shared_ptr<slq::MatrixStore> A;
     std::shared_ptr<slq::MatrixStore> B;   // OP_AFFINE_B only
     hipStream_t stream = nullptr;
+    // streams of the recurrences of an A + tB estimate that run side by
+    // side, one per t of a group (created on first use)
+    std::vector<hipStream_t> lane_streams;
     int device = 0;
     slq::Workspace ws;
     slq::ProfileState prof;

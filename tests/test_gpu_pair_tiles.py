@@ -262,3 +262,39 @@ def test_gram_general_b_on_union_tiles(monkeypatch):
     c = numpy.asarray(csr['convergence']['samples'])
     assert numpy.max(numpy.abs(got - c) / numpy.abs(c)) < 1e-10
     assert not numpy.array_equal(got, c)
+
+
+@pytest.mark.parametrize('gram,precision', [(False, None), (False, 'mixed'),
+                                            (True, None)])
+def test_t_lanes_equal_sequential(monkeypatch, gram, precision):
+    """The t of A + tB run side by side (eight lanes, so eleven t take two
+    groups) give the samples of the one-after-the-other run bit for bit,
+    also when some t converge before others (early stop per inquiry)."""
+    A = corr()
+    B = _spd_other_pattern(A.shape[0], 40, 0, 14)
+    ts = list(numpy.logspace(-2, 1, 11))
+    kw = dict(parameters=ts, method='slq', gpu=True, lanczos_degree=16,
+              min_num_samples=20, max_num_samples=150, error_rtol=2e-3,
+              seed=8, gram=gram, return_info=True)
+    if precision:
+        kw['precision'] = precision
+    Aop = imate_amd.AffineMatrixFunction(A, B)
+    monkeypatch.setenv('SLQ_CONCURRENT_T', '0')
+    v0, i0 = imate_amd.logdet(Aop, **kw)
+    monkeypatch.delenv('SLQ_CONCURRENT_T')
+    v1, i1 = imate_amd.logdet(Aop, **kw)
+    s0 = numpy.asarray(i0['convergence']['samples'])
+    s1 = numpy.asarray(i1['convergence']['samples'])
+    assert numpy.array_equal(s0, s1, equal_nan=True)
+    assert numpy.array_equal(numpy.asarray(v0), numpy.asarray(v1))
+    used = numpy.asarray(i1['convergence']['num_samples_used'])
+    assert used.min() < used.max()   # some t stopped earlier than others
+    if not gram and precision is None:
+        ref = oracle.slq(A, func='log', parameters=ts, B=B, affine=True,
+                         lanczos_degree=16, min_num_samples=20,
+                         max_num_samples=150, error_rtol=2e-3, seed=8)
+        want = ref['samples']
+        ok = ~numpy.isnan(want)
+        assert numpy.array_equal(ok, ~numpy.isnan(s1))
+        assert numpy.max(numpy.abs(s1[ok] - want[ok]) /
+                         numpy.abs(want[ok])) < 1e-10
