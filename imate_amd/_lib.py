@@ -83,8 +83,8 @@ SIGNATURES = {
         _P, _i32, _i32, _P, _i32, _f64, _i32, _i64, _i32, _f64, _i32, _f64, _f64,
         _i32, _i32, _P, _P, _P, _P, _P]),
     'slq_autotune': (ctypes.c_int, [_P, _i32, _P, _P]),
-    'slq_op_set_launch': (ctypes.c_int, [_P, _i32, _i32, _i32]),
-    'slq_op_get_launch': (ctypes.c_int, [_P, _P, _P, _P]),
+    'slq_op_set_launch': (ctypes.c_int, [_P, _i32, _i32, _i32, _i32]),
+    'slq_op_get_launch': (ctypes.c_int, [_P, _P, _P, _P, _P]),
     'slq_mlp_num_params': (ctypes.c_int, []),
     'slq_mlp_grad': (ctypes.c_int, [_P, _P, _P, _i32, _P, _P]),
     'slq_mlp_train': (ctypes.c_int, [_P, _P, _P, _i32, _i32, ctypes.c_float,

@@ -932,8 +932,18 @@ struct Plan {
     int q = 1;
     size_t off_words = 0, off_X0 = 0, off_alpha = 0, off_beta = 0, off_sizes = 0,
            off_samples = 0, off_params = 0, off_scratch = 0, total = 0;
+    // a second set of per-block outputs, so that the next block's recurrence
+    // can run while this block's quadrature and copy-back finish
+    size_t off_alpha2 = 0, off_beta2 = 0, off_sizes2 = 0, off_samples2 = 0;
+    size_t alpha_of(int par) const { return par ? off_alpha2 : off_alpha; }
+    size_t beta_of(int par) const { return par ? off_beta2 : off_beta; }
+    size_t sizes_of(int par) const { return par ? off_sizes2 : off_sizes; }
+    size_t samples_of(int par) const { return par ? off_samples2 : off_samples; }
     size_t scratch_bytes = 0;
     char* base = nullptr;   // device workspace the offsets are taken from
+    // false for slq_estimate_noalloc: no streams or events are created, so
+    // blocks run one after the other on the handle's stream
+    bool pipeline = true;
     // recurrences of several t of A + tB side by side: per lane alpha, beta,
     // sizes, recurrence scratch and the pair-tile values of its t (both
     // directions for Golub-Kahn), lane c at off_lanes + c * lane_stride
@@ -960,6 +970,10 @@ Plan make_plan(slq_op* op, int gram, int m, int ortho, int q, int vbytes)
     p.off_beta = o;    o += align256(sizeof(double) * kBlock * m);
     p.off_sizes = o;   o += align256(sizeof(int32_t) * kBlock);
     p.off_samples = o; o += align256(sizeof(double) * kBlock * q);
+    p.off_alpha2 = o;  o += align256(sizeof(double) * kBlock * m);
+    p.off_beta2 = o;   o += align256(sizeof(double) * kBlock * m);
+    p.off_sizes2 = o;  o += align256(sizeof(int32_t) * kBlock);
+    p.off_samples2 = o; o += align256(sizeof(double) * kBlock * q);
     p.off_params = o;  o += align256(sizeof(double) * q);
     p.off_scratch = o;
     p.scratch_bytes = recurrence_scratch_bytes(op, gram, m, ortho, vbytes);
@@ -1082,7 +1096,7 @@ int make_probes(slq_op* op, const Plan& p, const uint64_t base[4], int64_t first
 }
 
 int recurrence_block(slq_op* op, const Plan& p, double t, int gram, double tol,
-                     int ortho, int precision)
+                     int ortho, int precision, int par = 0)
 {
     RecurrenceArgs ra;
     ra.op = op;
@@ -1093,9 +1107,9 @@ int recurrence_block(slq_op* op, const Plan& p, double t, int gram, double tol,
     ra.ortho = ortho;
     ra.precision = precision;
     ra.X0 = at<void>(p, p.off_X0);
-    ra.alpha = at<double>(p, p.off_alpha);
-    ra.beta = at<double>(p, p.off_beta);
-    ra.sizes = at<int32_t>(p, p.off_sizes);
+    ra.alpha = at<double>(p, p.alpha_of(par));
+    ra.beta = at<double>(p, p.beta_of(par));
+    ra.sizes = at<int32_t>(p, p.sizes_of(par));
     return run_recurrence(ra, at<void>(p, p.off_scratch), p.scratch_bytes);
 }
 
@@ -1115,6 +1129,7 @@ struct LaneOp {
         op.spmm_grid = parent->spmm_grid;
         op.stream_grid = parent->stream_grid;
         op.spmm_split = parent->spmm_split;
+        op.tile_mode = parent->tile_mode;
         borrow(op.ab, parent->ab, vals);
         borrow(op.ab_t, parent->ab_t, vals_t);
     }
@@ -1272,7 +1287,7 @@ struct TuneKey {
 };
 
 struct TuneValue {
-    int spmm_grid, stream_grid, spmm_split;
+    int spmm_grid, stream_grid, spmm_split, tile_mode;
 };
 
 std::mutex g_tune_mu;
@@ -1305,6 +1320,7 @@ extern "C" int slq_autotune(slq_op* op, int32_t precision, int32_t* spmm_grid,
             op->tuned_spmm_grid = it->second.spmm_grid;
             op->tuned_stream_grid = it->second.stream_grid;
             op->tuned_spmm_split = it->second.spmm_split;
+            op->tuned_tile_mode = it->second.tile_mode;
             if (spmm_grid) *spmm_grid = op->tuned_spmm_grid;
             if (stream_grid) *stream_grid = op->tuned_stream_grid;
             return SLQ_OK;
@@ -1353,17 +1369,27 @@ extern "C" int slq_autotune(slq_op* op, int32_t precision, int32_t* spmm_grid,
             best_u = cap;
         }
     }
-    // wavefronts per tile of the matrix-core SpMM, at the chosen caps
-    int best_k = 0;
-    const int keep_k = op->spmm_split;
-    if (r == SLQ_OK && op->A->tiles.R == 16)
+    // operand form at the chosen caps: the 16-row tiles with 1, 2 or 4
+    // wavefronts per tile, or the CSR rows
+    int best_k = 0, best_m = 0;
+    const int keep_k = op->spmm_split, keep_m = op->tile_mode;
+    if (r == SLQ_OK && op->A->tiles.R > 0)
     {
         op->spmm_grid = best_s;
         op->stream_grid = best_u;
         best_t = 1e300;
-        for (int k : {1, 2, 4})
+        struct Form {
+            int mode, split;
+        };
+        std::vector<Form> forms;
+        if (op->A->tiles.R == 16)
+            forms = {{0, 1}, {0, 2}, {0, 4}, {1, 1}};
+        else
+            forms = {{0, 1}, {1, 1}};
+        for (const Form& f : forms)
         {
-            op->spmm_split = k;
+            op->tile_mode = f.mode;
+            op->spmm_split = f.split;
             const double t = time_block(op, steps, precision);
             if (t < 0)
             {
@@ -1373,20 +1399,23 @@ extern "C" int slq_autotune(slq_op* op, int32_t precision, int32_t* spmm_grid,
             if (t < best_t)
             {
                 best_t = t;
-                best_k = k;
+                best_m = f.mode + 1;
+                best_k = f.split;
             }
         }
     }
     op->spmm_grid = keep_s;
     op->stream_grid = keep_u;
     op->spmm_split = keep_k;
+    op->tile_mode = keep_m;
     if (r != SLQ_OK) return r;
     op->tuned_spmm_grid = best_s;
     op->tuned_stream_grid = best_u;
     op->tuned_spmm_split = best_k;
+    op->tuned_tile_mode = best_m;
     {
         std::lock_guard<std::mutex> lock(g_tune_mu);
-        g_tune_cache[key] = {best_s, best_u, best_k};
+        g_tune_cache[key] = {best_s, best_u, best_k, best_m};
     }
     if (spmm_grid) *spmm_grid = best_s;
     if (stream_grid) *stream_grid = best_u;
@@ -1398,7 +1427,7 @@ extern "C" int slq_autotune(slq_op* op, int32_t precision, int32_t* spmm_grid,
 // Swaps the autotuned launch settings into the handles of a batch and back.
 struct TunedLaunch {
     struct Saved {
-        int spmm_grid, stream_grid, spmm_split;
+        int spmm_grid, stream_grid, spmm_split, tile_mode;
     };
     slq_op** ops;
     int count;
@@ -1408,10 +1437,11 @@ struct TunedLaunch {
         for (int i = 0; i < count; ++i)
         {
             slq_op* op = ops[i];
-            saved[i] = {op->spmm_grid, op->stream_grid, op->spmm_split};
+            saved[i] = {op->spmm_grid, op->stream_grid, op->spmm_split, op->tile_mode};
             if (op->tuned_spmm_grid > 0) op->spmm_grid = op->tuned_spmm_grid;
             if (op->tuned_stream_grid > 0) op->stream_grid = op->tuned_stream_grid;
             if (op->tuned_spmm_split > 0) op->spmm_split = op->tuned_spmm_split;
+            if (op->tuned_tile_mode > 0) op->tile_mode = op->tuned_tile_mode - 1;
         }
     }
     ~TunedLaunch()
@@ -1421,6 +1451,7 @@ struct TunedLaunch {
             ops[i]->spmm_grid = saved[i].spmm_grid;
             ops[i]->stream_grid = saved[i].stream_grid;
             ops[i]->spmm_split = saved[i].spmm_split;
+            ops[i]->tile_mode = saved[i].tile_mode;
         }
     }
 };
@@ -2186,10 +2217,12 @@ int slq_op_destroy(slq_op* op)
     if (!op) return SLQ_OK;
     if (op->stream) (void) hipStreamSynchronize(op->stream);
     for (hipStream_t ls : op->lane_streams) (void) hipStreamSynchronize(ls);
+    if (op->aux_stream) (void) hipStreamSynchronize(op->aux_stream);
     prof_collect(op);
     for (hipEvent_t e : op->prof.pool) (void) hipEventDestroy(e);
     if (op->stream) (void) hipStreamDestroy(op->stream);
     for (hipStream_t ls : op->lane_streams) (void) hipStreamDestroy(ls);
+    if (op->aux_stream) (void) hipStreamDestroy(op->aux_stream);
     delete op;
     return SLQ_OK;
 }
@@ -2353,45 +2386,100 @@ int estimate_loop(slq_op* op, const Plan& p, EstimateArgs a)
         a.num_used[j] = 0;
         a.num_outliers[j] = 0;
     }
+    // One recurrence per block (a matrix, or A + tI with its Ritz shifts):
+    // the quadrature and copy-back of a block run on a second stream, and
+    // when the convergence test cannot stop the loop within a block (fewer
+    // than min_num_samples processed by its end) the next block's probes
+    // and recurrence are queued before the host waits for it. The kernels
+    // and their inputs are those of the plain order, so results are equal.
+    const bool pipelined = p.pipeline && p.lanes == 0 && required == 1;
+    if (pipelined && !op->aux_stream)
+        SLQ_CHECK(hipStreamCreateWithFlags(&op->aux_stream, hipStreamNonBlocking));
+    hipStream_t qstream = pipelined ? op->aux_stream : op->stream;
     // completed recurrence steps per block, kept only while profiling
     const bool count_steps = op->prof.enabled;
-    std::vector<int32_t> block_sizes(count_steps ? (size_t) kBlock * required : 0);
+    std::vector<int32_t> block_sizes[2];
+    std::vector<char> ran[2];
+    for (int k = 0; k < 2; ++k)
+    {
+        block_sizes[k].assign(count_steps ? (size_t) kBlock * required : 0, 0);
+        ran[k].assign(count_steps ? required : 0, 0);
+    }
+    hipEvent_t rec_done[2] = {nullptr, nullptr};
+    struct Events {
+        hipEvent_t* e;
+        ~Events()
+        {
+            for (int k = 0; k < 2; ++k)
+                if (e[k]) (void) hipEventDestroy(e[k]);
+        }
+    } events{rec_done};
+    if (pipelined)
+        for (int k = 0; k < 2; ++k)
+            SLQ_CHECK(hipEventCreateWithFlags(&rec_done[k], hipEventDisableTiming));
+
+    // probes, recurrences and quadratures of the block at s0 (outputs of
+    // parity par when pipelined)
+    auto enqueue = [&](int64_t s0, int par) -> int {
+        const int cnt = (int) std::min<int64_t>(bs, a.max_s - s0);
+        std::fill(ran[par].begin(), ran[par].end(), 0);
+        SLQ_TRY(make_probes(op, p, base, s0, cnt, vdt));
+        if (p.lanes > 0)
+            return lanes_block(op, p, a, cnt, nq, count_steps, block_sizes[par], ran[par]);
+        for (int j = 0; j < required; ++j)
+        {
+            if (!relation && converged[j]) continue;   // masked below anyway
+            const double t = op->kind != OP_MATRIX ? a.params[j] : 0.0;
+            SLQ_TRY(recurrence_block(op, p, t, a.gram, a.tol, a.orthogonalize, a.precision,
+                                     par));
+            if (pipelined)
+            {
+                SLQ_CHECK(hipEventRecord(rec_done[par], op->stream));
+                SLQ_CHECK(hipStreamWaitEvent(qstream, rec_done[par], 0));
+            }
+            SLQ_TRY(run_quadrature(qstream, at<double>(p, p.alpha_of(par)),
+                                   at<double>(p, p.beta_of(par)),
+                                   at<int32_t>(p, p.sizes_of(par)), cnt, a.m, a.gram, a.func,
+                                   a.func_params, a.exponent, nq,
+                                   relation ? params_dev : nullptr, q, relation ? 1 : 0, j, q,
+                                   at<double>(p, p.samples_of(par)), nullptr, nullptr));
+            if (count_steps && !pipelined)
+                SLQ_CHECK(hipMemcpyAsync(block_sizes[par].data() + (size_t) j * kBlock,
+                                         at<int32_t>(p, p.sizes_of(par)),
+                                         sizeof(int32_t) * kBlock, hipMemcpyDeviceToHost,
+                                         qstream));
+            if (count_steps) ran[par][j] = 1;
+        }
+        return SLQ_OK;
+    };
+
     int nproc = 0;
     int all_conv = 0;
-
     SLQ_CHECK(hipStreamSynchronize(op->stream));
     const auto t0 = std::chrono::steady_clock::now();
-    for (int64_t s0 = 0; s0 < a.max_s && !all_conv; s0 += bs)
+    int64_t s0 = 0;
+    int par = 0;
+    SLQ_TRY(enqueue(0, 0));
+    for (;;)
     {
         const int cnt = (int) std::min<int64_t>(bs, a.max_s - s0);
-        SLQ_TRY(make_probes(op, p, base, s0, cnt, vdt));
-        std::vector<char> ran(count_steps ? required : 0, 0);
-        if (p.lanes > 0)
-            SLQ_TRY(lanes_block(op, p, a, cnt, nq, count_steps, block_sizes, ran));
-        else
-            for (int j = 0; j < required; ++j)
-            {
-                if (!relation && converged[j]) continue;   // masked below anyway
-                const double t = op->kind != OP_MATRIX ? a.params[j] : 0.0;
-                SLQ_TRY(recurrence_block(op, p, t, a.gram, a.tol, a.orthogonalize,
-                                         a.precision));
-                SLQ_TRY(run_quadrature(op->stream, at<double>(p, p.off_alpha),
-                                       at<double>(p, p.off_beta), at<int32_t>(p, p.off_sizes),
-                                       cnt, a.m, a.gram, a.func, a.func_params, a.exponent, nq,
-                                       relation ? params_dev : nullptr, q, relation ? 1 : 0, j,
-                                       q, at<double>(p, p.off_samples), nullptr, nullptr));
-                if (count_steps)
-                {
-                    SLQ_CHECK(hipMemcpyAsync(block_sizes.data() + (size_t) j * kBlock,
-                                             at<int32_t>(p, p.off_sizes),
-                                             sizeof(int32_t) * kBlock, hipMemcpyDeviceToHost,
-                                             op->stream));
-                    ran[j] = 1;
-                }
-            }
-        SLQ_CHECK(hipMemcpyAsync(samples + s0 * q, at<double>(p, p.off_samples),
-                                 sizeof(double) * cnt * q, hipMemcpyDeviceToHost, op->stream));
-        SLQ_CHECK(hipStreamSynchronize(op->stream));
+        const int64_t s1 = s0 + bs;
+        const int nxt = pipelined ? par ^ 1 : 0;
+        bool ahead = false;
+        if (pipelined && s1 < a.max_s && nproc + cnt < a.min_s)
+        {
+            SLQ_TRY(enqueue(s1, nxt));
+            ahead = true;
+        }
+        // copies to host memory wait for the stream, so they are issued only
+        // here, when the host needs the block (a pipelined block's sizes too)
+        if (count_steps && pipelined)
+            SLQ_CHECK(hipMemcpyAsync(block_sizes[par].data(), at<int32_t>(p, p.sizes_of(par)),
+                                     sizeof(int32_t) * kBlock, hipMemcpyDeviceToHost,
+                                     qstream));
+        SLQ_CHECK(hipMemcpyAsync(samples + s0 * q, at<double>(p, p.samples_of(par)),
+                                 sizeof(double) * cnt * q, hipMemcpyDeviceToHost, qstream));
+        SLQ_CHECK(hipStreamSynchronize(qstream));
         for (int pi = 0; pi < cnt; ++pi)
         {
             const int64_t i = s0 + pi;
@@ -2407,16 +2495,21 @@ int estimate_loop(slq_op* op, const Plan& p, EstimateArgs a)
                 if (converged[j]) samples[i * q + j] = nan;
             if (count_steps)
                 for (int j = 0; j < required; ++j)
-                    if (ran[j])
+                    if (ran[par][j])
                     {
-                        const int32_t sz = block_sizes[(size_t) j * kBlock + pi];
+                        const int32_t sz = block_sizes[par][(size_t) j * kBlock + pi];
                         op->prof.steps += sz > 0 ? sz : a.m;
                     }
             a.processed[nproc++] = (int32_t) i;
             all_conv = check_convergence(samples, q, a.min_s, a.processed, nproc, a.confidence,
                                          a.atol, a.rtol, a.error, a.num_used, converged);
         }
+        if (all_conv || s1 >= a.max_s) break;
+        if (!ahead) SLQ_TRY(enqueue(s1, nxt));
+        s0 = s1;
+        par = nxt;
     }
+    SLQ_CHECK(hipStreamSynchronize(op->stream));
     const auto t1 = std::chrono::steady_clock::now();
     if (a.alg_wall_time) *a.alg_wall_time = std::chrono::duration<double>(t1 - t0).count();
     for (int i = nproc; i < a.max_s; ++i) a.processed[i] = 0;
@@ -2576,6 +2669,7 @@ int slq_estimate_noalloc(slq_op* op, const double* params, int32_t q, int32_t fu
     void* ptr = nullptr;
     SLQ_TRY(scratch_address(&ptr));
     p.base = (char*) ptr;
+    p.pipeline = false;
     return estimate_loop(op, p, a);
 }
 
@@ -2782,29 +2876,32 @@ int slq_profile_enable(slq_op* op, int32_t enable)
 }
 
 int slq_op_set_launch(slq_op* op, int32_t spmm_grid, int32_t stream_grid,
-                      int32_t spmm_split)
+                      int32_t spmm_split, int32_t tile_mode)
 {
     if (!op || spmm_grid < 0 || spmm_grid > kMaxGrid || stream_grid < 0 ||
         stream_grid > kMaxGrid || !(spmm_split == 0 || spmm_split == 1 || spmm_split == 2 ||
-                                    spmm_split == 4))
+                                    spmm_split == 4) ||
+        !(tile_mode == 0 || tile_mode == 1))
     {
-        set_error("slq_op_set_launch: grid caps must be 0..%d and the split 0, 1, 2 or 4",
-                  kMaxGrid);
+        set_error("slq_op_set_launch: grid caps must be 0..%d, the split 0, 1, 2 or 4 and "
+                  "the tile mode 0 or 1", kMaxGrid);
         return SLQ_E_ARG;
     }
     op->spmm_grid = spmm_grid;
     op->stream_grid = stream_grid;
     op->spmm_split = spmm_split;
+    op->tile_mode = tile_mode;
     return SLQ_OK;
 }
 
 int slq_op_get_launch(slq_op* op, int32_t* spmm_grid, int32_t* stream_grid,
-                      int32_t* spmm_split)
+                      int32_t* spmm_split, int32_t* tile_mode)
 {
     if (!op) return SLQ_E_ARG;
     if (spmm_grid) *spmm_grid = op->spmm_grid;
     if (stream_grid) *stream_grid = op->stream_grid;
     if (spmm_split) *spmm_split = op->spmm_split;
+    if (tile_mode) *tile_mode = op->tile_mode;
     return SLQ_OK;
 }
 

@@ -2069,7 +2069,7 @@ SpmmArgs<T> spmm_defaults()
 template <typename T>
 const DevTiles* tiles_for(slq_op* op, bool trans)
 {
-    if (op->kind == OP_AFFINE_B) return nullptr;
+    if (op->kind == OP_AFFINE_B || op->tile_mode == 1) return nullptr;
     const DevTiles& t = trans ? op->A->trans_tiles : op->A->tiles;
     if (t.R == 0) return nullptr;
     if (sizeof(T) == 4 && op->A->csr.dtype == SLQ_F64 && !t.vals_f32) return nullptr;
@@ -2111,7 +2111,9 @@ int pair_tiles_at(slq_op* op, double t, const DevTiles** out, bool trans = false
 {
     *out = nullptr;
     PairTiles& ab = trans ? op->ab_t : op->ab;
-    if (op->kind != OP_AFFINE_B || ab.tiles.R == 0 || !pair_tiles_enabled()) return SLQ_OK;
+    if (op->kind != OP_AFFINE_B || ab.tiles.R == 0 || !pair_tiles_enabled() ||
+        op->tile_mode == 1)
+        return SLQ_OK;
     const bool a64 = op->A->csr.dtype == SLQ_F64;
     const void* da = trans ? op->A->trans.data : op->A->csr.data;
     const void* db = trans ? op->B->trans.data : op->B->csr.data;

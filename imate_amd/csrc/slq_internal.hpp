@@ -187,6 +187,8 @@ struct slq_op {
     // streams of the recurrences of an A + tB estimate that run side by
     // side, one per t of a group (created on first use)
     std::vector<hipStream_t> lane_streams;
+    // the per-block quadrature and copy-back of slq_estimate (first use)
+    hipStream_t aux_stream = nullptr;
     int device = 0;
     slq::Workspace ws;
     slq::ProfileState prof;
@@ -203,6 +205,10 @@ struct slq_op {
     // wavefronts per tile of the matrix-core SpMM (1, 2, 4; 0 = default)
     int spmm_split = 0;
     int tuned_spmm_split = 0;
+    // operand form of the SpMM: 0 the row tiles where they were built,
+    // 1 the CSR rows (tuned: 0 = not tuned, else mode + 1)
+    int tile_mode = 0;
+    int tuned_tile_mode = 0;
 };
 
 namespace slq {

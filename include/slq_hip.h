@@ -243,10 +243,11 @@ int slq_estimate_batch(slq_op** ops, int32_t count, int32_t func,
                        double* samples, double* run_time, double* setup_time);
 
 /* Autotuner: times one 64-probe block of Lanczos steps over candidate launch
- * grids for the SpMM and for the streaming kernels, then (16-row tiles) over
- * 1, 2 and 4 wavefronts per tile of the matrix-core SpMM, keeps the fastest
- * in the handle for slq_estimate_batch and caches it per (rows, non-zeros
- * per row, value type, tile rows). Outputs the chosen caps when non-NULL. The other
+ * grids for the SpMM and for the streaming kernels, then over the SpMM's
+ * operand form (the 16-row tiles with 1, 2 or 4 wavefronts per tile, or the
+ * CSR rows), keeps the fastest in the handle for slq_estimate_batch and
+ * caches it per (rows, non-zeros per row, value type, tile rows). Outputs
+ * the chosen caps when non-NULL. The other
  * entry points keep the library's default grids on a tuned handle, so their
  * reduction order, and with it every result bit, does not depend on tuning.
  * Square matrix or A + tI handles only (Lanczos is what is timed).
@@ -255,17 +256,18 @@ int slq_autotune(slq_op* op, int32_t precision, int32_t* spmm_grid,
                  int32_t* stream_grid);
 
 /* Launch configuration of a handle's kernels: grid caps of the SpMM and of
- * the streaming kernels (0 .. 1024) and wavefronts per tile of the
- * matrix-core SpMM (1, 2 or 4); 0 keeps the library default (environment
- * SLQ_GRID / SLQ_STREAM_GRID / SLQ_SPMM_SPLIT, else the built-in). Every
- * setting gives results within rounding of every other; for a fixed
- * setting they are reproducible bit for bit (the reduction order follows
- * the grid and the split). No reference counterpart (imate has no tunable
- * kernels). */
+ * the streaming kernels (0 .. 1024), wavefronts per tile of the
+ * matrix-core SpMM (1, 2 or 4) and the SpMM's operand form (tile_mode 0:
+ * the row tiles where they were built, 1: the CSR rows); 0 keeps the
+ * library default (environment SLQ_GRID / SLQ_STREAM_GRID / SLQ_SPMM_SPLIT,
+ * else the built-in). Every setting gives results within rounding of every
+ * other; for a fixed setting they are reproducible bit for bit (the
+ * reduction order follows the grid, the split and the form). No reference
+ * counterpart (imate has no tunable kernels). */
 int slq_op_set_launch(slq_op* op, int32_t spmm_grid, int32_t stream_grid,
-                      int32_t spmm_split);
+                      int32_t spmm_split, int32_t tile_mode);
 int slq_op_get_launch(slq_op* op, int32_t* spmm_grid, int32_t* stream_grid,
-                      int32_t* spmm_split);
+                      int32_t* spmm_split, int32_t* tile_mode);
 
 /* ---- learned surrogate (no reference counterpart) ----
  * A 2 -> 128 -> 128 -> 1 SiLU MLP used as a surrogate of SLQ estimates

@@ -21,7 +21,7 @@ pytestmark = pytest.mark.gpu
 def _matrix(A, split):
     M = imate_amd.Matrix(A)
     h = M.get_linear_operator(gpu=True).handle
-    _lib.check(_lib.load().slq_op_set_launch(h, 0, 0, split),
+    _lib.check(_lib.load().slq_op_set_launch(h, 0, 0, split, 0),
                'slq_op_set_launch')
     return M
 
@@ -56,7 +56,7 @@ def test_split_reorth_mixed_and_affine(split):
     assert rel.max() < 1e-5
     F = imate_amd.AffineMatrixFunction(A)
     h = F.get_linear_operator(gpu=True).handle
-    _lib.check(_lib.load().slq_op_set_launch(h, 0, 0, split),
+    _lib.check(_lib.load().slq_op_set_launch(h, 0, 0, split, 0),
                'slq_op_set_launch')
     ts = [1e-3, 0.5]
     vals, info = imate_amd.logdet(F, parameters=ts, **kw)
@@ -105,7 +105,31 @@ def test_split_products_vs_numpy(gpu_lib, split):
     ref = A @ X
     assert numpy.max(numpy.abs(Y - ref)) <= 1e-13 * numpy.abs(ref).max()
     got = ctypes.c_int32(0)
-    _lib.check(gpu_lib.slq_op_get_launch(h, None, None, ctypes.byref(got)),
-               'slq_op_get_launch')
+    _lib.check(gpu_lib.slq_op_get_launch(h, None, None, ctypes.byref(got),
+                                         None), 'slq_op_get_launch')
     assert got.value == split
-    assert gpu_lib.slq_op_set_launch(h, 0, 0, 3) == _lib.SLQ_E_ARG
+    assert gpu_lib.slq_op_set_launch(h, 0, 0, 3, 0) == _lib.SLQ_E_ARG
+    assert gpu_lib.slq_op_set_launch(h, 0, 0, 1, 2) == _lib.SLQ_E_ARG
+
+
+def test_csr_operand_form_vs_oracle_and_tiles():
+    """tile_mode 1 (the CSR rows on a handle that has 16-row tiles): the
+    oracle's samples at 1e-10, and different bits from the tile path (so
+    the run did take the other path)."""
+    A = sparse_correlation_matrix(4096, nnz_per_row=40, seed=9)
+    kw = dict(method='slq', gpu=True, lanczos_degree=20, min_num_samples=64,
+              max_num_samples=64, seed=3, return_info=True)
+    M = imate_amd.Matrix(A)
+    h = M.get_linear_operator(gpu=True).handle
+    _, tiles = imate_amd.logdet(M, **kw)
+    _lib.check(_lib.load().slq_op_set_launch(h, 0, 0, 0, 1),
+               'slq_op_set_launch')
+    _, csr = imate_amd.logdet(M, **kw)
+    ref = oracle.slq(A, func='log', lanczos_degree=20, min_num_samples=64,
+                     max_num_samples=64, seed=3)
+    want = ref['samples'][:, 0]
+    for info in (tiles, csr):
+        got = info['convergence']['samples']
+        assert numpy.max(numpy.abs(got - want) / numpy.abs(want)) < 1e-10
+    assert not numpy.array_equal(tiles['convergence']['samples'],
+                                 csr['convergence']['samples'])
