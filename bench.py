@@ -189,7 +189,15 @@ def precision_form(args, compact=False):
             log('precision form: %s' % json.dumps(res))
             rows.append(res)
     out = {'form': 'precision', 'n': n, 'nnz': int(A.nnz),
-           'lanczos_degree': m, 'samples': s, 'runs': rows}
+           'lanczos_degree': m, 'samples': s, 'runs': rows,
+           'reorthogonalization': {
+               'mfma_utilization': 0.0,
+               'why': 'every probe has its own Krylov basis, so h[k][p] = '
+                      'sum_i V[i][k][p] r[i][p] is a batched GEMV with no '
+                      'operand shared across probes: 2 flops per basis '
+                      'value read (0.5 flop/B FP32, 0.25 flop/B FP64), '
+                      'bound by HBM; the Gram-Schmidt kernels run on the '
+                      'vector ALUs at 3.8-5.5 TB/s'}}
     if not compact:
         out['exact'] = exact
     else:
@@ -789,6 +797,15 @@ def main():
                                                 avg_spmm, spmm_n.value,
                                                 avg_upd, upd_n.value))
 
+    # the achievable streaming rate on this device: a plain copy kernel
+    # (1 GiB read + 1 GiB written per launch), for frac_copy_peak
+    copy_ms = ctypes.c_double(0)
+    copy_gbps = None
+    if lib.slq_copy_bandwidth(1 << 30, 10, ctypes.byref(copy_ms)) == 0 and \
+            copy_ms.value > 0:
+        copy_gbps = 2 * (1 << 30) / (copy_ms.value * 1e-3) / 1e9
+        log('copy kernel: %.0f GB/s' % copy_gbps)
+
     est = float(numpy.mean(vals))
     ref = cholesky_reference(n, k, args.seed) if args.func == 'logdet' else None
     accuracy = {'estimate': est,
@@ -833,6 +850,14 @@ def main():
                      'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
                      'frac': achieved / HBM_PEAK_GBS,
                      'traffic': measured_traffic(n, A.nnz),
+                     'traffic_source': 'rocprofv3 FETCH_SIZE and WRITE_SIZE of '
+                                       'this workload scaled by the copy '
+                                       'kernel calibration, recorded in '
+                                       'profiles/spmm_traffic.json (not '
+                                       'measured in this run)',
+                     'copy_peak_GBps': copy_gbps,
+                     'frac_copy_peak': (achieved / copy_gbps
+                                        if copy_gbps else None),
                      'kernel': 'mfma_spmm_kernel<double> + tile sums (one '
                                'Lanczos SpMM launch)',
                      'bytes_per_launch': bspmm,

@@ -497,82 +497,6 @@ __global__ __launch_bounds__(WPG * kWave) void csr_blocked_kernel(SpmmArgs<T> a,
     if (a.red_mode != 0) wg_partial<WPG>(red, a.rb);
 }
 
-// Row-tiled form: one wavefront per tile of R rows, R accumulators per lane.
-// Each gathered operand row X[u] feeds the R values of column u, read with
-// scalar loads (the slab address is wavefront-uniform, and a scalar operand
-// feeds the FMA directly).
-template <typename T, int R, bool NT>
-__global__ __launch_bounds__(kThreads) void tiled_spmm_kernel(
-    SpmmArgs<T> a, const T* __restrict__ X, T* __restrict__ Y, T* __restrict__ Vout,
-    const T* __restrict__ Sub, int64_t ntiles, const int64_t* __restrict__ tptr,
-    const int32_t* __restrict__ tcols, const T* __restrict__ tvals)
-{
-    const int lane = threadIdx.x % kWave;
-    const int wave = wave_id();
-    const Sweep sw = sweep(ntiles, wave);
-    const T s = a.xscale ? (T) a.xscale[lane] : T(1);
-    const T tt = (T) a.t;
-    const T sub_c = Sub ? (T) a.sub_coef[lane] : T(0);
-    const T sub_s = Sub ? (T) a.sub_scale[lane] : T(0);
-    double red = 0.0;
-    for (int64_t t = sw.first; t < sw.end; t += sw.stride)
-    {
-        T acc[R];
-#pragma unroll
-        for (int r = 0; r < R; ++r) acc[r] = T(0);
-        const int64_t beg = tptr[t];
-        const int64_t end = tptr[t + 1];
-        for (int64_t base = beg; base < end; base += kWave)
-        {
-            const int64_t rem = end - base;
-            const int cnt = rem < kWave ? (int) rem : kWave;
-            const int cl = lane < cnt ? tcols[base + lane] : 0;
-            const T* __restrict__ vb = tvals + base * R;
-            int k = 0;
-            for (; k + 4 <= cnt; k += 4)
-            {
-                const int c0 = readlane(cl, k);
-                const int c1 = readlane(cl, k + 1);
-                const int c2 = readlane(cl, k + 2);
-                const int c3 = readlane(cl, k + 3);
-                const T x0 = s * X[(int64_t) c0 * kBlock + lane];
-                const T x1 = s * X[(int64_t) c1 * kBlock + lane];
-                const T x2 = s * X[(int64_t) c2 * kBlock + lane];
-                const T x3 = s * X[(int64_t) c3 * kBlock + lane];
-                const T* v = vb + (int64_t) k * R;
-#pragma unroll
-                for (int r = 0; r < R; ++r) acc[r] = fma(v[r], x0, acc[r]);
-#pragma unroll
-                for (int r = 0; r < R; ++r) acc[r] = fma(v[R + r], x1, acc[r]);
-#pragma unroll
-                for (int r = 0; r < R; ++r) acc[r] = fma(v[2 * R + r], x2, acc[r]);
-#pragma unroll
-                for (int r = 0; r < R; ++r) acc[r] = fma(v[3 * R + r], x3, acc[r]);
-            }
-            for (; k < cnt; ++k)
-            {
-                const int c = readlane(cl, k);
-                const T x = s * X[(int64_t) c * kBlock + lane];
-                const T* v = vb + (int64_t) k * R;
-#pragma unroll
-                for (int r = 0; r < R; ++r) acc[r] = fma(v[r], x, acc[r]);
-            }
-        }
-#pragma unroll
-        for (int r = 0; r < R; ++r)
-        {
-            const int64_t row = t * R + r;
-            if (row < a.nrows)
-                spmm_epilogue<T, NT>(a, row, lane, acc[r], s, tt, sub_c, sub_s, X, Y, Vout,
-                                     Sub, red);
-        }
-    }
-    spmm_finish<T>(a, red);
-}
-
-// One chunk (up to 64 slabs) of a tile, scalar-loaded slab values: the
-// slab address is wavefront-uniform, so the R values come in with one scalar
-// load and feed the FMAs as scalar operands; gathers in groups of four.
 template <typename T, int R>
 __device__ __forceinline__ void slabs_scalar(const T* __restrict__ X,
                                              const int32_t* __restrict__ tcols,
@@ -2024,11 +1948,6 @@ int nt_default()
     return v;
 }
 
-// Schedule of 8-row tiles: 1 (default) = handed out in order
-// (tiled_spmm_sched_kernel), 0 = the static interleaved sweep
-// (tiled_spmm_kernel), kept for comparison and for launches without ticket
-// counters. SLQ_SPMM overrides. 16-row tiles always run the in-order
-// matrix-core kernel (their value layout is the MFMA operand layout).
 // Column block of the blocked CSR SpMM, or 0 for the row-per-wavefront
 // kernel: blocked when the rows are sorted, long (>= 128 non-zeros on
 // average) and the gathered operand is well beyond the L2 (> 16 MiB), and
@@ -2044,15 +1963,6 @@ int csr_blocked_cols(const SpmmArgs<T>& s)
     if ((double) s.ncols * kBlock * sizeof(T) <= 16.0 * (1 << 20)) return 0;
     if (s.avg_row_nnz < 128.0) return 0;
     return cb;
-}
-
-int spmm_variant()
-{
-    static int v = [] {
-        const char* e = std::getenv("SLQ_SPMM");
-        return e ? std::atoi(e) : 1;
-    }();
-    return v;
 }
 
 template <typename T>
@@ -2197,11 +2107,9 @@ void launch_spmm_nt(slq_op* op, const SpmmArgs<T>& s, const SpmmIO<T>& io, const
         const T* tv = (sizeof(T) == 4 && op->A->csr.dtype == SLQ_F64)
                           ? (const T*) tl->vals_f32 : (const T*) tl->vals;
         const int grid = grid_for(tl->ntiles, op->spmm_grid);
-        // 16-row tiles (matrix-core layout) always take the in-order
-        // schedule; launch_spmm has checked its buffers
-        const bool sched = tl->R == 16 || (spmm_variant() != 0 && s.rb.tickets &&
-                                           (s.red_mode == 0 || s.rb.tile_partial));
-        if (sched)
+        // tiles are handed out in order from the XCD's ticket counter (16-row
+        // tiles on the matrix cores, 8-row tiles on the vector ALUs);
+        // launch_spmm has checked the counters and partial rows
         {
             const int split = tl->R == 16 ? spmm_split(op) : 1;
             if (tl->R == 16 && split == 2)
@@ -2230,15 +2138,6 @@ void launch_spmm_nt(slq_op* op, const SpmmArgs<T>& s, const SpmmIO<T>& io, const
                 hipLaunchKernelGGL((total_kernel<T>), dim3(1), dim3(kTotalWaves * kWave), 0,
                                    op->stream, s, rg);
             }
-        }
-        else
-        {
-            hipLaunchKernelGGL((tiled_spmm_kernel<T, 8, NT>), dim3(grid), dim3(kThreads), 0,
-                               op->stream, s, io.X, io.Y, io.Vout, io.Sub, tl->ntiles, tl->ptr,
-                               tl->cols, tv);
-            if (s.red_mode)
-                hipLaunchKernelGGL((total_kernel<T>), dim3(1), dim3(kTotalWaves * kWave), 0,
-                                   op->stream, s, grid);
         }
     }
     else if (csr_blocked_cols(s) > 0)
@@ -2298,9 +2197,9 @@ int launch_spmm(slq_op* op, const SpmmArgs<T>& s, const SpmmIO<T>& io,
         set_error("internal: the dense SpMM needs its split scratch");
         return SLQ_E_ARG;
     }
-    if (tl && tl->R == 16 && (!s.rb.tickets || (s.red_mode && !s.rb.tile_partial)))
+    if (tl && (!s.rb.tickets || (s.red_mode && !s.rb.tile_partial)))
     {
-        set_error("internal: the matrix-core SpMM needs ticket counters and tile partials");
+        set_error("internal: the tiled SpMM needs ticket counters and tile partials");
         return SLQ_E_ARG;
     }
     hipEvent_t ev = nullptr;
