@@ -24,6 +24,7 @@ SLQ_E_NOTSPD = -5
 
 SLQ_F64 = 0
 SLQ_F32 = 1
+SLQ_F128 = 2
 
 FUNC_LOG = 0
 FUNC_INV = 1
@@ -90,6 +91,16 @@ SIGNATURES = {
     'slq_mlp_train': (ctypes.c_int, [_P, _P, _P, _i32, _i32, ctypes.c_float,
                                      _P]),
     'slq_mlp_predict': (ctypes.c_int, [_P, _P, _i32, _i32, _P, _P]),
+    'slq_host_op_create_csr': (ctypes.c_int, [_i64, _i64, _i64, _P, _P, _P,
+                                              _i32, _P]),
+    'slq_host_op_create_csc': (ctypes.c_int, [_i64, _i64, _i64, _P, _P, _P,
+                                              _i32, _P]),
+    'slq_host_op_create_dense': (ctypes.c_int, [_i64, _i64, _P, _i32, _P]),
+    'slq_host_op_create_affine': (ctypes.c_int, [_P, _P, _P]),
+    'slq_host_op_destroy': (ctypes.c_int, [_P]),
+    'slq_estimate_host': (ctypes.c_int, [
+        _P, _P, _i32, _i32, _P, _i32, _f64, _i32, _i64, _i32, _f64, _i32, _i32,
+        _f64, _f64, _f64, _f64, _i32, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
 }
 
 _lib = None
@@ -163,3 +174,17 @@ def dtype_code(dtype):
         return SLQ_F32
     raise TypeError('When gpu is enabled, data type should be "float32" or '
                     '"float64".')
+
+
+def host_dtype_code(dtype):
+    """Value type code of a host (gpu=False) handle: float32, float64 or
+    float128 (long double), the types imate's CPU path accepts."""
+    dtype = numpy.dtype(dtype)
+    if dtype == numpy.float64:
+        return SLQ_F64
+    if dtype == numpy.float32:
+        return SLQ_F32
+    if dtype == numpy.longdouble:
+        return SLQ_F128
+    raise TypeError('Data type should be "float32", "float64", or '
+                    '"float128".')

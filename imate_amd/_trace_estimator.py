@@ -3,9 +3,9 @@ output allocation, the call into ``slq_estimate`` and the ``info`` dict.
 
 Mirrors imate/_trace_estimator/trace_estimator.pyx:38-390 and
 trace_estimator_utilities.py:24-571 (same argument semantics, defaults,
-validation messages, return shapes and ``info`` schema). The GPU branch calls
-the HIP library through its C ABI; ``gpu=False`` is not served by this
-package.
+validation messages, return shapes and ``info`` schema). Both branches call
+the library through its C ABI: ``gpu=True`` the HIP path (``slq_estimate``),
+``gpu=False`` its host implementation (``slq_estimate_host``).
 """
 
 import re
@@ -323,11 +323,9 @@ def trace_estimator(A, parameters, func, gram, exponent, min_num_samples,
     if plot:
         raise NotImplementedError('plot=True (matplotlib convergence plot) '
                                   'is not part of imate_amd.')
-    if not gpu:
-        raise NotImplementedError(
-            'imate_amd implements the GPU path only: call with gpu=True '
-            '(the CPU path, gpu=False, is served by imate itself).')
-    if data_type_name == b'float128':
+    if not gpu and PRECISIONS[precision] != _lib.PREC_NATIVE:
+        raise ValueError('"precision" other than "native" needs gpu=True.')
+    if gpu and data_type_name == b'float128':
         raise TypeError('When gpu is enabled, data type should be ' +
                         '"float32" or "float64".')
 
@@ -349,34 +347,52 @@ def trace_estimator(A, parameters, func, gram, exponent, min_num_samples,
     init_proc_time = time.process_time()
 
     lib = _lib.load()
-    gpu_Aop = Aop.get_linear_operator(gpu=True,
-                                      num_gpu_devices=num_gpu_devices)
-    props = gpu_Aop.get_device_properties()
-    num_gpu_multiprocessors = props['num_multiprocessors'][0]
-    num_gpu_threads_per_multiprocessor = \
-        props['num_threads_per_multiprocessor'][0]
-    # one device: the reproducible configuration of the reference
-    num_gpu_devices = 1
-
     if Aop.get_num_parameters() > 0:
         params = numpy.ascontiguousarray(
             numpy.atleast_1d(numpy.asarray(parameters, dtype=numpy.float64)))
     else:
         params = None
-
     fcode, fparams = function_spec(func)
-    _lib.check(lib.slq_estimate(
-        gpu_Aop.handle, _lib.ptr(params), q, fcode, _lib.ptr(fparams),
-        int(gram),
-        float(exponent), int(orthogonalize), int(seed), int(lanczos_degree),
-        float(lanczos_tol), int(min_num_samples), int(max_num_samples),
-        float(error_atol), float(error_rtol), float(confidence_level),
-        float(outlier_significance_level), int(block_size),
-        PRECISIONS[precision], _lib.ptr(trace), _lib.ptr(error),
-        _lib.ptr(samples), _lib.ptr(processed_samples_indices),
-        _lib.ptr(num_samples_used), _lib.ptr(num_outliers),
-        _lib.ptr(converged), _lib.ptr(all_converged),
-        _lib.ptr(alg_wall_times)), 'slq_estimate')
+
+    if gpu:
+        gpu_Aop = Aop.get_linear_operator(gpu=True,
+                                          num_gpu_devices=num_gpu_devices)
+        props = gpu_Aop.get_device_properties()
+        num_gpu_multiprocessors = props['num_multiprocessors'][0]
+        num_gpu_threads_per_multiprocessor = \
+            props['num_threads_per_multiprocessor'][0]
+        # one device: the reproducible configuration of the reference
+        num_gpu_devices = 1
+        _lib.check(lib.slq_estimate(
+            gpu_Aop.handle, _lib.ptr(params), q, fcode, _lib.ptr(fparams),
+            int(gram),
+            float(exponent), int(orthogonalize), int(seed),
+            int(lanczos_degree),
+            float(lanczos_tol), int(min_num_samples), int(max_num_samples),
+            float(error_atol), float(error_rtol), float(confidence_level),
+            float(outlier_significance_level), int(block_size),
+            PRECISIONS[precision], _lib.ptr(trace), _lib.ptr(error),
+            _lib.ptr(samples), _lib.ptr(processed_samples_indices),
+            _lib.ptr(num_samples_used), _lib.ptr(num_outliers),
+            _lib.ptr(converged), _lib.ptr(all_converged),
+            _lib.ptr(alg_wall_times)), 'slq_estimate')
+    else:
+        # CPU path (trace_estimator.pyx:246-277): the library's host
+        # implementation over num_threads threads; no HIP call
+        cpu_Aop = Aop.get_linear_operator(gpu=False)
+        num_gpu_multiprocessors = 0
+        num_gpu_threads_per_multiprocessor = 0
+        _lib.check(lib.slq_estimate_host(
+            cpu_Aop.handle, _lib.ptr(params), q, fcode, _lib.ptr(fparams),
+            int(gram), float(exponent), int(orthogonalize), int(seed),
+            int(lanczos_degree), float(lanczos_tol), int(min_num_samples),
+            int(max_num_samples), float(error_atol), float(error_rtol),
+            float(confidence_level), float(outlier_significance_level),
+            int(num_threads), _lib.ptr(trace), _lib.ptr(error),
+            _lib.ptr(samples), _lib.ptr(processed_samples_indices),
+            _lib.ptr(num_samples_used), _lib.ptr(num_outliers),
+            _lib.ptr(converged), _lib.ptr(all_converged),
+            _lib.ptr(alg_wall_times)), 'slq_estimate_host')
 
     tot_wall_time = time.perf_counter() - init_wall_time
     cpu_proc_time = time.process_time() - init_proc_time

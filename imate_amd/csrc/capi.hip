@@ -777,8 +777,8 @@ int build_transpose(hipStream_t s, const DevCSR& A, DevCSR& T)
 // ---------------------------------------------------------------- statistics
 
 namespace {
-
 double sign_d(double x) { return (x > 0) - (x < 0); }
+}  // namespace
 
 // erf_inv, special_functions.cpp:63-104: rational initial guess with the
 // published coefficients the reference cites, then two Newton steps.
@@ -922,6 +922,8 @@ int64_t clock_seed()
     return (int64_t) (((uint64_t) ts.tv_sec * 1000000000ULL + (uint64_t) ts.tv_nsec) &
                       0x7fffffffffffffffULL);
 }
+
+namespace {
 
 // Workspace carve-up for one estimate / recurrence call.
 struct Plan {

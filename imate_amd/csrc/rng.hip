@@ -273,6 +273,22 @@ void thread_state(int64_t seed, int thread_id, uint64_t out[4])
     }
 }
 
+void stream_at(const uint64_t base[4], uint64_t k, uint64_t out[4])
+{
+    const auto& t = jump_tables();
+    uint64_t s[4] = {base[0], base[1], base[2], base[3]};
+    for (int e = 0; e < 64; ++e)
+        if ((k >> e) & 1)
+        {
+            uint64_t y[4];
+            apply_h(t[e], s, y);
+            for (int w = 0; w < 4; ++w) s[w] = y[w];
+        }
+    for (int w = 0; w < 4; ++w) out[w] = s[w];
+}
+
+uint64_t stream_next(uint64_t s[4]) { return next_h(s); }
+
 int rng_words(hipStream_t s, const uint64_t base[4], int64_t W,
               int64_t first_sample, int32_t count, uint64_t* words)
 {

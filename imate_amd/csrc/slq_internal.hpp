@@ -8,12 +8,12 @@
 #include <vector>
 
 #include "slq_hip.h"
+#include "estimator.hpp"
 
 namespace slq {
 
 // ---------------------------------------------------------------- errors
 
-void set_error(const char* fmt, ...);
 int hip_status(hipError_t e, const char* what, const char* file, int line);
 
 #define SLQ_CHECK(expr)                                                      \
@@ -221,7 +221,6 @@ int rng_words(hipStream_t s, const uint64_t base_state[4], int64_t W,
               int64_t first_sample, int32_t count, uint64_t* words);
 int rng_expand(hipStream_t s, const uint64_t* words, int64_t W, int64_t len,
                int32_t count, int dtype, void* X);
-void thread_state(int64_t seed, int thread_id, uint64_t out[4]);
 
 // Lanczos / Golub-Kahn over one probe block; outputs alpha/beta [m][kBlock]
 // (FP64, device) and sizes [kBlock] (device).

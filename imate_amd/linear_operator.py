@@ -7,8 +7,9 @@ The operator is uploaded lazily, once, on first GPU use (like
 passing a ``Matrix`` instead of a scipy matrix keeps A resident in HBM across
 calls (the usage imate recommends, logdet/_slq_method.pyx:218-222).
 
-Only the GPU path exists in this package: ``get_linear_operator(gpu=False)``
-raises, since the CPU path of imate is not part of the MI355X build.
+``get_linear_operator(gpu=False)`` makes the host copy used by the CPU
+path (``slq_estimate_host``), like imate's pycMatrix / pycAffineMatrixFunction
+(py_c_matrix.pyx:119-215, py_c_affine_matrix_function.pyx:172-305).
 """
 
 import ctypes
@@ -19,6 +20,59 @@ import scipy.sparse
 from . import _lib
 
 __all__ = ['LinearOperator', 'Matrix', 'AffineMatrixFunction', 'DeviceMatrix']
+
+
+class HostOperator(object):
+    """Owner of one ``slq_host_op*`` handle (host copy of the operator for
+    the CPU path)."""
+
+    def __init__(self, handle, parents=()):
+        self.handle = handle
+        self._parents = parents
+
+    @staticmethod
+    def from_matrix(A):
+        """Host copy of A: dense for a numpy array (imate's cDenseMatrix),
+        CSC for a scipy CSC matrix (cCSCMatrix), CSR otherwise
+        (cCSRMatrix)."""
+        lib = _lib.load()
+        if A.ndim != 2:
+            raise ValueError('Input matrix should be a 2-dimensional array.')
+        code = _lib.host_dtype_code(A.dtype)
+        h = ctypes.c_void_p()
+        if not scipy.sparse.issparse(A):
+            D = numpy.ascontiguousarray(A)
+            _lib.check(lib.slq_host_op_create_dense(
+                D.shape[0], D.shape[1], _lib.ptr(D), code, ctypes.byref(h)),
+                'slq_host_op_create_dense')
+            return HostOperator(h)
+        if A.format == 'csc':
+            fn, name, M = lib.slq_host_op_create_csc, 'slq_host_op_create_csc', A
+        else:
+            fn, name, M = (lib.slq_host_op_create_csr, 'slq_host_op_create_csr',
+                           scipy.sparse.csr_matrix(A))
+        indptr = numpy.ascontiguousarray(M.indptr, dtype=numpy.int64)
+        indices = numpy.ascontiguousarray(M.indices, dtype=numpy.int32)
+        data = numpy.ascontiguousarray(M.data, dtype=A.dtype)
+        _lib.check(fn(M.shape[0], M.shape[1], M.nnz, _lib.ptr(indptr),
+                      _lib.ptr(indices), _lib.ptr(data), code,
+                      ctypes.byref(h)), name)
+        return HostOperator(h)
+
+    @staticmethod
+    def affine(A_host, B_host=None):
+        lib = _lib.load()
+        h = ctypes.c_void_p()
+        _lib.check(lib.slq_host_op_create_affine(
+            A_host.handle, B_host.handle if B_host is not None else None,
+            ctypes.byref(h)), 'slq_host_op_create_affine')
+        return HostOperator(h, parents=(A_host, B_host))
+
+    def __del__(self):
+        h = getattr(self, 'handle', None)
+        if h is not None and h.value and _lib._lib is not None:
+            _lib._lib.slq_host_op_destroy(h)
+            self.handle = None
 
 
 class DeviceOperator(object):
@@ -185,9 +239,8 @@ class LinearOperator(object):
         return self.num_parameters
 
     def _initialize_cpu(self):
-        raise NotImplementedError(
-            'imate_amd implements the GPU path only (gpu=True); the CPU '
-            'path (gpu=False) is served by imate itself.')
+        raise NotImplementedError('This method should be called by a '
+                                  'derived class.')
 
     def _nnz(self, A):
         if scipy.sparse.issparse(A):
@@ -226,6 +279,11 @@ class Matrix(LinearOperator):
         self.gpu = gpu
         self.num_gpu_devices = num_gpu_devices
 
+    def _initialize_cpu(self):
+        if not self.initialized_on_cpu:
+            self.cpu_Aop = HostOperator.from_matrix(self.A)
+            self.initialized_on_cpu = True
+
     def set_values(self, data):
         """Replace the stored values of A (same sparsity pattern, CSR order)
         in place, also on the device, without re-uploading the pattern or
@@ -234,6 +292,9 @@ class Matrix(LinearOperator):
         self.A, data = _new_values(self.A, data)
         if self.initialized_on_gpu:
             self.gpu_Aop.set_values(data)
+        # the host copy is made again on its next use
+        self.cpu_Aop = None
+        self.initialized_on_cpu = False
 
     def get_num_rows(self):
         return self.A.shape[0]
@@ -286,10 +347,24 @@ class AffineMatrixFunction(LinearOperator):
         self.gpu = gpu
         self.num_gpu_devices = num_gpu_devices
 
+    def _initialize_cpu(self):
+        if not self.initialized_on_cpu:
+            A_host = HostOperator.from_matrix(self.A)
+            B_host = None
+            if self.B is not None:
+                B = self.B
+                if B.dtype != self.A.dtype:
+                    B = B.astype(self.A.dtype)
+                B_host = HostOperator.from_matrix(B)
+            self.cpu_Aop = HostOperator.affine(A_host, B_host)
+            self.initialized_on_cpu = True
+
     def set_values(self, data):
         """Replace the stored values of A (not B) in place; see
         ``Matrix.set_values``."""
         self.A, data = _new_values(self.A, data)
+        self.cpu_Aop = None
+        self.initialized_on_cpu = False
         if self.initialized_on_gpu:
             self._A_dev.set_values(data)
 
@@ -358,6 +433,12 @@ class DeviceMatrix(LinearOperator):
             self._initialize_cpu()
         self.gpu = gpu
         self.num_gpu_devices = num_gpu_devices
+
+    def _initialize_cpu(self):
+        # the CPU path needs a host copy: one copy back from the device
+        if not self.initialized_on_cpu:
+            self.cpu_Aop = HostOperator.from_matrix(self.to_csr())
+            self.initialized_on_cpu = True
 
     def get_num_rows(self):
         return self.shape[0]

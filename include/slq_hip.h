@@ -357,6 +357,60 @@ int slq_profile_steps(slq_op* op, int64_t* steps);
  * the achievable HBM bandwidth. No reference counterpart. */
 int slq_copy_bandwidth(int64_t bytes, int32_t iters, double* ms_per_launch);
 
+/* ------------------------------------------------------------ CPU path
+ *
+ * imate's gpu=False branch (trace_estimator.pyx:246-277) on the host cores,
+ * in this library's own form: probes in blocks of 8 stored row-interleaved,
+ * one multiply of the operator per block and step, rows split over
+ * num_threads worker threads with reductions summed in a fixed order (the
+ * result does not depend on the thread count). Same probes as slq_estimate
+ * (thread-0 stream), same eigensolve and quadrature code as the device path.
+ * No HIP call: usable on a machine without a GPU. Host handles copy the
+ * caller's arrays. */
+
+#define SLQ_F128 2  /* long double (numpy float128); host handles only */
+
+typedef struct slq_host_op slq_host_op;
+
+/* Host copy of a CSR / CSC matrix (pycMatrix.set_csr_matrix_* /
+ * set_csc_matrix_*, imate/_c_linear_operator/py_c_matrix.pyx:119-215; the
+ * matrices of c_csr_matrix.cpp / c_csc_matrix.cpp). A CSC matrix is kept as
+ * the CSR matrix it equals. dtype SLQ_F64, SLQ_F32 or SLQ_F128. */
+int slq_host_op_create_csr(int64_t nrows, int64_t ncols, int64_t nnz,
+                           const int64_t* indptr, const int32_t* indices,
+                           const void* data, int32_t dtype, slq_host_op** out);
+int slq_host_op_create_csc(int64_t nrows, int64_t ncols, int64_t nnz,
+                           const int64_t* indptr, const int32_t* indices,
+                           const void* data, int32_t dtype, slq_host_op** out);
+/* Host copy of a row-major dense matrix (pycMatrix.set_dense_matrix_*,
+ * py_c_matrix.pyx:216-380; c_dense_matrix.cpp). */
+int slq_host_op_create_dense(int64_t nrows, int64_t ncols, const void* data,
+                             int32_t dtype, slq_host_op** out);
+/* A + t B; B == NULL means the identity (pycAffineMatrixFunction,
+ * imate/_c_linear_operator/py_c_affine_matrix_function.pyx:172-305). B must
+ * have A's shape and dtype. */
+int slq_host_op_create_affine(slq_host_op* A, slq_host_op* B, slq_host_op** out);
+int slq_host_op_destroy(slq_host_op* op);
+
+/* SLQ trace estimation on the host: replaces pyc_trace_estimator /
+ * cTraceEstimator<T>::c_trace_estimator
+ * (imate/_c_trace_estimator/py_c_trace_estimator.pyx:39-172,
+ * c_trace_estimator.cpp:191-307,410-~600). Arguments and outputs as
+ * slq_estimate, with num_threads host threads (>= 1) in place of the
+ * device's block size and precision. Samples equal those of imate's CPU
+ * path with one thread, and those of slq_estimate, to rounding. */
+int slq_estimate_host(slq_host_op* op, const double* params, int32_t num_inquiries,
+                      int32_t func, const double* func_params, int32_t gram,
+                      double exponent, int32_t orthogonalize, int64_t seed,
+                      int32_t lanczos_degree, double lanczos_tol,
+                      int32_t min_num_samples, int32_t max_num_samples,
+                      double error_atol, double error_rtol, double confidence_level,
+                      double outlier_significance_level, int32_t num_threads,
+                      double* trace, double* error, double* samples,
+                      int32_t* processed_samples_indices, int32_t* num_samples_used,
+                      int32_t* num_outliers, int32_t* converged,
+                      int32_t* all_converged, double* alg_wall_time);
+
 #ifdef __cplusplus
 }
 #endif

@@ -121,11 +121,6 @@ def test_non_square_without_gram_rejected():
         imate_amd.logdet(A, method='slq', gpu=True)
 
 
-def test_gpu_false_is_not_served():
-    with pytest.raises(NotImplementedError):
-        imate_amd.logdet(A_SMALL, method='slq', gpu=False)
-
-
 def test_other_methods_are_out_of_scope():
     with pytest.raises(NotImplementedError):
         imate_amd.logdet(A_SMALL, method='cholesky')

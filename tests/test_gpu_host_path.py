@@ -1,0 +1,55 @@
+"""gpu=False (the library's host path) against gpu=True on the same seeded
+probes: per sample, FP64, 1e-10 relative; FP32 at 2e-5 (both are within
+3.4e-6 of FP64 on these matrices)."""
+
+import numpy
+import pytest
+import scipy.sparse
+
+import imate_amd
+from imate_amd.sample_matrices import sparse_correlation_matrix, sparse_gaussian
+
+pytestmark = pytest.mark.gpu
+
+
+def _pair(fn, A, **kw):
+    kw = dict(method='slq', min_num_samples=48, max_num_samples=48, seed=11,
+              return_info=True, **kw)
+    _, ic = fn(A, gpu=False, num_threads=8, **kw)
+    _, ig = fn(A, gpu=True, **kw)
+    return (numpy.asarray(ic['convergence']['samples'], dtype=numpy.float64),
+            numpy.asarray(ig['convergence']['samples'], dtype=numpy.float64))
+
+
+def _rel(a, b):
+    return float(numpy.max(numpy.abs(a - b) / numpy.maximum(1.0, numpy.abs(b))))
+
+
+@pytest.mark.parametrize('ortho', [0, -1])
+def test_lanczos_cpu_equals_gpu(ortho):
+    A = sparse_correlation_matrix(20000, nnz_per_row=30, seed=2)
+    c, g = _pair(imate_amd.logdet, A, lanczos_degree=30, orthogonalize=ortho)
+    assert _rel(c, g) < 1e-10
+
+
+def test_golub_kahn_cpu_equals_gpu():
+    A = sparse_gaussian(6000, 2500, 0.01, seed=3) * 0.3
+    c, g = _pair(imate_amd.traceinv, A, gram=True, lanczos_degree=20)
+    assert _rel(c, g) < 1e-10
+
+
+def test_affine_cpu_equals_gpu():
+    A = sparse_correlation_matrix(8000, nnz_per_row=30, seed=4)
+    B = scipy.sparse.diags(numpy.linspace(0.5, 1.5, 8000)).tocsr()
+    ts = [0.0, 0.5, 2.0]
+    for op in (imate_amd.AffineMatrixFunction(A),
+               imate_amd.AffineMatrixFunction(A, B)):
+        c, g = _pair(imate_amd.logdet, op, parameters=ts, lanczos_degree=20)
+        assert _rel(c, g) < 1e-10
+
+
+def test_float32_cpu_equals_gpu():
+    A = sparse_correlation_matrix(20000, nnz_per_row=30, seed=5,
+                                  dtype='float32')
+    c, g = _pair(imate_amd.logdet, A, lanczos_degree=20)
+    assert _rel(c, g) < 2e-5
