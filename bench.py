@@ -119,6 +119,24 @@ def cpu_baseline(A, m, threads, seed, per_thread=8):
     return steps / dt, dt, steps
 
 
+def cpu_path(A, m, threads, seed, probes=64):
+    """This library's own CPU path (imate_amd with gpu=False, the host
+    implementation slq_estimate_host) on the same matrix and cores: probes
+    in blocks of 8 per pass over A. Not the baseline (that is the oracle
+    above); reported beside it."""
+    import imate_amd
+    M = imate_amd.Matrix(A)
+    _, info = imate_amd.logdet(M, method='slq', gpu=False, lanczos_degree=m,
+                               min_num_samples=probes, max_num_samples=probes,
+                               seed=seed, num_threads=threads,
+                               return_info=True)
+    dt = info['time']['alg_wall_time']
+    return {'value': probes * m / dt, 'unit': 'matvec-steps/s',
+            'cores': threads,
+            'sample': '%d probes x %d Lanczos steps in %.1fs (sampling loop)'
+                      % (probes, m, dt)}
+
+
 def cpu_calibration():
     """The port's speed relative to imate's own CPU Lanczos loop (imate's
     sources compiled into oracle/_ref), measured on the same cores in the
@@ -827,6 +845,9 @@ def main():
         if cal is not None:
             cpu['calibration'] = cal
         log('cpu baseline: %.1f matvec-steps/s on %d threads' % (rate, threads))
+        cpu['product_cpu_path'] = cpu_path(A, m, threads, args.seed)
+        log('gpu=False path: %.1f matvec-steps/s on %d threads'
+            % (cpu['product_cpu_path']['value'], threads))
 
     out = {
         'metric': 'Lanczos matvec-steps/s (plain FP64 GPU SLQ logdet)',
