@@ -216,6 +216,8 @@ def precision_form(args, compact=False):
                       'value read (0.5 flop/B FP32, 0.25 flop/B FP64), '
                       'bound by HBM; the Gram-Schmidt kernels run on the '
                       'vector ALUs at 3.8-5.5 TB/s'}}
+    out['reorthogonalization']['block_krylov'] = block_krylov_leg(
+        Aop, s, m, args.seed, exact, ('mixed',) if compact else ('mixed', 'native'))
     if not compact:
         out['exact'] = exact
     else:
@@ -223,6 +225,58 @@ def precision_form(args, compact=False):
         out['mixed_over_native_speed'] = (mix['logdet_matvec_steps_per_s'] /
                                           nat['logdet_matvec_steps_per_s'])
     return out
+
+
+def block_krylov_leg(Aop, s, m, seed, exact, precisions):
+    """The block Lanczos form (krylov='block') on the precision-form matrix:
+    the 64 probes of a block share one basis, so its full
+    re-orthogonalisation is a GEMM against the stored basis, run on the
+    matrix cores (bk_tn_kernel: Q^T W; bk_nn_kernel: W -= Q H). Reports the
+    call time (unprofiled run), the logdet error, and the GEMM kernels'
+    device time and flops from HIP events around every launch (profiled
+    run): achieved TFLOP/s and their fraction of the MI355X dense matrix
+    peak of the GEMM's type (FP32 157.3, FP64 78.6 TFLOP/s)."""
+    import ctypes
+    import imate_amd
+    from imate_amd import _lib
+    lib = _lib.load()
+    h = Aop.get_linear_operator(gpu=True).handle
+    rows = []
+    for prec in precisions:
+        kw = dict(method='slq', gpu=True, krylov='block', lanczos_degree=m,
+                  min_num_samples=s, max_num_samples=s, seed=seed,
+                  precision=prec, return_info=True)
+        imate_amd.logdet(Aop, **kw)     # warm-up: workspace, rocSOLVER load
+        t0 = time.perf_counter()
+        ld, info = imate_amd.logdet(Aop, **kw)
+        el = time.perf_counter() - t0
+        _lib.check(lib.slq_profile_enable(h, 1), 'slq_profile_enable')
+        imate_amd.logdet(Aop, **kw)
+        ms = ctypes.c_double(0.0)
+        nl = ctypes.c_int64(0)
+        fl = ctypes.c_double(0.0)
+        _lib.check(lib.slq_profile_gemm(h, ctypes.byref(ms), ctypes.byref(nl),
+                                        ctypes.byref(fl)), 'slq_profile_gemm')
+        _lib.check(lib.slq_profile_enable(h, 0), 'slq_profile_enable')
+        peak = 157.3 if prec == 'mixed' else 78.6
+        tflops = fl.value / (ms.value * 1e-3) / 1e12 if ms.value > 0 else None
+        r = {'precision': prec, 'gemm_dtype': 'f32' if prec == 'mixed' else 'f64',
+             'seconds': el, 'logdet_matvec_steps_per_s': s * m / el,
+             'logdet': ld, 'logdet_mc_err': float(info['error']['absolute_error']),
+             'gemm_ms': ms.value, 'gemm_launches': nl.value,
+             'gemm_flops': fl.value, 'gemm_tflops': tflops,
+             'mfma_peak_tflops': peak,
+             'gemm_mfma_fraction': tflops / peak if tflops else None}
+        if exact:
+            r['logdet_rel_err'] = abs(ld - exact['logdet']) / abs(exact['logdet'])
+        log('block Krylov form: %s' % json.dumps(r))
+        rows.append(r)
+    return {'lanczos_degree': m, 'block': 64,
+            'tridiagonal_size': 64 * m,
+            'note': 'opt-in form (its samples differ from imate\'s per-probe '
+                    'Lanczos); the block tridiagonal eigensolve (rocSOLVER '
+                    'dsyevd, 64 m = 3840) costs about as much as the GEMMs',
+            'runs': rows}
 
 
 def batched_form(args, compact=False):

@@ -68,6 +68,9 @@ SIGNATURES = {
         _P, _P, _i32, _i32, _P, _i32, _f64, _i32, _i64, _i32, _f64, _i32, _i32,
         _f64, _f64, _f64, _f64, _i32, _i32, _P, _P, _P, _P, _P, _P, _P, _P,
         _P]),
+    'slq_estimate_block_krylov': (ctypes.c_int, [
+        _P, _P, _i32, _i32, _P, _f64, _i64, _i32, _i32, _i32, _f64, _f64, _f64,
+        _f64, _i32, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
     'slq_rademacher': (ctypes.c_int, [_i64, _i64, _i64, _i32, ctypes.c_int,
                                       _P, _P]),
     'slq_op_spmm': (ctypes.c_int, [_P, _f64, _i32, _i32, _P, _P]),
@@ -79,6 +82,7 @@ SIGNATURES = {
     'slq_profile_enable': (ctypes.c_int, [_P, _i32]),
     'slq_profile_read': (ctypes.c_int, [_P, _P, _P, _P, _P]),
     'slq_profile_steps': (ctypes.c_int, [_P, _P]),
+    'slq_profile_gemm': (ctypes.c_int, [_P, _P, _P, _P]),
     'slq_copy_bandwidth': (ctypes.c_int, [_i64, _i32, _P]),
     'slq_estimate_batch': (ctypes.c_int, [
         _P, _i32, _i32, _P, _i32, _f64, _i32, _i64, _i32, _f64, _i32, _f64, _f64,

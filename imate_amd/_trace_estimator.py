@@ -284,13 +284,17 @@ def trace_estimator(A, parameters, func, gram, exponent, min_num_samples,
                     max_num_samples, error_atol, error_rtol, confidence_level,
                     outlier_significance_level, lanczos_degree, lanczos_tol,
                     orthogonalize, seed, num_threads, num_gpu_devices,
-                    verbose, plot, gpu, precision=None, block_size=0):
+                    verbose, plot, gpu, precision=None, block_size=0,
+                    krylov=None):
     """SLQ estimate of tr f((A + tB)^p) (trace_estimator.pyx:38-390).
 
     ``func`` is 'log', 'inv', 'identity' or a (name, coefficients) pair
     (see ``function_spec``). ``precision`` ('native' or
-    'mixed') and ``block_size`` (probes per device block, <= 64) are options
-    of this package; their defaults reproduce the reference behaviour.
+    'mixed'), ``block_size`` (probes per device block, <= 64) and ``krylov``
+    (None or 'single': one Lanczos recurrence per probe, imate's method;
+    'block': block Lanczos over each 64-probe block with a shared, fully
+    re-orthogonalised basis, ``slq_estimate_block_krylov``) are options of
+    this package; their defaults reproduce the reference behaviour.
     """
     if num_threads < 1:
         import os
@@ -323,6 +327,11 @@ def trace_estimator(A, parameters, func, gram, exponent, min_num_samples,
     if plot:
         raise NotImplementedError('plot=True (matplotlib convergence plot) '
                                   'is not part of imate_amd.')
+    if krylov not in (None, 'single', 'block'):
+        raise ValueError('"krylov" should be None, "single" or "block".')
+    block_krylov = krylov == 'block'
+    if block_krylov and (not gpu or gram):
+        raise ValueError('krylov="block" needs gpu=True and gram=False.')
     if not gpu and PRECISIONS[precision] != _lib.PREC_NATIVE:
         raise ValueError('"precision" other than "native" needs gpu=True.')
     if gpu and data_type_name == b'float128':
@@ -363,6 +372,19 @@ def trace_estimator(A, parameters, func, gram, exponent, min_num_samples,
             props['num_threads_per_multiprocessor'][0]
         # one device: the reproducible configuration of the reference
         num_gpu_devices = 1
+    if gpu and block_krylov:
+        _lib.check(lib.slq_estimate_block_krylov(
+            gpu_Aop.handle, _lib.ptr(params), q, fcode, _lib.ptr(fparams),
+            float(exponent), int(seed), int(lanczos_degree),
+            int(min_num_samples), int(max_num_samples), float(error_atol),
+            float(error_rtol), float(confidence_level),
+            float(outlier_significance_level), PRECISIONS[precision],
+            _lib.ptr(trace), _lib.ptr(error), _lib.ptr(samples),
+            _lib.ptr(processed_samples_indices), _lib.ptr(num_samples_used),
+            _lib.ptr(num_outliers), _lib.ptr(converged),
+            _lib.ptr(all_converged), _lib.ptr(alg_wall_times)),
+            'slq_estimate_block_krylov')
+    elif gpu:
         _lib.check(lib.slq_estimate(
             gpu_Aop.handle, _lib.ptr(params), q, fcode, _lib.ptr(fparams),
             int(gram),

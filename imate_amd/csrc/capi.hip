@@ -389,8 +389,40 @@ void prof_end(slq_op* op, hipEvent_t ev0, int kind)
     (kind == 0 ? op->prof.spmm_pending : op->prof.update_pending).push_back({ev0, e});
 }
 
+void prof_end_gemm(slq_op* op, hipEvent_t ev0, double flops)
+{
+    if (!ev0) return;
+    hipEvent_t e;
+    if (!op->prof.pool.empty())
+    {
+        e = op->prof.pool.back();
+        op->prof.pool.pop_back();
+    }
+    else if (hipEventCreate(&e) != hipSuccess)
+        return;
+    (void) hipEventRecord(e, op->stream);
+    op->prof.gemm_pending.push_back({ev0, e});
+    op->prof.gemm_pending_flops.push_back(flops);
+}
+
 void prof_collect(slq_op* op)
 {
+    for (size_t k = 0; k < op->prof.gemm_pending.size(); ++k)
+    {
+        auto& pr = op->prof.gemm_pending[k];
+        float ms = 0.f;
+        if (hipEventSynchronize(pr.second) == hipSuccess &&
+            hipEventElapsedTime(&ms, pr.first, pr.second) == hipSuccess)
+        {
+            op->prof.gemm_ms += ms;
+            op->prof.gemm_launches += 1;
+            op->prof.gemm_flops += op->prof.gemm_pending_flops[k];
+        }
+        op->prof.pool.push_back(pr.first);
+        op->prof.pool.push_back(pr.second);
+    }
+    op->prof.gemm_pending.clear();
+    op->prof.gemm_pending_flops.clear();
     for (int kind = 0; kind < 2; ++kind)
     {
         auto& pend = kind == 0 ? op->prof.spmm_pending : op->prof.update_pending;
@@ -2222,6 +2254,7 @@ int slq_op_destroy(slq_op* op)
     if (op->aux_stream) (void) hipStreamSynchronize(op->aux_stream);
     prof_collect(op);
     for (hipEvent_t e : op->prof.pool) (void) hipEventDestroy(e);
+    release_solver(op);
     if (op->stream) (void) hipStreamDestroy(op->stream);
     for (hipStream_t ls : op->lane_streams) (void) hipStreamDestroy(ls);
     if (op->aux_stream) (void) hipStreamDestroy(op->aux_stream);
@@ -2250,6 +2283,8 @@ struct EstimateArgs {
     double *trace, *error, *samples;
     int32_t *processed, *num_used, *num_outliers, *converged, *all_converged;
     double* alg_wall_time;
+    // 1: block Lanczos over each 64-probe block (slq_estimate_block_krylov)
+    int32_t krylov = 0;
 };
 
 int check_estimate(const char* who, const slq_op* op, const EstimateArgs& a)
@@ -2394,7 +2429,7 @@ int estimate_loop(slq_op* op, const Plan& p, EstimateArgs a)
     // than min_num_samples processed by its end) the next block's probes
     // and recurrence are queued before the host waits for it. The kernels
     // and their inputs are those of the plain order, so results are equal.
-    const bool pipelined = p.pipeline && p.lanes == 0 && required == 1;
+    const bool pipelined = p.pipeline && p.lanes == 0 && required == 1 && !a.krylov;
     if (pipelined && !op->aux_stream)
         SLQ_CHECK(hipStreamCreateWithFlags(&op->aux_stream, hipStreamNonBlocking));
     hipStream_t qstream = pipelined ? op->aux_stream : op->stream;
@@ -2425,6 +2460,33 @@ int estimate_loop(slq_op* op, const Plan& p, EstimateArgs a)
     auto enqueue = [&](int64_t s0, int par) -> int {
         const int cnt = (int) std::min<int64_t>(bs, a.max_s - s0);
         std::fill(ran[par].begin(), ran[par].end(), 0);
+        if (a.krylov)
+        {
+            // every probe of the block (also past max_num_samples): the
+            // block's basis needs all 64
+            SLQ_TRY(make_probes(op, p, base, s0, kBlock, vdt));
+            for (int j = 0; j < required; ++j)
+            {
+                if (!relation && converged[j]) continue;
+                BlockKrylovArgs ba;
+                ba.op = op;
+                ba.t = op->kind != OP_MATRIX ? a.params[j] : 0.0;
+                ba.m = a.m;
+                ba.value_dtype = vdt;
+                ba.X0 = at<void>(p, p.off_X0);
+                ba.func = a.func;
+                ba.func_params = a.func_params;
+                ba.exponent = a.exponent;
+                ba.params_dev = relation ? params_dev : nullptr;
+                ba.q = q;
+                ba.shift = relation ? 1 : 0;
+                ba.j_out = j;
+                ba.q_stride = q;
+                ba.samples_dev = at<double>(p, p.samples_of(par));
+                SLQ_TRY(run_block_krylov(ba, at<void>(p, p.off_scratch), p.scratch_bytes));
+            }
+            return SLQ_OK;
+        }
         SLQ_TRY(make_probes(op, p, base, s0, cnt, vdt));
         if (p.lanes > 0)
             return lanes_block(op, p, a, cnt, nq, count_steps, block_sizes[par], ran[par]);
@@ -2675,6 +2737,41 @@ int slq_estimate_noalloc(slq_op* op, const double* params, int32_t q, int32_t fu
     return estimate_loop(op, p, a);
 }
 
+int slq_estimate_block_krylov(slq_op* op, const double* params, int32_t q, int32_t func,
+                              const double* func_params, double exponent, int64_t seed,
+                              int32_t m, int32_t min_s, int32_t max_s, double atol,
+                              double rtol, double confidence, double outlier,
+                              int32_t precision, double* trace, double* error,
+                              double* samples, int32_t* processed, int32_t* num_used,
+                              int32_t* num_outliers, int32_t* converged,
+                              int32_t* all_converged, double* alg_wall_time)
+{
+    EstimateArgs a{params, q, func, func_params, 0, exponent, -1, seed, m,
+                   0.0, min_s, max_s, atol, rtol, confidence, outlier, kBlock,
+                   precision, trace, error, samples, processed, num_used, num_outliers,
+                   converged, all_converged, alg_wall_time};
+    a.krylov = 1;
+    SLQ_TRY(check_estimate("slq_estimate_block_krylov", op, a));
+    const int64_t n = op->A->csr.nrows;
+    if ((int64_t) m * kBlock > n || m > 256)
+    {
+        set_error("slq_estimate_block_krylov: lanczos_degree x 64 = %lld must not exceed the "
+                  "matrix size %lld (and lanczos_degree must be at most 256)",
+                  (long long) m * kBlock, (long long) n);
+        return SLQ_E_ARG;
+    }
+    DeviceGuard guard(op->device);
+    const int vdt = value_dtype(op, precision);
+    SLQ_TRY(prepare(op, 0, precision));
+    Plan p = make_plan(op, 0, m, 0, q, vdt == SLQ_F64 ? 8 : 4);
+    p.total -= p.scratch_bytes;
+    p.scratch_bytes = block_krylov_scratch_bytes(op, m, vdt == SLQ_F64 ? 8 : 4);
+    p.total += p.scratch_bytes;
+    SLQ_TRY(op->ws.reserve(p.total));
+    p.base = (char*) op->ws.base;
+    return estimate_loop(op, p, a);
+}
+
 int slq_rademacher(int64_t seed, int64_t n, int64_t first_sample, int32_t count,
                    int dtype, void* out, uint64_t* words)
 {
@@ -2874,6 +2971,9 @@ int slq_profile_enable(slq_op* op, int32_t enable)
     op->prof.update_ms = 0.0;
     op->prof.update_launches = 0;
     op->prof.steps = 0;
+    op->prof.gemm_ms = 0.0;
+    op->prof.gemm_launches = 0;
+    op->prof.gemm_flops = 0.0;
     return SLQ_OK;
 }
 
@@ -2923,6 +3023,16 @@ int slq_profile_read(slq_op* op, double* spmm_ms, int64_t* spmm_launches,
     if (spmm_launches) *spmm_launches = op->prof.spmm_launches;
     if (update_ms) *update_ms = op->prof.update_ms;
     if (update_launches) *update_launches = op->prof.update_launches;
+    return SLQ_OK;
+}
+
+int slq_profile_gemm(slq_op* op, double* ms, int64_t* launches, double* flops)
+{
+    if (!op) return SLQ_E_ARG;
+    prof_collect(op);
+    if (ms) *ms = op->prof.gemm_ms;
+    if (launches) *launches = op->prof.gemm_launches;
+    if (flops) *flops = op->prof.gemm_flops;
     return SLQ_OK;
 }
 

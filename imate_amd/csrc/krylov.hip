@@ -2693,6 +2693,51 @@ int run_recurrence(const RecurrenceArgs& ra, void* scratch, size_t scratch_bytes
     return ra.gram ? golub_kahn<double>(ra, bump) : lanczos<double>(ra, bump);
 }
 
+size_t spmm_block_scratch_bytes(const slq_op* op)
+{
+    return redbuf_bytes(sched_tiles(op));
+}
+
+namespace {
+
+template <typename T>
+int spmm_block_t(slq_op* op, double t, const T* X, T* Y, Bump& bump)
+{
+    MatrixStore& A = *op->A;
+    const bool f32 = sizeof(T) == 4;
+    const RedBuf rb = take_redbuf(bump, sched_tiles(op));
+    if (!redbuf_ok(rb))
+    {
+        set_error("internal: SpMM workspace too small");
+        return SLQ_E_OOM;
+    }
+    zero_counters(op->stream, rb);
+    SpmmArgs<T> s = spmm_defaults<T>();
+    fill_matrix<T>(s, A, op->kind == OP_AFFINE_B ? op->B.get() : nullptr, false, f32);
+    s.shift_len = A.csr.nrows < A.csr.ncols ? A.csr.nrows : A.csr.ncols;
+    s.t = t;
+    s.rb = rb;
+    const DevTiles* tl = nullptr;
+    int shift_mode = 0;
+    SLQ_TRY(operand_tiles<T>(op, t, &tl, &shift_mode));
+    s.shift_mode = shift_mode;
+    SpmmIO<T> io;
+    io.X = X;
+    io.Y = Y;
+    return launch_spmm<T>(op, s, io, tl);
+}
+
+}  // namespace
+
+int spmm_block(slq_op* op, double t, int value_dtype, const void* X, void* Y, void* scratch,
+               size_t scratch_bytes)
+{
+    Bump bump{(char*) scratch, scratch_bytes};
+    if (value_dtype == SLQ_F32)
+        return spmm_block_t<float>(op, t, (const float*) X, (float*) Y, bump);
+    return spmm_block_t<double>(op, t, (const double*) X, (double*) Y, bump);
+}
+
 int run_spmm(slq_op* op, double t, int transpose, const void* X, void* Y,
              int value_dtype)
 {

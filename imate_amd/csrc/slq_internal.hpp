@@ -156,6 +156,12 @@ struct ProfileState {
     std::vector<hipEvent_t> pool;            // free events
     std::vector<std::pair<hipEvent_t, hipEvent_t>> spmm_pending;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> update_pending;
+    // matrix-core GEMMs of the block Krylov form (Q^T W and W -= Q H)
+    double gemm_ms = 0.0;
+    int64_t gemm_launches = 0;
+    double gemm_flops = 0.0;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> gemm_pending;
+    std::vector<double> gemm_pending_flops;
 };
 
 // A + tB in the tiled form (the general affine operator): the tile
@@ -209,6 +215,8 @@ struct slq_op {
     // 1 the CSR rows (tuned: 0 = not tuned, else mode + 1)
     int tile_mode = 0;
     int tuned_tile_mode = 0;
+    // rocBLAS handle of the block Krylov form's eigensolver (first use)
+    void* rocblas = nullptr;
 };
 
 namespace slq {
@@ -249,7 +257,41 @@ int run_quadrature(hipStream_t s, const double* alpha, const double* beta,
                    int shift, int j_out, int q_stride, double* samples_dev,
                    double* theta_dev, double* tau_dev);
 
-// Plain block SpMM (tests): Y = (A + tB) X or its transpose.
+// Y = (A + tB) X on one 64-probe block (value type SLQ_F64 or SLQ_F32; the
+// FP32 copy of an FP64 matrix must exist), with the SpMM's counters and
+// partial rows taken from `scratch` (spmm_block_scratch_bytes of them).
+size_t spmm_block_scratch_bytes(const slq_op* op);
+int spmm_block(slq_op* op, double t, int value_dtype, const void* X, void* Y, void* scratch,
+               size_t scratch_bytes);
+
+// Block Lanczos over one 64-probe block with full re-orthogonalisation of
+// the shared basis (block_krylov.hip): m block steps from the probe block
+// X0 ([len][64], value type of the run), eigensolve of the 64 m x 64 m block
+// tridiagonal and block Gauss quadrature; per-probe samples into
+// samples_dev[p * q_stride + col] like run_quadrature (Ritz shift by
+// params_dev[j] - params_dev[0] when shift).
+struct BlockKrylovArgs {
+    slq_op* op = nullptr;
+    double t = 0.0;
+    int m = 0;
+    int value_dtype = SLQ_F64;
+    const void* X0 = nullptr;
+    int func = 0;
+    const double* func_params = nullptr;
+    double exponent = 1.0;
+    const double* params_dev = nullptr;
+    int q = 1;
+    int shift = 0;
+    int j_out = 0;
+    int q_stride = 1;
+    double* samples_dev = nullptr;
+};
+size_t block_krylov_scratch_bytes(const slq_op* op, int m, int value_bytes);
+int run_block_krylov(const BlockKrylovArgs& a, void* scratch, size_t scratch_bytes);
+// Frees the handle's rocBLAS handle (if the block form created one).
+void release_solver(slq_op* op);
+
+// Plain block SpMM (tests): Y = (A + tB) X or its transpose.// Plain block SpMM (tests): Y = (A + tB) X or its transpose.
 int run_spmm(slq_op* op, double t, int transpose, const void* X, void* Y,
              int value_dtype);
 
@@ -266,6 +308,8 @@ int dense_transpose(hipStream_t s, MatrixStore& M);
 // Profiling hooks around kernel launches.
 void prof_begin(slq_op* op, hipEvent_t* ev);
 void prof_end(slq_op* op, hipEvent_t ev0, int kind);
+// end of a timed block Krylov GEMM of `flops` floating-point operations
+void prof_end_gemm(slq_op* op, hipEvent_t ev0, double flops);
 void prof_collect(slq_op* op);
 
 int copy_bandwidth(int64_t bytes, int iters, double* ms);

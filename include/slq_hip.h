@@ -291,6 +291,29 @@ int slq_mlp_predict(const float* params, const float* x, int32_t Q,
 
 /* ---- component entry points (tests, calibration and benchmarks) ---- */
 
+/* Block Lanczos form (an opt-in with no counterpart in imate, which runs
+ * one Lanczos recurrence per probe: c_lanczos_tridiagonalization.cpp:117-235
+ * with c_orthogonalization.cpp:125-224 as its re-orthogonalisation). The 64
+ * probes of a block (consecutive samples of the thread-0 stream) share one
+ * Krylov basis, fully re-orthogonalised by matrix-core GEMMs against all
+ * stored basis blocks; lanczos_degree = m block steps give a 64 m x 64 m
+ * block tridiagonal (eigensolve by rocSOLVER dsyevd, loaded on first use) and
+ * per-probe block Gauss quadrature samples, fed to the reference's
+ * convergence test like slq_estimate's. Arguments and outputs as
+ * slq_estimate, without gram (Lanczos only), orthogonalize (always full),
+ * lanczos_tol and block_size (64). Requires 64 m <= the matrix size. */
+int slq_estimate_block_krylov(slq_op* op, const double* params, int32_t num_inquiries,
+                              int32_t func, const double* func_params, double exponent,
+                              int64_t seed, int32_t lanczos_degree,
+                              int32_t min_num_samples, int32_t max_num_samples,
+                              double error_atol, double error_rtol,
+                              double confidence_level, double outlier_significance_level,
+                              int32_t precision, double* trace, double* error,
+                              double* samples, int32_t* processed_samples_indices,
+                              int32_t* num_samples_used, int32_t* num_outliers,
+                              int32_t* converged, int32_t* all_converged,
+                              double* alg_wall_time);
+
 /* `count` consecutive Rademacher probes of length n, starting at sample
  * `first_sample` of the thread-0 stream of `seed`, written probe-major into
  * host `out[count][n]` (dtype SLQ_F64 / SLQ_F32), and the raw stream words
@@ -351,6 +374,14 @@ int slq_profile_read(slq_op* op, double* spmm_ms, int64_t* spmm_launches,
  * recurrences a sample ran). Like imate's info['convergence'], no reference
  * counterpart for the count itself. */
 int slq_profile_steps(slq_op* op, int64_t* steps);
+
+/* Accumulated device time (ms, HIP events around each launch),
+ * launches and floating-point operations of the block Krylov form's
+ * matrix-core GEMMs (the Q^T W products and the W -= Q H updates of its
+ * re-orthogonalisation and Cholesky QR) while profiling is enabled on the
+ * handle. No counterpart in the reference (its re-orthogonalisation is not
+ * a GEMM: c_orthogonalization.cpp:125-224). */
+int slq_profile_gemm(slq_op* op, double* ms, int64_t* launches, double* flops);
 
 /* Device copy of `bytes` bytes (a plain streaming kernel) timed with HIP
  * events over `iters` launches; used to calibrate rocprof byte counters and
