@@ -26,8 +26,9 @@
 //
 // Matrix-core kernels (v_mfma_f64_16x16x4f64 / v_mfma_f32_16x16x4f32):
 //   bk_tn_kernel: partial 64 x 64 blocks of Q_b^T W, one wavefront per
-//     (row chunk, basis block) holding all 16 output tiles; partial blocks
-//     are added in a fixed order by bk_reduce_kernel (deterministic);
+//     (row chunk, basis block) holding all 16 output tiles, the chunk's
+//     rows of W shared through LDS by four blocks; partial blocks are added
+//     in a fixed order by bk_reduce_kernel (deterministic);
 //   bk_nn_kernel: W -= sum_b Q_b H_b (or Y = X H), one wave per 32 rows,
 //     H_b staged through LDS, the contraction index permuted so that each
 //     lane reads 16 consecutive values of its basis row.
@@ -95,27 +96,33 @@ struct Mma<float> {
 };
 
 // P[c][b] = sum over rows i of chunk c of Q_b[i][p] W[i][q] (64 x 64,
-// row-major in p), Q_b = Q + b * bstride. Grid (ceil(chunks / 4), nkb),
-// 256 threads: wave w of workgroup g takes chunk c = 4 g + w and computes
-// the whole 64 x 64 block over its rows (16 accumulator tiles), so every
-// value of Q_b and W is loaded once. MFMA operands per four rows:
-// A[r][k] = Q_b[i0 + k][16 pb + r], B[k][c] = W[i0 + k][16 qb + c], with
-// r = c = lane % 16 and k = lane / 16. Chunks are a multiple of 8 rows.
+// row-major in p), Q_b = Q + b * bstride. Grid (chunks, ceil(nkb / 4)),
+// 256 threads: the four waves of a workgroup take basis blocks b = 4 y + w
+// of the same chunk, so the chunk's rows of W are staged once through LDS
+// (16 rows at a time, double buffered, the next rows loaded during this
+// step's MFMAs) and serve all four; each wave computes the whole 64 x 64
+// block (16 accumulator tiles) and reads its rows of Q_b once. MFMA
+// operands per four rows: A[r][k] = Q_b[i0 + k][16 pb + r],
+// B[k][c] = W[i0 + k][16 qb + c], with r = c = lane % 16 and k = lane / 16.
+// Chunks are a multiple of 16 rows.
 template <typename T>
 __global__ __launch_bounds__(kBkThreads) void bk_tn_kernel(const T* __restrict__ Q,
                                                            int64_t bstride,
                                                            const T* __restrict__ W,
                                                            int64_t n, int64_t chunk_rows,
-                                                           int nch, int nkb,
-                                                           double* __restrict__ P)
+                                                           int nkb, double* __restrict__ P)
 {
     using V4 = typename Mma<T>::V4;
+    constexpr int kRows = 16;
+    constexpr int kPer = kRows * kB / kBkThreads;   // staged values per thread
+    __shared__ T Ws[2][kRows * kB];
     const int lane = threadIdx.x & 63;
-    const int c = blockIdx.x * 4 + (threadIdx.x >> 6);
-    const int b = blockIdx.y;
-    if (c >= nch) return;
+    const int wave = threadIdx.x >> 6;
+    const int c = blockIdx.x;
+    const int b = blockIdx.y * 4 + wave;
+    const bool active = b < nkb;
     const int r = lane & 15, kq = lane >> 4;
-    const T* __restrict__ Qb = Q + (int64_t) b * bstride;
+    const T* __restrict__ Qb = Q + (int64_t) (active ? b : 0) * bstride;
     const int64_t r0 = (int64_t) c * chunk_rows;
     const int64_t r1 = r0 + chunk_rows < n ? r0 + chunk_rows : n;
     V4 acc[4][4];
@@ -123,51 +130,59 @@ __global__ __launch_bounds__(kBkThreads) void bk_tn_kernel(const T* __restrict__
     for (int pb = 0; pb < 4; ++pb)
 #pragma unroll
         for (int qb = 0; qb < 4; ++qb) acc[pb][qb] = V4{T(0), T(0), T(0), T(0)};
-    // two row groups of four rows per iteration; for FP32 the loads of the
-    // next iteration are issued before this iteration's MFMAs (for FP64 the
-    // extra registers would halve the waves per SIMD, which measured slower)
-    constexpr int kG = 2;
-    constexpr bool kAhead = sizeof(T) == 4;
-    T a[kG][4], w[kG][4];
-    auto load = [&](int64_t i0, T (&ga)[kG][4], T (&gw)[kG][4]) {
+    auto load_w = [&](int64_t i0, T (&v)[kPer]) {
 #pragma unroll
-        for (int g = 0; g < kG; ++g)
+        for (int k = 0; k < kPer; ++k)
         {
-            const int64_t i = i0 + 4 * g + kq;
-            const bool ok = i < r1;
-#pragma unroll
-            for (int x = 0; x < 4; ++x)
-            {
-                ga[g][x] = ok ? Qb[i * kB + x * 16 + r] : T(0);
-                gw[g][x] = ok ? W[i * kB + x * 16 + r] : T(0);
-            }
+            const int e = k * kBkThreads + threadIdx.x;
+            const int64_t i = i0 + e / kB;
+            v[k] = i < r1 ? W[i * kB + e % kB] : T(0);
         }
     };
-    if (kAhead) load(r0, a, w);
-    for (int64_t i0 = r0; i0 < r1; i0 += 4 * kG)
+    T wv[kPer];
+    load_w(r0, wv);
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) Ws[0][k * kBkThreads + threadIdx.x] = wv[k];
+    __syncthreads();
+    int buf = 0;
+    for (int64_t i0 = r0; i0 < r1; i0 += kRows)
     {
-        T an[kG][4], wn[kG][4];
-        if (kAhead) load(i0 + 4 * kG, an, wn);
-        else load(i0, a, w);
+        const bool more = i0 + kRows < r1;
+        if (more) load_w(i0 + kRows, wv);
+        if (active)
+        {
+            T a[4][4];
 #pragma unroll
-        for (int g = 0; g < kG; ++g)
+            for (int g = 0; g < 4; ++g)
+            {
+                const int64_t i = i0 + 4 * g + kq;
+                const bool ok = i < r1;
 #pragma unroll
-            for (int pb = 0; pb < 4; ++pb)
+                for (int x = 0; x < 4; ++x) a[g][x] = ok ? Qb[i * kB + x * 16 + r] : T(0);
+            }
+            const T* __restrict__ ws = Ws[buf];
 #pragma unroll
-                for (int qb = 0; qb < 4; ++qb)
-                    acc[pb][qb] = Mma<T>::op(a[g][pb], w[g][qb], acc[pb][qb]);
-        if (kAhead)
+            for (int g = 0; g < 4; ++g)
+            {
+                T w[4];
+#pragma unroll
+                for (int x = 0; x < 4; ++x) w[x] = ws[(4 * g + kq) * kB + x * 16 + r];
+#pragma unroll
+                for (int pb = 0; pb < 4; ++pb)
+#pragma unroll
+                    for (int qb = 0; qb < 4; ++qb)
+                        acc[pb][qb] = Mma<T>::op(a[g][pb], w[qb], acc[pb][qb]);
+            }
+        }
+        if (more)
         {
 #pragma unroll
-            for (int g = 0; g < kG; ++g)
-#pragma unroll
-                for (int x = 0; x < 4; ++x)
-                {
-                    a[g][x] = an[g][x];
-                    w[g][x] = wn[g][x];
-                }
+            for (int k = 0; k < kPer; ++k) Ws[buf ^ 1][k * kBkThreads + threadIdx.x] = wv[k];
         }
+        buf ^= 1;
+        __syncthreads();
     }
+    if (!active) return;
     double* __restrict__ Pc = P + ((int64_t) c * nkb + b) * kTile;
 #pragma unroll
     for (int pb = 0; pb < 4; ++pb)
@@ -214,7 +229,8 @@ __global__ __launch_bounds__(256) void bk_reduce_kernel(const double* __restrict
 // k-step s of block b the MFMA contracts over p = 16 k + s (k = lane / 16),
 // so lane (r, k) reads X_b[row r][16 k .. 16 k + 15]: 128 B (FP64)
 // contiguous per lane. X and Y must not overlap.
-// 16-row tiles of Y per wave in bk_nn_kernel: two for FP32, one for FP64
+// 16-row tiles of Y per wave in bk_nn_kernel: two for FP32 (four measured
+// slower), one for FP64
 // (whose operand registers for the prefetch would otherwise halve the waves
 // per SIMD).
 template <typename T>
@@ -573,13 +589,15 @@ struct BkBuffers {
     double* U = nullptr;      // [64][N]
 };
 
-// Row chunks (one per wavefront) of a Q^T W product over nkb basis blocks:
-// enough wavefronts to fill the chip, at most 4096 partial blocks.
+// Row chunks of a Q^T W product over nkb basis blocks (one workgroup per
+// chunk and four blocks): about 1024 workgroups, at least 64 chunks, chunks
+// of at least 16 rows.
 int tn_chunks(int64_t n, int nkb)
 {
-    int c = 4096 / nkb;
+    const int groups = (nkb + 3) / 4;
+    int c = 1024 / groups;
     if (c < 64) c = 64;
-    const int64_t most = (n + 7) / 8;
+    const int64_t most = (n + 15) / 16;
     if (c > most) c = (int) most;
     return c < 1 ? 1 : c;
 }
@@ -603,12 +621,12 @@ int tn(slq_op* op, const T* Q, int64_t bstride, int nkb, const T* W, int64_t n,
     const hipStream_t s = op->stream;
     const int nch0 = tn_chunks(n, nkb);
     int64_t chunk_rows = (n + nch0 - 1) / nch0;
-    chunk_rows = (chunk_rows + 7) / 8 * 8;
+    chunk_rows = (chunk_rows + 15) / 16 * 16;
     const int nch = (int) ((n + chunk_rows - 1) / chunk_rows);
     hipEvent_t ev;
     prof_begin(op, &ev);
-    hipLaunchKernelGGL(bk_tn_kernel<T>, dim3((nch + 3) / 4, nkb), dim3(kBkThreads), 0, s, Q,
-                       bstride, W, n, chunk_rows, nch, nkb, bb.P);
+    hipLaunchKernelGGL(bk_tn_kernel<T>, dim3(nch, (nkb + 3) / 4), dim3(kBkThreads), 0, s, Q,
+                       bstride, W, n, chunk_rows, nkb, bb.P);
     prof_end_gemm(op, ev, 2.0 * (double) n * kTile * nkb);
     const int64_t len = (int64_t) nkb * kTile;
     hipLaunchKernelGGL(bk_reduce_kernel, dim3((unsigned) ((len + 63) / 64)), dim3(256), 0, s,
