@@ -275,7 +275,8 @@ def block_krylov_leg(Aop, s, m, seed, exact, precisions):
             'tridiagonal_size': 64 * m,
             'note': 'opt-in form (its samples differ from imate\'s per-probe '
                     'Lanczos); the block tridiagonal eigensolve (rocSOLVER '
-                    'dsyevd, 64 m = 3840) costs about as much as the GEMMs',
+                    'dsyevd of the 64 m x 64 m matrix) takes most of the '
+                    'call, about 2.5 times the GEMMs\' time',
             'runs': rows}
 
 
