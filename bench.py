@@ -70,7 +70,9 @@ def spmm_bytes(n, nnz, b=64, vbytes=8):
     """Algorithmic HBM bytes of one Lanczos SpMM launch: the CSR arrays once
     (values, int32 columns, int64 row pointers), the gathered operand block
     read once and the product block written once:
-    nnz (s_v + 4) + 8 (n + 1) + 2 n b s_v."""
+    nnz (s_v + 4) + 8 (n + 1) + 2 n b s_v. (The epilogue's read of
+    v_{j-1}, n b s_v more after a block's first step, is added where the
+    launch does it.)"""
     return nnz * (vbytes + 4) + (n + 1) * 8 + 2 * n * b * vbytes
 
 
@@ -862,9 +864,15 @@ def main():
     ms_per_step = elapsed / args.steps * 1e3
     avg_spmm = spmm_ms.value / max(spmm_n.value, 1)
     avg_upd = upd_ms.value / max(upd_n.value, 1)
-    bspmm = spmm_bytes(n, A.nnz)
+    # with the beta term in the SpMM epilogue (the library default) every
+    # SpMM after a block's first also reads v_{j-1}'s block once and the
+    # update reads two blocks and writes one; otherwise the update reads
+    # three blocks after the first step
+    beta_in_spmm = os.environ.get('SLQ_SPMM_BETA', '1').strip() != '0'
+    block = n * 64 * 8
+    bspmm = spmm_bytes(n, A.nnz) + (block * (m - 1) / m if beta_in_spmm else 0)
     achieved = bspmm / (avg_spmm * 1e-3) / 1e9
-    upd_bytes = 4 * n * 64 * 8
+    upd_bytes = block * (3 if beta_in_spmm else 3 + (m - 1) / m)
     log('timed %d steps: %.1f ms/step, %.0f matvec-steps/s; SpMM %.3f ms '
         '(%d launches), update %.3f ms (%d)' % (args.steps, ms_per_step, value,
                                                 avg_spmm, spmm_n.value,
@@ -935,6 +943,10 @@ def main():
                      'frac_copy_peak': (achieved / copy_gbps
                                         if copy_gbps else None),
                      'kernel': 'mfma_spmm_kernel<double> + tile sums (one '
+                               'Lanczos SpMM launch; its epilogue also '
+                               'subtracts beta_{j-1} v_{j-1})'
+                               if beta_in_spmm else
+                               'mfma_spmm_kernel<double> + tile sums (one '
                                'Lanczos SpMM launch)',
                      'bytes_per_launch': bspmm,
                      'avg_launch_ms': avg_spmm},

@@ -306,6 +306,8 @@ __device__ __forceinline__ void spmm_epilogue(const SpmmArgs<T>& a, int64_t row,
     if (Vout || a.red_mode == 1 || shift_row) v = s * X[o];
     // imate adds t x through subtract_scaled_vector(-t): y - (-t) x
     if (shift_row) y = y + tt * v;
+    // the reduction term v . (A + tB) v is taken before the subtraction
+    if (a.red_mode == 1) red += (double) v * (double) y;
     if (Sub) y = y - sub_c * (sub_s * Sub[o]);
     if constexpr (NT)
     {
@@ -318,8 +320,7 @@ __device__ __forceinline__ void spmm_epilogue(const SpmmArgs<T>& a, int64_t row,
         if (Vout) Vout[o] = v;
         Y[o] = y;
     }
-    if (a.red_mode == 1) red += (double) v * (double) y;
-    else if (a.red_mode == 2) red += (double) y * (double) y;
+    if (a.red_mode == 2) red += (double) y * (double) y;
 }
 
 // Stores the grid total of the SpMM's reduction term (alpha, or |u|^2 /
@@ -726,16 +727,15 @@ __device__ __forceinline__ void spmm_epilogue4(const SpmmArgs<T>& a, int64_t row
         y[q] = yin[q];
         // imate adds t x through subtract_scaled_vector(-t): y - (-t) x
         if (shift_row) y[q] = y[q] + tt * v[q];
+        // the reduction term v . (A + tB) v is taken before the subtraction
+        if (a.red_mode == 1) red[q] += (double) v[q] * (double) y[q];
         if (Sub) y[q] = y[q] - sub_c[q] * (sub_s[q] * sb[q]);
     }
     if (Vout) store_row4<NT>(Vout + o, v);
     store_row4<NT>(Y + o, y);
 #pragma unroll
     for (int q = 0; q < 4; ++q)
-    {
-        if (a.red_mode == 1) red[q] += (double) v[q] * (double) y[q];
-        else if (a.red_mode == 2) red[q] += (double) y[q] * (double) y[q];
-    }
+        if (a.red_mode == 2) red[q] += (double) y[q] * (double) y[q];
 }
 
 // Tiled SpMM on the matrix cores. A 16-row tile times the gathered operand
@@ -2319,6 +2319,19 @@ int normalize_in_place(slq_op* op, T* X, int64_t n, double* scale)
     return SLQ_OK;
 }
 
+// Where the ring Lanczos subtracts beta_{j-1} v_{j-1}: in the SpMM's
+// epilogue (SLQ_SPMM_BETA = 1, the default: the SpMM reads v_{j-1}'s own
+// rows while it streams its output, and the update kernel then reads two
+// blocks instead of three) or in the update kernel (0).
+bool spmm_beta_enabled()
+{
+    static const bool v = [] {
+        const char* e = std::getenv("SLQ_SPMM_BETA");
+        return !(e && std::atoi(e) == 0);
+    }();
+    return v;
+}
+
 // Lanczos without re-orthogonalisation: the basis is never stored. Step j
 // gathers X_j = r_{j-1} (the previous residual, unnormalised) with its
 // per-probe scale s_j = 1 / beta_{j-1}; the update reads v_j = s_j X_j and
@@ -2353,6 +2366,7 @@ int lanczos_ring(const RecurrenceArgs& ra, Bump& bump)
     int shift_mode = 0;
     SLQ_TRY(operand_tiles<T>(op, ra.t, &tl, &shift_mode));
     const T* X0 = (const T*) ra.X0;
+    const bool beta_in_spmm = spmm_beta_enabled();
     for (int j = 0; j < m; ++j)
     {
         T* W = B[j % 3];
@@ -2371,15 +2385,24 @@ int lanczos_ring(const RecurrenceArgs& ra, Bump& bump)
         SpmmIO<T> io;
         io.X = Xj;
         io.Y = W;
+        // r = (w - beta_{j-1} v_{j-1}) - alpha_j v_j when the epilogue takes
+        // the beta term (alpha_j = v_j . w is summed before it)
+        const bool fused = j > 0 && beta_in_spmm;
+        if (fused)
+        {
+            io.Sub = Xjm1;
+            s.sub_coef = ra.beta + (size_t) (j - 1) * kBlock;
+            s.sub_scale = S[(j + 2) % 3];
+        }
         SLQ_TRY(launch_spmm<T>(op, s, io, tl));
 
         UpdateArgs u;
         std::memset(&u, 0, sizeof(u));
         u.nrows = n;
         u.alpha = ra.alpha + (size_t) j * kBlock;
-        u.beta = j > 0 ? ra.beta + (size_t) (j - 1) * kBlock : nullptr;
+        u.beta = j > 0 && !fused ? ra.beta + (size_t) (j - 1) * kBlock : nullptr;
         u.vscale = S[j % 3];
-        u.uscale = j > 0 ? S[(j + 2) % 3] : nullptr;
+        u.uscale = j > 0 && !fused ? S[(j + 2) % 3] : nullptr;
         u.do_norm = 1;
         u.rb = rb;
         u.beta_out = ra.beta + (size_t) j * kBlock;
@@ -2387,7 +2410,7 @@ int lanczos_ring(const RecurrenceArgs& ra, Bump& bump)
         u.sizes = ra.sizes;
         u.j = j;
         u.tol_sqrt_n = tol_sqrt_n;
-        SLQ_TRY(launch_update<T>(op, u, W, Xj, Xjm1));
+        SLQ_TRY(launch_update<T>(op, u, W, Xj, fused ? nullptr : Xjm1));
     }
     return SLQ_OK;
 }
