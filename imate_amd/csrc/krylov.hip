@@ -2319,10 +2319,10 @@ int normalize_in_place(slq_op* op, T* X, int64_t n, double* scale)
     return SLQ_OK;
 }
 
-// Where the ring Lanczos subtracts beta_{j-1} v_{j-1}: in the SpMM's
-// epilogue (SLQ_SPMM_BETA = 1, the default: the SpMM reads v_{j-1}'s own
-// rows while it streams its output, and the update kernel then reads two
-// blocks instead of three) or in the update kernel (0).
+// Where Lanczos subtracts beta_{j-1} v_{j-1}: in the SpMM's epilogue
+// (SLQ_SPMM_BETA = 1, the default: the SpMM reads v_{j-1}'s own rows while
+// it streams its output, and the update kernel then reads two blocks
+// instead of three) or in the update kernel (0).
 bool spmm_beta_enabled()
 {
     static const bool v = [] {
@@ -2437,6 +2437,7 @@ int lanczos(const RecurrenceArgs& ra, Bump& bump)
     const size_t vst = slot_stride(n);
     T* V = bump.take<T>(vst * nslots);
     double* scale = bump.take<double>(kBlock);
+    double* ones = bump.take<double>(kBlock);   // scale of the stored basis
     const RedBuf rb = take_redbuf(bump, sched_tiles(op));
     double* h1 = nullptr;
     double* h2 = nullptr;
@@ -2447,12 +2448,15 @@ int lanczos(const RecurrenceArgs& ra, Bump& bump)
         h2 = bump.take<double>((size_t) (m + 1) * kBlock);
         partial_k = bump.take<double>((size_t) kMaxGrid * (m + 1) * kBlock);
     }
-    if (!P || !Q || !V || !scale || !redbuf_ok(rb) || (reorth && (!h1 || !h2 || !partial_k)))
+    if (!P || !Q || !V || !scale || !ones || !redbuf_ok(rb) ||
+        (reorth && (!h1 || !h2 || !partial_k)))
     {
         set_error("internal: Lanczos workspace too small");
         return SLQ_E_OOM;
     }
     zero_counters(op->stream, rb);
+    hipLaunchKernelGGL(set_ones_kernel, dim3(1), dim3(kBlock), 0, op->stream, ones);
+    const bool beta_in_spmm = spmm_beta_enabled();
     // initial_beta = |probe| = sqrt(n) exactly for a Rademacher probe
     const double s0 = 1.0 / std::sqrt((double) n);
     hipLaunchKernelGGL(init_block_kernel, dim3(1), dim3(kBlock), 0, op->stream,
@@ -2484,13 +2488,21 @@ int lanczos(const RecurrenceArgs& ra, Bump& bump)
         io.X = src;
         io.Y = W;
         io.Vout = Vj;
+        // beta_{j-1} v_{j-1} subtracted in the epilogue (see lanczos_ring)
+        const bool fused = j > 0 && beta_in_spmm;
+        if (fused)
+        {
+            io.Sub = Vjm1;
+            s.sub_coef = ra.beta + (size_t) (j - 1) * kBlock;
+            s.sub_scale = ones;
+        }
         SLQ_TRY(launch_spmm<T>(op, s, io, tl));
 
         UpdateArgs u;
         std::memset(&u, 0, sizeof(u));
         u.nrows = n;
         u.alpha = ra.alpha + (size_t) j * kBlock;
-        u.beta = j > 0 ? ra.beta + (size_t) (j - 1) * kBlock : nullptr;
+        u.beta = j > 0 && !fused ? ra.beta + (size_t) (j - 1) * kBlock : nullptr;
         u.do_norm = reorth ? 0 : 1;
         u.rb = rb;
         u.beta_out = ra.beta + (size_t) j * kBlock;
@@ -2498,7 +2510,7 @@ int lanczos(const RecurrenceArgs& ra, Bump& bump)
         u.sizes = ra.sizes;
         u.j = j;
         u.tol_sqrt_n = tol_sqrt_n;
-        SLQ_TRY(launch_update<T>(op, u, W, Vj, Vjm1));
+        SLQ_TRY(launch_update<T>(op, u, W, Vj, fused ? nullptr : Vjm1));
 
         if (reorth)
         {
@@ -2675,6 +2687,7 @@ size_t recurrence_scratch_bytes(const slq_op* op, int gram, int m, int ortho,
             add(blk);
             add(blk);
             add(slot_stride(nr) * value_bytes * m);
+            add(sizeof(double) * kBlock);
             add(sizeof(double) * kBlock);
         }
         else
