@@ -705,6 +705,19 @@ def gram_form(args):
            'mc_error_95': float(info['error']['absolute_error']),
            'marchenko_pastur_wishart_mean': mp,
            'rel_diff_vs_wishart_mean': abs(float(val) - mp) / abs(mp)}
+    # the precision form: FP32 copies of A and A^T and FP32 vectors, so each
+    # gathered operand row is 256 B instead of 512 (FP64 recurrences)
+    imate_amd.logdet(D, lanczos_degree=2, min_num_samples=1, max_num_samples=1,
+                     precision='mixed', **kw)
+    t0 = time.perf_counter()
+    valm, infom = imate_amd.logdet(D, lanczos_degree=m, min_num_samples=s,
+                                   max_num_samples=s, precision='mixed', **kw)
+    el_m = time.perf_counter() - t0
+    out['mixed'] = {'seconds': el_m, 'gk_steps_per_s': steps / el_m,
+                    'speedup_vs_fp64': elapsed / el_m, 'logdet': float(valm),
+                    'mc_error_95': float(infom['error']['absolute_error']),
+                    'rel_diff_vs_fp64': abs(float(valm) - float(val)) / abs(float(val)),
+                    'rel_diff_vs_wishart_mean': abs(float(valm) - mp) / abs(mp)}
     log('gram: %s' % json.dumps(out))
     return out
 
