@@ -53,3 +53,18 @@ def test_float32_cpu_equals_gpu():
                                   dtype='float32')
     c, g = _pair(imate_amd.logdet, A, lanczos_degree=20)
     assert _rel(c, g) < 2e-5
+
+
+def test_host_path_full_size_vs_oracle():
+    """gpu=False at the headline size (n = 2^20, 53.3 M non-zeros,
+    lanczos_degree 30): the first 16 samples against the oracle at 1e-10
+    (on the GPU machine's host cores; the CPU path makes no HIP call)."""
+    from oracle import oracle
+    A = sparse_correlation_matrix(1 << 20, nnz_per_row=50.0, seed=0)
+    _, info = imate_amd.logdet(A, method='slq', gpu=False, lanczos_degree=30,
+                               min_num_samples=16, max_num_samples=16, seed=0,
+                               num_threads=16, return_info=True)
+    want, sizes = oracle.first_samples(A, 'log', 30, 16, seed=0, threads=16)
+    got = info['convergence']['samples'][:16]
+    assert numpy.all(sizes == 30)
+    assert _rel(got, want) < 1e-10
