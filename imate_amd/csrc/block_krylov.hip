@@ -589,6 +589,18 @@ struct BkBuffers {
     double* U = nullptr;      // [64][N]
 };
 
+// The per-block outputs of the recurrence that the eigensolve and the
+// quadrature read, one set per parity so that block k's eigensolve can run
+// on a second stream while block k + 1's recurrence runs.
+struct BkParity {
+    double* R0 = nullptr;
+    int* flag = nullptr;
+    double* T = nullptr;
+    double* D = nullptr;
+    double* E = nullptr;
+    double* U = nullptr;
+};
+
 // Row chunks of a Q^T W product over nkb basis blocks (one workgroup per
 // chunk and four blocks): about 1024 workgroups, at least 64 chunks, chunks
 // of at least 16 rows.
@@ -684,55 +696,79 @@ int block_lanczos(const BlockKrylovArgs& a, Carve& bump)
     bb.P = bump.take<double>((size_t) max_partial_blocks(n, m) * kTile);
     bb.H = bump.take<double>((size_t) m * kTile);
     bb.AB = bump.take<double>((size_t) 2 * m * kTile);
-    bb.R0 = bump.take<double>(kTile);
     bb.R1 = bump.take<double>(kTile);
     bb.Rinv = bump.take<double>(kTile);
     bb.G = bump.take<double>(kTile);
-    bb.flag = bump.take<int>(64);
-    bb.T = bump.take<double>((size_t) N * N);
-    bb.D = bump.take<double>((size_t) N);
-    bb.E = bump.take<double>((size_t) N);
-    bb.U = bump.take<double>((size_t) kB * N);
-    if (!Q || !W || !W2 || !bb.P || !bb.H || !bb.AB || !bb.R0 || !bb.R1 || !bb.Rinv || !bb.G ||
-        !bb.flag || !bb.T || !bb.D || !bb.E || !bb.U)
+    BkParity par[2];
+    for (int k = 0; k < 2; ++k)
+    {
+        par[k].R0 = bump.take<double>(kTile);
+        par[k].flag = bump.take<int>(64);
+        par[k].T = bump.take<double>((size_t) N * N);
+        par[k].D = bump.take<double>((size_t) N);
+        par[k].E = bump.take<double>((size_t) N);
+        par[k].U = bump.take<double>((size_t) kB * N);
+    }
+    const BkParity& pp = par[a.par & 1];
+    bb.R0 = pp.R0;
+    bb.flag = pp.flag;
+    bb.T = pp.T;
+    bb.D = pp.D;
+    bb.E = pp.E;
+    bb.U = pp.U;
+    if (!Q || !W || !W2 || !bb.P || !bb.H || !bb.AB || !bb.R1 || !bb.Rinv || !bb.G ||
+        !par[1].R0 || !par[1].flag || !par[1].T || !par[1].D || !par[1].E || !par[1].U)
     {
         set_error("internal: block Krylov workspace too small");
         return SLQ_E_OOM;
     }
+    if (a.flags_out) *a.flags_out = bb.flag;
     void* spmm_scratch = bump.p;
     const size_t spmm_bytes = bump.left;
-    hipLaunchKernelGGL(bk_zero_int_kernel, dim3(1), dim3(64), 0, s, bb.flag, 64);
-    // the B_{m-1} slot is never computed (nor read): keep it defined
+    if (a.phase & 1)
     {
-        const int64_t len = (int64_t) 2 * m * kTile;
-        hipLaunchKernelGGL(bk_zero_kernel, dim3((unsigned) ((len + 255) / 256)), dim3(256), 0,
-                           s, bb.AB, len);
-    }
+        // the rank-loss flag gathers every recurrence of a block
+        if (a.first_in_block)
+            hipLaunchKernelGGL(bk_zero_int_kernel, dim3(1), dim3(64), 0, s, bb.flag, 64);
+        // the B_{m-1} slot is never computed (nor read): keep it defined
+        {
+            const int64_t len = (int64_t) 2 * m * kTile;
+            hipLaunchKernelGGL(bk_zero_kernel, dim3((unsigned) ((len + 255) / 256)),
+                               dim3(256), 0, s, bb.AB, len);
+        }
 
-    SLQ_TRY(cholqr2<T>(op, (const T*) a.X0, W2, Q, n, bb, bb.R0));
-    for (int j = 0; j < m; ++j)
-    {
-        T* Qj = Q + (size_t) j * blk;
-        SLQ_TRY(spmm_block(op, a.t, sizeof(T) == 4 ? SLQ_F32 : SLQ_F64, Qj, W, spmm_scratch,
-                           spmm_bytes));
-        // pass 1: A_j = sym(Q_j^T W), W -= Q H
-        SLQ_TRY(tn<T>(op, Q, (int64_t) blk, j + 1, W, n, bb, bb.H));
-        hipLaunchKernelGGL(bk_sym_kernel, dim3(1), dim3(256), 0, s,
-                           (const double*) (bb.H + (size_t) j * kTile),
-                           bb.AB + (size_t) 2 * j * kTile);
-        SLQ_TRY((nn<T, true>(op, Q, (int64_t) blk, j + 1, bb.H, W, n)));
-        // pass 2
-        SLQ_TRY(tn<T>(op, Q, (int64_t) blk, j + 1, W, n, bb, bb.H));
-        SLQ_TRY((nn<T, true>(op, Q, (int64_t) blk, j + 1, bb.H, W, n)));
-        if (j + 1 < m)
-            SLQ_TRY(cholqr2<T>(op, W, W2, Q + (size_t) (j + 1) * blk, n, bb,
-                               bb.AB + (size_t) (2 * j + 1) * kTile));
+        SLQ_TRY(cholqr2<T>(op, (const T*) a.X0, W2, Q, n, bb, bb.R0));
+        for (int j = 0; j < m; ++j)
+        {
+            T* Qj = Q + (size_t) j * blk;
+            SLQ_TRY(spmm_block(op, a.t, sizeof(T) == 4 ? SLQ_F32 : SLQ_F64, Qj, W, spmm_scratch,
+                               spmm_bytes));
+            // pass 1: A_j = sym(Q_j^T W), W -= Q H
+            SLQ_TRY(tn<T>(op, Q, (int64_t) blk, j + 1, W, n, bb, bb.H));
+            hipLaunchKernelGGL(bk_sym_kernel, dim3(1), dim3(256), 0, s,
+                               (const double*) (bb.H + (size_t) j * kTile),
+                               bb.AB + (size_t) 2 * j * kTile);
+            SLQ_TRY((nn<T, true>(op, Q, (int64_t) blk, j + 1, bb.H, W, n)));
+            // pass 2
+            SLQ_TRY(tn<T>(op, Q, (int64_t) blk, j + 1, W, n, bb, bb.H));
+            SLQ_TRY((nn<T, true>(op, Q, (int64_t) blk, j + 1, bb.H, W, n)));
+            if (j + 1 < m)
+                SLQ_TRY(cholqr2<T>(op, W, W2, Q + (size_t) (j + 1) * blk, n, bb,
+                                   bb.AB + (size_t) (2 * j + 1) * kTile));
+        }
+        // block tridiagonal, eigensolve, quadrature
+        const int64_t NN = N * N;
+        hipLaunchKernelGGL(bk_assemble_kernel, dim3((unsigned) ((NN + 255) / 256)), dim3(256),
+                           0, s, (const double*) bb.AB, m, bb.T);
+        SLQ_CHECK(hipGetLastError());
+        // the eigensolve (phase 2, maybe on another stream) waits for this
+        if (a.qstream && a.qstream != s) SLQ_CHECK(hipEventRecord(a.rec_done, s));
     }
-    // block tridiagonal, eigensolve, quadrature
-    const int64_t NN = N * N;
-    hipLaunchKernelGGL(bk_assemble_kernel, dim3((unsigned) ((NN + 255) / 256)), dim3(256), 0, s,
-                       (const double*) bb.AB, m, bb.T);
-    SLQ_CHECK(hipGetLastError());
+    if (!(a.phase & 2)) return SLQ_OK;
+    // eigensolve and quadrature on the quadrature stream (the handle's
+    // stream when not pipelined), after the recurrence
+    const hipStream_t qs = a.qstream ? a.qstream : s;
+    if (qs != s) SLQ_CHECK(hipStreamWaitEvent(qs, a.rec_done, 0));
     Solver& sv = solver();
     if (!sv.ok)
     {
@@ -751,7 +787,7 @@ int block_lanczos(const BlockKrylovArgs& a, Carve& bump)
         op->rocblas = h;
     }
     rocblas_handle h = (rocblas_handle) op->rocblas;
-    if (sv.set_stream(h, s) != rocblas_status_success ||
+    if (sv.set_stream(h, qs) != rocblas_status_success ||
         sv.dsyevd(h, rocblas_evect_original, rocblas_fill_lower, (rocblas_int) N, bb.T,
                   (rocblas_int) N, bb.D, bb.E, bb.flag + 1) != rocblas_status_success)
     {
@@ -759,7 +795,7 @@ int block_lanczos(const BlockKrylovArgs& a, Carve& bump)
         return SLQ_E_HIP;
     }
     hipLaunchKernelGGL(bk_weights_kernel, dim3((unsigned) ((N * kB + 255) / 256)), dim3(256),
-                       0, s, (const double*) bb.R0, (const double*) bb.T, N, bb.U);
+                       0, qs, (const double*) bb.R0, (const double*) bb.T, N, bb.U);
     BkQuadArgs qa;
     qa.N = N;
     qa.theta = bb.D;
@@ -774,19 +810,9 @@ int block_lanczos(const BlockKrylovArgs& a, Carve& bump)
     qa.j_out = a.j_out;
     qa.q_stride = a.q_stride;
     qa.samples = a.samples_dev;
-    hipLaunchKernelGGL(bk_quadrature_kernel, dim3(kB), dim3(256), 0, s, qa);
+    hipLaunchKernelGGL(bk_quadrature_kernel, dim3(kB), dim3(256), 0, qs, qa);
     SLQ_CHECK(hipGetLastError());
-    // a basis that lost rank makes the block's samples meaningless
-    int flag = 0;
-    SLQ_CHECK(hipMemcpyAsync(&flag, bb.flag, sizeof(int), hipMemcpyDeviceToHost, s));
-    SLQ_CHECK(hipStreamSynchronize(s));
-    if (flag)
-    {
-        set_error("block Krylov: the block basis lost rank (lanczos_degree x 64 = %lld too "
-                  "large for this %lld-row matrix, or a zero probe block)",
-                  (long long) N, (long long) n);
-        return SLQ_E_ARG;
-    }
+    // the caller reads the flags (*flags_out) where it waits for the block
     return SLQ_OK;
 }
 
@@ -801,9 +827,10 @@ size_t block_krylov_scratch_bytes(const slq_op* op, int m, int value_bytes)
     size_t b = r(blk * m) + 2 * r(blk);
     b += r(sizeof(double) * (size_t) max_partial_blocks(n, m) * kTile);
     b += r(sizeof(double) * (size_t) m * kTile) + r(sizeof(double) * (size_t) 2 * m * kTile);
-    b += 4 * r(sizeof(double) * kTile) + r(sizeof(int) * 64);
-    b += r(sizeof(double) * (size_t) N * N) + 2 * r(sizeof(double) * (size_t) N) +
-         r(sizeof(double) * (size_t) kB * N);
+    b += 3 * r(sizeof(double) * kTile);
+    b += 2 * (r(sizeof(double) * kTile) + r(sizeof(int) * 64) +
+              r(sizeof(double) * (size_t) N * N) + 2 * r(sizeof(double) * (size_t) N) +
+              r(sizeof(double) * (size_t) kB * N));
     b += spmm_block_scratch_bytes(op);
     return b;
 }

@@ -2429,7 +2429,7 @@ int estimate_loop(slq_op* op, const Plan& p, EstimateArgs a)
     // than min_num_samples processed by its end) the next block's probes
     // and recurrence are queued before the host waits for it. The kernels
     // and their inputs are those of the plain order, so results are equal.
-    const bool pipelined = p.pipeline && p.lanes == 0 && required == 1 && !a.krylov;
+    const bool pipelined = p.pipeline && p.lanes == 0 && required == 1;
     if (pipelined && !op->aux_stream)
         SLQ_CHECK(hipStreamCreateWithFlags(&op->aux_stream, hipStreamNonBlocking));
     hipStream_t qstream = pipelined ? op->aux_stream : op->stream;
@@ -2443,6 +2443,11 @@ int estimate_loop(slq_op* op, const Plan& p, EstimateArgs a)
         ran[k].assign(count_steps ? required : 0, 0);
     }
     hipEvent_t rec_done[2] = {nullptr, nullptr};
+    // block Krylov form: device flags (rank loss, eigensolver info) of each
+    // parity's block and their host copies, read where the host waits
+    int* bk_flags[2] = {nullptr, nullptr};
+    int bk_flags_host[2] = {0, 0};
+    BlockKrylovArgs bk_args[2];
     struct Events {
         hipEvent_t* e;
         ~Events()
@@ -2465,10 +2470,20 @@ int estimate_loop(slq_op* op, const Plan& p, EstimateArgs a)
             // every probe of the block (also past max_num_samples): the
             // block's basis needs all 64
             SLQ_TRY(make_probes(op, p, base, s0, kBlock, vdt));
+            bool first = true;
             for (int j = 0; j < required; ++j)
             {
                 if (!relation && converged[j]) continue;
                 BlockKrylovArgs ba;
+                ba.par = par;
+                ba.qstream = pipelined ? qstream : nullptr;
+                ba.rec_done = pipelined ? rec_done[par] : nullptr;
+                ba.first_in_block = first;
+                ba.flags_out = &bk_flags[par];
+                // pipelined: the eigensolve is issued by the loop below,
+                // after the next block's recurrence is queued
+                ba.phase = pipelined ? 1 : 3;
+                first = false;
                 ba.op = op;
                 ba.t = op->kind != OP_MATRIX ? a.params[j] : 0.0;
                 ba.m = a.m;
@@ -2484,6 +2499,7 @@ int estimate_loop(slq_op* op, const Plan& p, EstimateArgs a)
                 ba.q_stride = q;
                 ba.samples_dev = at<double>(p, p.samples_of(par));
                 SLQ_TRY(run_block_krylov(ba, at<void>(p, p.off_scratch), p.scratch_bytes));
+                bk_args[par] = ba;
             }
             return SLQ_OK;
         }
@@ -2535,6 +2551,14 @@ int estimate_loop(slq_op* op, const Plan& p, EstimateArgs a)
             SLQ_TRY(enqueue(s1, nxt));
             ahead = true;
         }
+        if (a.krylov && pipelined)
+        {
+            // this block's eigensolve and quadrature (rocSOLVER synchronises
+            // with its stream on the host; the next block is already queued)
+            BlockKrylovArgs f = bk_args[par];
+            f.phase = 2;
+            SLQ_TRY(run_block_krylov(f, at<void>(p, p.off_scratch), p.scratch_bytes));
+        }
         // copies to host memory wait for the stream, so they are issued only
         // here, when the host needs the block (a pipelined block's sizes too)
         if (count_steps && pipelined)
@@ -2543,7 +2567,19 @@ int estimate_loop(slq_op* op, const Plan& p, EstimateArgs a)
                                      qstream));
         SLQ_CHECK(hipMemcpyAsync(samples + s0 * q, at<double>(p, p.samples_of(par)),
                                  sizeof(double) * cnt * q, hipMemcpyDeviceToHost, qstream));
+        if (a.krylov && bk_flags[par])
+            SLQ_CHECK(hipMemcpyAsync(bk_flags_host, bk_flags[par], sizeof(int) * 2,
+                                     hipMemcpyDeviceToHost, qstream));
         SLQ_CHECK(hipStreamSynchronize(qstream));
+        if (a.krylov && bk_flags_host[0])
+        {
+            // a basis that lost rank makes the block's samples meaningless
+            SLQ_CHECK(hipStreamSynchronize(op->stream));
+            set_error("block Krylov: the block basis lost rank (lanczos_degree x 64 = %d too "
+                      "large for this %lld-row matrix)", a.m * kBlock,
+                      (long long) op->A->csr.nrows);
+            return SLQ_E_ARG;
+        }
         for (int pi = 0; pi < cnt; ++pi)
         {
             const int64_t i = s0 + pi;

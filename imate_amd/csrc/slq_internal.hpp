@@ -285,6 +285,21 @@ struct BlockKrylovArgs {
     int j_out = 0;
     int q_stride = 1;
     double* samples_dev = nullptr;
+    // buffers of parity par (two blocks in flight); the eigensolve and
+    // quadrature run on qstream after rec_done (recorded on the handle's
+    // stream) when qstream is set; first_in_block clears the block's flags;
+    // *flags_out receives the device address of its two flags (rank loss,
+    // eigensolver info), to be read once the block is done
+    int par = 0;
+    hipStream_t qstream = nullptr;
+    hipEvent_t rec_done = nullptr;
+    bool first_in_block = true;
+    int** flags_out = nullptr;
+    // 1: recurrence and block tridiagonal (handle's stream); 2: eigensolve
+    // and quadrature; 3: both. Phase 2 of parity par may be issued after
+    // phase 1 of the next block, so that rocSOLVER's host synchronisations
+    // wait while that block's recurrence is already queued.
+    int phase = 3;
 };
 size_t block_krylov_scratch_bytes(const slq_op* op, int m, int value_bytes);
 int run_block_krylov(const BlockKrylovArgs& a, void* scratch, size_t scratch_bytes);
