@@ -125,18 +125,18 @@ class Team {
 public:
     explicit Team(int n) : n_(n)
     {
-        for (int t = 1; t < n; ++t) th_.emplace_back([this, t] { loop(t); });
-    }
-    ~Team()
-    {
+        try
         {
-            std::lock_guard<std::mutex> l(mu_);
-            stop_ = true;
-            ++gen_;
+            for (int t = 1; t < n; ++t) th_.emplace_back([this, t] { loop(t); });
         }
-        go_.notify_all();
-        for (auto& t : th_) t.join();
+        catch (...)
+        {
+            // the threads already started must end before the members go
+            shutdown();
+            throw;
+        }
     }
+    ~Team() { shutdown(); }
     int size() const { return n_; }
 
     void run(const std::function<void(int)>& f)
@@ -171,6 +171,19 @@ public:
     }
 
 private:
+    void shutdown()
+    {
+        {
+            std::lock_guard<std::mutex> l(mu_);
+            stop_ = true;
+            ++gen_;
+        }
+        go_.notify_all();
+        for (auto& t : th_)
+            if (t.joinable()) t.join();
+        th_.clear();
+    }
+
     void loop(int t)
     {
         uint64_t seen = 0;
