@@ -121,3 +121,33 @@ def test_gemm_profile_counts_the_block_products(c1):
     assert nl.value == 2 * (2 + 2 * m + 2 * (m - 1))
     assert fl.value == units * 2.0 * n * 64 * 64
     assert ms.value > 0.0
+
+
+def test_affine_general_b_each_t(c1):
+    """A + tB with a general B: one block recurrence per t (no Ritz shift),
+    run one after the other on the handle's stream."""
+    n = c1.shape[0]
+    B = scipy.sparse.diags(numpy.linspace(0.5, 2.0, n)).tocsr()
+    ts = [0.1, 1.5]
+    op = imate_amd.AffineMatrixFunction(c1, B)
+    val, got = _run(imate_amd.logdet, op, 6, 64, 12, parameters=ts)
+    Z = oracle.rademacher(12, n, 64).T
+    for j, t in enumerate(ts):
+        want = ref_samples((c1 + t * B).tocsr(), Z, 6, numpy.log)[:, 0]
+        assert _rel(got[:, j], want) < 1e-9
+
+
+def test_convergence_stops_early(c1):
+    """min < max: the per-sample convergence test stops the loop at the
+    first sample after which the inquiry converged (blocks still run all 64
+    probes); samples before it are the block form's, later ones NaN."""
+    val, info = imate_amd.traceinv(c1, method='slq', gpu=True, krylov='block',
+                                   lanczos_degree=6, min_num_samples=64,
+                                   max_num_samples=640, error_rtol=0.05,
+                                   seed=13, return_info=True)
+    used = int(info['convergence']['num_samples_used'])
+    assert 64 <= used < 640
+    got = numpy.asarray(info['convergence']['samples'], dtype=numpy.float64)
+    assert numpy.all(numpy.isnan(got[used:]))
+    want = _ref(c1, 13, used, 6, lambda x: 1.0 / x)
+    assert _rel(got[:used].reshape(used, -1), want) < 1e-9
