@@ -276,9 +276,9 @@ def block_krylov_leg(Aop, s, m, seed, exact, precisions):
     return {'lanczos_degree': m, 'block': 64,
             'tridiagonal_size': 64 * m,
             'note': 'opt-in form (its samples differ from imate\'s per-probe '
-                    'Lanczos); the block tridiagonal eigensolve (rocSOLVER '
-                    'dsyevd of the 64 m x 64 m matrix) takes most of the '
-                    'call, about 2.5 times the GEMMs\' time',
+                    'Lanczos); the block tridiagonal eigensolves (64 m x 64 m, '
+                    'one batched rocSOLVER dsyevd call per group of blocks) '
+                    'take the largest part of the call',
             'runs': rows}
 
 

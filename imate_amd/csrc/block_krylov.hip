@@ -16,8 +16,14 @@
 //     H = Q_{0..j}^T W; W -= Q_{0..j} H                    (pass 2)
 //     W = Q_{j+1} B_j                (Cholesky QR twice)
 //   T = block tridiagonal (A_j on the diagonal, B_j below), 64 m x 64 m
-//   T = Y diag(theta) Y^T            (rocSOLVER dsyevd, loaded on first use)
+//   T = Y diag(theta) Y^T            (rocSOLVER, loaded on first use)
 //   sample_p = sum_k f(theta_k^p) (sum_l R_0[l][p] Y[l][k])^2
+//
+// The eigensolve is a chain of small latency-bound launches (the dense
+// tridiagonalisation takes 92 of its 133 ms at 64 m = 3840), so the
+// recurrences of a group of blocks keep their T in slots and one batched
+// rocsolver_dsyevd_strided_batched call solves them all (80 ms per matrix
+// at four, 71 at eight).
 //
 // which is the block Gauss quadrature of z_p^T f(A) z_p (exact for
 // polynomials of degree < 2m, like the scalar rule). The first pass of the
@@ -548,7 +554,7 @@ struct Solver {
     decltype(&rocblas_create_handle) create = nullptr;
     decltype(&rocblas_destroy_handle) destroy = nullptr;
     decltype(&rocblas_set_stream) set_stream = nullptr;
-    decltype(&rocsolver_dsyevd) dsyevd = nullptr;
+    decltype(&rocsolver_dsyevd_strided_batched) dsyevd = nullptr;
 };
 
 std::mutex g_solver_mu;
@@ -567,7 +573,7 @@ Solver& solver()
     s.create = (decltype(s.create)) dlsym(blas, "rocblas_create_handle");
     s.destroy = (decltype(s.destroy)) dlsym(blas, "rocblas_destroy_handle");
     s.set_stream = (decltype(s.set_stream)) dlsym(blas, "rocblas_set_stream");
-    s.dsyevd = (decltype(s.dsyevd)) dlsym(sol, "rocsolver_dsyevd");
+    s.dsyevd = (decltype(s.dsyevd)) dlsym(sol, "rocsolver_dsyevd_strided_batched");
     s.ok = s.create && s.destroy && s.set_stream && s.dsyevd;
     return s;
 }
@@ -578,27 +584,22 @@ struct BkBuffers {
     double* P = nullptr;      // partial blocks [max_chunks * m][4096]
     double* H = nullptr;      // [m][4096]
     double* AB = nullptr;     // [2 m][4096]
-    double* R0 = nullptr;
+    double* R0 = nullptr;     // R_0 of the recurrence's slot
     double* R1 = nullptr;
     double* Rinv = nullptr;
     double* G = nullptr;
-    int* flag = nullptr;      // [0] rank loss, [1] eigensolver info
-    double* T = nullptr;      // [N][N]
-    double* D = nullptr;      // [N]
-    double* E = nullptr;      // [N]
-    double* U = nullptr;      // [64][N]
+    int* flag = nullptr;      // [0] rank loss, [1 + s] eigensolver info of slot s
 };
 
-// The per-block outputs of the recurrence that the eigensolve and the
-// quadrature read, one set per parity so that block k's eigensolve can run
-// on a second stream while block k + 1's recurrence runs.
-struct BkParity {
-    double* R0 = nullptr;
-    int* flag = nullptr;
-    double* T = nullptr;
-    double* D = nullptr;
-    double* E = nullptr;
-    double* U = nullptr;
+// The block tridiagonals of a group, one slot each, laid out for the
+// batched eigensolve (T, D and E strided by slot), with each slot's R_0.
+struct BkSlots {
+    int count = 0;
+    double* R0 = nullptr;     // [count][4096]
+    double* T = nullptr;      // [count][N][N]
+    double* D = nullptr;      // [count][N]
+    double* E = nullptr;      // [count][N]
+    double* U = nullptr;      // [64][N], one slot at a time
 };
 
 // Row chunks of a Q^T W product over nkb basis blocks (one workgroup per
@@ -687,6 +688,7 @@ int block_lanczos(const BlockKrylovArgs& a, Carve& bump)
     const int64_t n = op->A->csr.nrows;
     const int m = a.m;
     const int64_t N = (int64_t) m * kB;
+    const int64_t NN = N * N;
     const hipStream_t s = op->stream;
     const size_t blk = (size_t) n * kB;
     T* Q = bump.take<T>(blk * m);
@@ -699,25 +701,16 @@ int block_lanczos(const BlockKrylovArgs& a, Carve& bump)
     bb.R1 = bump.take<double>(kTile);
     bb.Rinv = bump.take<double>(kTile);
     bb.G = bump.take<double>(kTile);
-    BkParity par[2];
-    for (int k = 0; k < 2; ++k)
-    {
-        par[k].R0 = bump.take<double>(kTile);
-        par[k].flag = bump.take<int>(64);
-        par[k].T = bump.take<double>((size_t) N * N);
-        par[k].D = bump.take<double>((size_t) N);
-        par[k].E = bump.take<double>((size_t) N);
-        par[k].U = bump.take<double>((size_t) kB * N);
-    }
-    const BkParity& pp = par[a.par & 1];
-    bb.R0 = pp.R0;
-    bb.flag = pp.flag;
-    bb.T = pp.T;
-    bb.D = pp.D;
-    bb.E = pp.E;
-    bb.U = pp.U;
+    bb.flag = bump.take<int>(64);
+    BkSlots sl;
+    sl.count = block_krylov_slots(m);
+    sl.R0 = bump.take<double>((size_t) sl.count * kTile);
+    sl.T = bump.take<double>((size_t) sl.count * NN);
+    sl.D = bump.take<double>((size_t) sl.count * N);
+    sl.E = bump.take<double>((size_t) sl.count * N);
+    sl.U = bump.take<double>((size_t) kB * N);
     if (!Q || !W || !W2 || !bb.P || !bb.H || !bb.AB || !bb.R1 || !bb.Rinv || !bb.G ||
-        !par[1].R0 || !par[1].flag || !par[1].T || !par[1].D || !par[1].E || !par[1].U)
+        !bb.flag || !sl.R0 || !sl.T || !sl.D || !sl.E || !sl.U)
     {
         set_error("internal: block Krylov workspace too small");
         return SLQ_E_OOM;
@@ -725,11 +718,20 @@ int block_lanczos(const BlockKrylovArgs& a, Carve& bump)
     if (a.flags_out) *a.flags_out = bb.flag;
     void* spmm_scratch = bump.p;
     const size_t spmm_bytes = bump.left;
-    if (a.phase & 1)
+    if (a.phase == 0)
     {
-        // the rank-loss flag gathers every recurrence of a block
-        if (a.first_in_block)
-            hipLaunchKernelGGL(bk_zero_int_kernel, dim3(1), dim3(64), 0, s, bb.flag, 64);
+        hipLaunchKernelGGL(bk_zero_int_kernel, dim3(1), dim3(64), 0, s, bb.flag, 1);
+        SLQ_CHECK(hipGetLastError());
+        return SLQ_OK;
+    }
+    if (a.phase == 1)
+    {
+        if (a.slot < 0 || a.slot >= sl.count)
+        {
+            set_error("internal: block Krylov slot %d out of range", a.slot);
+            return SLQ_E_ARG;
+        }
+        bb.R0 = sl.R0 + (size_t) a.slot * kTile;
         // the B_{m-1} slot is never computed (nor read): keep it defined
         {
             const int64_t len = (int64_t) 2 * m * kTile;
@@ -756,19 +758,19 @@ int block_lanczos(const BlockKrylovArgs& a, Carve& bump)
                 SLQ_TRY(cholqr2<T>(op, W, W2, Q + (size_t) (j + 1) * blk, n, bb,
                                    bb.AB + (size_t) (2 * j + 1) * kTile));
         }
-        // block tridiagonal, eigensolve, quadrature
-        const int64_t NN = N * N;
+        // the block tridiagonal, into the slot
         hipLaunchKernelGGL(bk_assemble_kernel, dim3((unsigned) ((NN + 255) / 256)), dim3(256),
-                           0, s, (const double*) bb.AB, m, bb.T);
+                           0, s, (const double*) bb.AB, m, sl.T + (size_t) a.slot * NN);
         SLQ_CHECK(hipGetLastError());
-        // the eigensolve (phase 2, maybe on another stream) waits for this
-        if (a.qstream && a.qstream != s) SLQ_CHECK(hipEventRecord(a.rec_done, s));
+        return SLQ_OK;
     }
-    if (!(a.phase & 2)) return SLQ_OK;
-    // eigensolve and quadrature on the quadrature stream (the handle's
-    // stream when not pipelined), after the recurrence
-    const hipStream_t qs = a.qstream ? a.qstream : s;
-    if (qs != s) SLQ_CHECK(hipStreamWaitEvent(qs, a.rec_done, 0));
+    // phase 2: one batched eigensolve of the group's slots, then the
+    // quadrature of each
+    if (a.nslots < 1 || a.nslots > sl.count)
+    {
+        set_error("internal: block Krylov batch of %d slots (at most %d)", a.nslots, sl.count);
+        return SLQ_E_ARG;
+    }
     Solver& sv = solver();
     if (!sv.ok)
     {
@@ -787,50 +789,63 @@ int block_lanczos(const BlockKrylovArgs& a, Carve& bump)
         op->rocblas = h;
     }
     rocblas_handle h = (rocblas_handle) op->rocblas;
-    if (sv.set_stream(h, qs) != rocblas_status_success ||
-        sv.dsyevd(h, rocblas_evect_original, rocblas_fill_lower, (rocblas_int) N, bb.T,
-                  (rocblas_int) N, bb.D, bb.E, bb.flag + 1) != rocblas_status_success)
+    if (sv.set_stream(h, s) != rocblas_status_success ||
+        sv.dsyevd(h, rocblas_evect_original, rocblas_fill_lower, (rocblas_int) N, sl.T,
+                  (rocblas_int) N, (rocblas_stride) NN, sl.D, (rocblas_stride) N, sl.E,
+                  (rocblas_stride) N, bb.flag + 1, (rocblas_int) a.nslots) !=
+            rocblas_status_success)
     {
-        set_error("block Krylov: rocsolver_dsyevd failed");
+        set_error("block Krylov: rocsolver_dsyevd_strided_batched failed");
         return SLQ_E_HIP;
     }
-    hipLaunchKernelGGL(bk_weights_kernel, dim3((unsigned) ((N * kB + 255) / 256)), dim3(256),
-                       0, qs, (const double*) bb.R0, (const double*) bb.T, N, bb.U);
-    BkQuadArgs qa;
-    qa.N = N;
-    qa.theta = bb.D;
-    qa.U = bb.U;
-    qa.info = bb.flag + 1;
-    qa.func = a.func;
-    for (int k = 0; k < 4; ++k) qa.fp[k] = a.func_params ? a.func_params[k] : 0.0;
-    qa.exponent = a.exponent;
-    qa.params = a.params_dev;
-    qa.q = a.q;
-    qa.shift = a.shift;
-    qa.j_out = a.j_out;
-    qa.q_stride = a.q_stride;
-    qa.samples = a.samples_dev;
-    hipLaunchKernelGGL(bk_quadrature_kernel, dim3(kB), dim3(256), 0, qs, qa);
+    for (int k = 0; k < a.nslots; ++k)
+    {
+        hipLaunchKernelGGL(bk_weights_kernel, dim3((unsigned) ((N * kB + 255) / 256)), dim3(256),
+                           0, s, (const double*) (sl.R0 + (size_t) k * kTile),
+                           (const double*) (sl.T + (size_t) k * NN), N, sl.U);
+        BkQuadArgs qa;
+        qa.N = N;
+        qa.theta = sl.D + (size_t) k * N;
+        qa.U = sl.U;
+        qa.info = bb.flag + 1 + k;
+        qa.func = a.func;
+        for (int e = 0; e < 4; ++e) qa.fp[e] = a.func_params ? a.func_params[e] : 0.0;
+        qa.exponent = a.exponent;
+        qa.params = a.params_dev;
+        qa.q = a.q;
+        qa.shift = a.shift;
+        qa.j_out = a.j_out[k];
+        qa.q_stride = a.q_stride;
+        qa.samples = a.samples_dev[k];
+        hipLaunchKernelGGL(bk_quadrature_kernel, dim3(kB), dim3(256), 0, s, qa);
+    }
     SLQ_CHECK(hipGetLastError());
-    // the caller reads the flags (*flags_out) where it waits for the block
     return SLQ_OK;
 }
 
 }  // namespace
+
+int block_krylov_slots(int m)
+{
+    const size_t tbytes = sizeof(double) * (size_t) m * kB * (size_t) m * kB;
+    size_t k = ((size_t) 1 << 30) / (tbytes ? tbytes : 1);
+    if (k > (size_t) kBkMaxSlots) k = kBkMaxSlots;
+    return k < 1 ? 1 : (int) k;
+}
 
 size_t block_krylov_scratch_bytes(const slq_op* op, int m, int value_bytes)
 {
     auto r = [](size_t b) { return (b + 255) & ~(size_t) 255; };
     const int64_t n = op->A->csr.nrows;
     const int64_t N = (int64_t) m * kB;
+    const size_t S = (size_t) block_krylov_slots(m);
     const size_t blk = (size_t) n * kB * value_bytes;
     size_t b = r(blk * m) + 2 * r(blk);
     b += r(sizeof(double) * (size_t) max_partial_blocks(n, m) * kTile);
     b += r(sizeof(double) * (size_t) m * kTile) + r(sizeof(double) * (size_t) 2 * m * kTile);
-    b += 3 * r(sizeof(double) * kTile);
-    b += 2 * (r(sizeof(double) * kTile) + r(sizeof(int) * 64) +
-              r(sizeof(double) * (size_t) N * N) + 2 * r(sizeof(double) * (size_t) N) +
-              r(sizeof(double) * (size_t) kB * N));
+    b += 3 * r(sizeof(double) * kTile) + r(sizeof(int) * 64);
+    b += r(sizeof(double) * S * kTile) + r(sizeof(double) * S * (size_t) N * N) +
+         2 * r(sizeof(double) * S * (size_t) N) + r(sizeof(double) * (size_t) kB * N);
     b += spmm_block_scratch_bytes(op);
     return b;
 }

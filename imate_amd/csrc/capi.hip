@@ -983,6 +983,9 @@ struct Plan {
     // directions for Golub-Kahn), lane c at off_lanes + c * lane_stride
     int lanes = 0;
     size_t off_lanes = 0, lane_stride = 0;
+    // block Krylov form: slots of a group and the group's samples [slots][64][q]
+    int bk_slots = 0;
+    size_t off_bk_samples = 0;
     size_t lane_alpha = 0, lane_beta = 0, lane_sizes = 0, lane_vals = 0, lane_vals_t = 0,
            lane_scratch = 0;
 };
@@ -2392,6 +2395,111 @@ int lanes_block(slq_op* op, const Plan& p, const EstimateArgs& a, int cnt, doubl
     return r;
 }
 
+// The block Krylov form's loop (slq_estimate_block_krylov): blocks of 64
+// probes in groups, each block but the last of a group ending before
+// min_num_samples, so the convergence test cannot stop the loop inside it.
+// Every recurrence of a group (one per block, or per unconverged inquiry of
+// A + tB with a general B) keeps its block tridiagonal in a slot of its own;
+// the slots' eigensolves then run as one batched call (a group with more
+// recurrences than slots is solved a slotful at a time). The samples of the
+// group come back in one copy and are tested in order by `process`.
+template <typename Process>
+int krylov_groups(slq_op* op, const Plan& p, const EstimateArgs& a, const uint64_t* base,
+                  int vdt, const double* params_dev, const int32_t* converged,
+                  const int& all_conv, Process&& process)
+{
+    const int q = a.q;
+    const bool relation = op->kind == OP_AFFINE_I;
+    const int required = relation ? 1 : q;
+    const int S = p.bk_slots;
+    double* gsamples = at<double>(p, p.off_bk_samples);   // [S][64][q]
+    void* scratch = at<void>(p, p.off_scratch);
+    int* flags_dev = nullptr;
+    BlockKrylovArgs c;   // fields every phase reads
+    c.op = op;
+    c.m = a.m;
+    c.value_dtype = vdt;
+    c.flags_out = &flags_dev;
+    int64_t s0 = 0;
+    while (!all_conv && s0 < a.max_s)
+    {
+        std::vector<int> todo;
+        for (int j = 0; j < required; ++j)
+            if (relation || !converged[j]) todo.push_back(j);
+        const int per = (int) todo.size();
+        int g = 1;
+        for (;;)
+        {
+            const int64_t end = std::min<int64_t>(s0 + (int64_t) g * kBlock, a.max_s);
+            if (end >= a.min_s || end >= a.max_s || g >= S || (g + 1) * per > S) break;
+            ++g;
+        }
+        BlockKrylovArgs z = c;
+        z.phase = 0;
+        SLQ_TRY(run_block_krylov(z, scratch, p.scratch_bytes));
+        BlockKrylovArgs solve = c;
+        solve.phase = 2;
+        solve.func = a.func;
+        solve.func_params = a.func_params;
+        solve.exponent = a.exponent;
+        solve.params_dev = relation ? params_dev : nullptr;
+        solve.q = q;
+        solve.shift = relation ? 1 : 0;
+        solve.q_stride = q;
+        int slot = 0;
+        for (int k = 0; k < g; ++k)
+        {
+            // every probe of the block (also past max_num_samples): the
+            // block's basis needs all 64
+            SLQ_TRY(make_probes(op, p, base, s0 + (int64_t) k * kBlock, kBlock, vdt));
+            for (int j : todo)
+            {
+                BlockKrylovArgs r = c;
+                r.phase = 1;
+                r.slot = slot;
+                r.t = op->kind != OP_MATRIX ? a.params[j] : 0.0;
+                r.X0 = at<void>(p, p.off_X0);
+                SLQ_TRY(run_block_krylov(r, scratch, p.scratch_bytes));
+                solve.j_out[slot] = j;
+                solve.samples_dev[slot] = gsamples + (size_t) k * kBlock * q;
+                if (++slot == S)
+                {
+                    solve.nslots = slot;
+                    SLQ_TRY(run_block_krylov(solve, scratch, p.scratch_bytes));
+                    slot = 0;
+                }
+            }
+        }
+        if (slot > 0)
+        {
+            solve.nslots = slot;
+            SLQ_TRY(run_block_krylov(solve, scratch, p.scratch_bytes));
+        }
+        const int64_t gend = std::min<int64_t>(s0 + (int64_t) g * kBlock, a.max_s);
+        int rank_lost = 0;
+        SLQ_CHECK(hipMemcpyAsync(a.samples + s0 * q, gsamples, sizeof(double) * (gend - s0) * q,
+                                 hipMemcpyDeviceToHost, op->stream));
+        SLQ_CHECK(hipMemcpyAsync(&rank_lost, flags_dev, sizeof(int), hipMemcpyDeviceToHost,
+                                 op->stream));
+        SLQ_CHECK(hipStreamSynchronize(op->stream));
+        if (rank_lost)
+        {
+            // a basis that lost rank makes the block's samples meaningless
+            set_error("block Krylov: the block basis lost rank (lanczos_degree x 64 = %d too "
+                      "large for this %lld-row matrix)", a.m * kBlock,
+                      (long long) op->A->csr.nrows);
+            return SLQ_E_ARG;
+        }
+        for (int k = 0; k < g; ++k)
+        {
+            const int64_t b0 = s0 + (int64_t) k * kBlock;
+            process(b0, (int) std::min<int64_t>(kBlock, a.max_s - b0), 0);
+        }
+        s0 = gend;
+    }
+    return SLQ_OK;
+}
+
 // The sampling loop of one estimate on the workspace p.base (sized by p),
 // following cu_trace_estimator.cu:197-310 with one device: probes in blocks
 // of 64 in sample order, one host synchronisation per block (the block's
@@ -2429,7 +2537,7 @@ int estimate_loop(slq_op* op, const Plan& p, EstimateArgs a)
     // than min_num_samples processed by its end) the next block's probes
     // and recurrence are queued before the host waits for it. The kernels
     // and their inputs are those of the plain order, so results are equal.
-    const bool pipelined = p.pipeline && p.lanes == 0 && required == 1;
+    const bool pipelined = p.pipeline && p.lanes == 0 && required == 1 && !a.krylov;
     if (pipelined && !op->aux_stream)
         SLQ_CHECK(hipStreamCreateWithFlags(&op->aux_stream, hipStreamNonBlocking));
     hipStream_t qstream = pipelined ? op->aux_stream : op->stream;
@@ -2443,11 +2551,6 @@ int estimate_loop(slq_op* op, const Plan& p, EstimateArgs a)
         ran[k].assign(count_steps ? required : 0, 0);
     }
     hipEvent_t rec_done[2] = {nullptr, nullptr};
-    // block Krylov form: device flags (rank loss, eigensolver info) of each
-    // parity's block and their host copies, read where the host waits
-    int* bk_flags[2] = {nullptr, nullptr};
-    int bk_flags_host[2] = {0, 0};
-    BlockKrylovArgs bk_args[2];
     struct Events {
         hipEvent_t* e;
         ~Events()
@@ -2465,44 +2568,6 @@ int estimate_loop(slq_op* op, const Plan& p, EstimateArgs a)
     auto enqueue = [&](int64_t s0, int par) -> int {
         const int cnt = (int) std::min<int64_t>(bs, a.max_s - s0);
         std::fill(ran[par].begin(), ran[par].end(), 0);
-        if (a.krylov)
-        {
-            // every probe of the block (also past max_num_samples): the
-            // block's basis needs all 64
-            SLQ_TRY(make_probes(op, p, base, s0, kBlock, vdt));
-            bool first = true;
-            for (int j = 0; j < required; ++j)
-            {
-                if (!relation && converged[j]) continue;
-                BlockKrylovArgs ba;
-                ba.par = par;
-                ba.qstream = pipelined ? qstream : nullptr;
-                ba.rec_done = pipelined ? rec_done[par] : nullptr;
-                ba.first_in_block = first;
-                ba.flags_out = &bk_flags[par];
-                // pipelined: the eigensolve is issued by the loop below,
-                // after the next block's recurrence is queued
-                ba.phase = pipelined ? 1 : 3;
-                first = false;
-                ba.op = op;
-                ba.t = op->kind != OP_MATRIX ? a.params[j] : 0.0;
-                ba.m = a.m;
-                ba.value_dtype = vdt;
-                ba.X0 = at<void>(p, p.off_X0);
-                ba.func = a.func;
-                ba.func_params = a.func_params;
-                ba.exponent = a.exponent;
-                ba.params_dev = relation ? params_dev : nullptr;
-                ba.q = q;
-                ba.shift = relation ? 1 : 0;
-                ba.j_out = j;
-                ba.q_stride = q;
-                ba.samples_dev = at<double>(p, p.samples_of(par));
-                SLQ_TRY(run_block_krylov(ba, at<void>(p, p.off_scratch), p.scratch_bytes));
-                bk_args[par] = ba;
-            }
-            return SLQ_OK;
-        }
         SLQ_TRY(make_probes(op, p, base, s0, cnt, vdt));
         if (p.lanes > 0)
             return lanes_block(op, p, a, cnt, nq, count_steps, block_sizes[par], ran[par]);
@@ -2535,51 +2600,9 @@ int estimate_loop(slq_op* op, const Plan& p, EstimateArgs a)
 
     int nproc = 0;
     int all_conv = 0;
-    SLQ_CHECK(hipStreamSynchronize(op->stream));
-    const auto t0 = std::chrono::steady_clock::now();
-    int64_t s0 = 0;
-    int par = 0;
-    SLQ_TRY(enqueue(0, 0));
-    for (;;)
-    {
-        const int cnt = (int) std::min<int64_t>(bs, a.max_s - s0);
-        const int64_t s1 = s0 + bs;
-        const int nxt = pipelined ? par ^ 1 : 0;
-        bool ahead = false;
-        if (pipelined && s1 < a.max_s && nproc + cnt < a.min_s)
-        {
-            SLQ_TRY(enqueue(s1, nxt));
-            ahead = true;
-        }
-        if (a.krylov && pipelined)
-        {
-            // this block's eigensolve and quadrature (rocSOLVER synchronises
-            // with its stream on the host; the next block is already queued)
-            BlockKrylovArgs f = bk_args[par];
-            f.phase = 2;
-            SLQ_TRY(run_block_krylov(f, at<void>(p, p.off_scratch), p.scratch_bytes));
-        }
-        // copies to host memory wait for the stream, so they are issued only
-        // here, when the host needs the block (a pipelined block's sizes too)
-        if (count_steps && pipelined)
-            SLQ_CHECK(hipMemcpyAsync(block_sizes[par].data(), at<int32_t>(p, p.sizes_of(par)),
-                                     sizeof(int32_t) * kBlock, hipMemcpyDeviceToHost,
-                                     qstream));
-        SLQ_CHECK(hipMemcpyAsync(samples + s0 * q, at<double>(p, p.samples_of(par)),
-                                 sizeof(double) * cnt * q, hipMemcpyDeviceToHost, qstream));
-        if (a.krylov && bk_flags[par])
-            SLQ_CHECK(hipMemcpyAsync(bk_flags_host, bk_flags[par], sizeof(int) * 2,
-                                     hipMemcpyDeviceToHost, qstream));
-        SLQ_CHECK(hipStreamSynchronize(qstream));
-        if (a.krylov && bk_flags_host[0])
-        {
-            // a basis that lost rank makes the block's samples meaningless
-            SLQ_CHECK(hipStreamSynchronize(op->stream));
-            set_error("block Krylov: the block basis lost rank (lanczos_degree x 64 = %d too "
-                      "large for this %lld-row matrix)", a.m * kBlock,
-                      (long long) op->A->csr.nrows);
-            return SLQ_E_ARG;
-        }
+    // the convergence test over the samples of the block at s0 (cnt of them,
+    // in host memory), in order; samples past the stopping one become NaN
+    auto process = [&](int64_t s0, int cnt, int par) {
         for (int pi = 0; pi < cnt; ++pi)
         {
             const int64_t i = s0 + pi;
@@ -2604,10 +2627,46 @@ int estimate_loop(slq_op* op, const Plan& p, EstimateArgs a)
             all_conv = check_convergence(samples, q, a.min_s, a.processed, nproc, a.confidence,
                                          a.atol, a.rtol, a.error, a.num_used, converged);
         }
-        if (all_conv || s1 >= a.max_s) break;
-        if (!ahead) SLQ_TRY(enqueue(s1, nxt));
-        s0 = s1;
-        par = nxt;
+    };
+    SLQ_CHECK(hipStreamSynchronize(op->stream));
+    const auto t0 = std::chrono::steady_clock::now();
+    if (a.krylov)
+    {
+        SLQ_TRY(krylov_groups(op, p, a, base, vdt, params_dev, converged, all_conv, process));
+    }
+    else
+    {
+        int64_t s0 = 0;
+        int par = 0;
+        SLQ_TRY(enqueue(0, 0));
+        for (;;)
+        {
+            const int cnt = (int) std::min<int64_t>(bs, a.max_s - s0);
+            const int64_t s1 = s0 + bs;
+            const int nxt = pipelined ? par ^ 1 : 0;
+            bool ahead = false;
+            if (pipelined && s1 < a.max_s && nproc + cnt < a.min_s)
+            {
+                SLQ_TRY(enqueue(s1, nxt));
+                ahead = true;
+            }
+            // copies to host memory wait for the stream, so they are issued
+            // only here, when the host needs the block (a pipelined block's
+            // sizes too)
+            if (count_steps && pipelined)
+                SLQ_CHECK(hipMemcpyAsync(block_sizes[par].data(),
+                                         at<int32_t>(p, p.sizes_of(par)),
+                                         sizeof(int32_t) * kBlock, hipMemcpyDeviceToHost,
+                                         qstream));
+            SLQ_CHECK(hipMemcpyAsync(samples + s0 * q, at<double>(p, p.samples_of(par)),
+                                     sizeof(double) * cnt * q, hipMemcpyDeviceToHost, qstream));
+            SLQ_CHECK(hipStreamSynchronize(qstream));
+            process(s0, cnt, par);
+            if (all_conv || s1 >= a.max_s) break;
+            if (!ahead) SLQ_TRY(enqueue(s1, nxt));
+            s0 = s1;
+            par = nxt;
+        }
     }
     SLQ_CHECK(hipStreamSynchronize(op->stream));
     const auto t1 = std::chrono::steady_clock::now();
@@ -2800,7 +2859,13 @@ int slq_estimate_block_krylov(slq_op* op, const double* params, int32_t q, int32
     const int vdt = value_dtype(op, precision);
     SLQ_TRY(prepare(op, 0, precision));
     Plan p = make_plan(op, 0, m, 0, q, vdt == SLQ_F64 ? 8 : 4);
-    p.total -= p.scratch_bytes;
+    // the group's samples, then the block form's own scratch, in place of
+    // the recurrence scratch
+    p.total = p.off_scratch;
+    p.bk_slots = block_krylov_slots(m);
+    p.off_bk_samples = p.total;
+    p.total += align256(sizeof(double) * (size_t) p.bk_slots * kBlock * q);
+    p.off_scratch = p.total;
     p.scratch_bytes = block_krylov_scratch_bytes(op, m, vdt == SLQ_F64 ? 8 : 4);
     p.total += p.scratch_bytes;
     SLQ_TRY(op->ws.reserve(p.total));

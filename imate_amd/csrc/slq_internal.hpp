@@ -264,49 +264,51 @@ size_t spmm_block_scratch_bytes(const slq_op* op);
 int spmm_block(slq_op* op, double t, int value_dtype, const void* X, void* Y, void* scratch,
                size_t scratch_bytes);
 
-// Block Lanczos over one 64-probe block with full re-orthogonalisation of
-// the shared basis (block_krylov.hip): m block steps from the probe block
-// X0 ([len][64], value type of the run), eigensolve of the 64 m x 64 m block
-// tridiagonal and block Gauss quadrature; per-probe samples into
-// samples_dev[p * q_stride + col] like run_quadrature (Ritz shift by
-// params_dev[j] - params_dev[0] when shift).
+// Block Lanczos with full re-orthogonalisation of the shared basis of a
+// 64-probe block (block_krylov.hip), run in groups of blocks:
+//   phase 0: clears the group's rank-loss flag;
+//   phase 1: m block steps from the probe block X0 ([len][64], value type of
+//     the run) for A + tB, the block tridiagonal (64 m x 64 m) kept in slot
+//     `slot`;
+//   phase 2: the eigensolves of slots 0 .. nslots - 1 in one batched call,
+//     then block Gauss quadrature; slot s writes its per-probe samples to
+//     samples_dev[s][p * q_stride + j_out[s]] like run_quadrature (every t
+//     of the Ritz shift, by params_dev[j] - params_dev[0], when shift).
+// All on the handle's stream. *flags_out receives the device address of the
+// flags: [0] rank loss of a recurrence since phase 0, [1 + s] eigensolver
+// info of slot s.
+constexpr int kBkMaxSlots = 8;
 struct BlockKrylovArgs {
     slq_op* op = nullptr;
-    double t = 0.0;
+    int phase = 0;
     int m = 0;
     int value_dtype = SLQ_F64;
+    int** flags_out = nullptr;
+    // phase 1
+    int slot = 0;
+    double t = 0.0;
     const void* X0 = nullptr;
+    // phase 2
+    int nslots = 0;
+    int j_out[kBkMaxSlots] = {};
+    double* samples_dev[kBkMaxSlots] = {};
     int func = 0;
     const double* func_params = nullptr;
     double exponent = 1.0;
     const double* params_dev = nullptr;
     int q = 1;
     int shift = 0;
-    int j_out = 0;
     int q_stride = 1;
-    double* samples_dev = nullptr;
-    // buffers of parity par (two blocks in flight); the eigensolve and
-    // quadrature run on qstream after rec_done (recorded on the handle's
-    // stream) when qstream is set; first_in_block clears the block's flags;
-    // *flags_out receives the device address of its two flags (rank loss,
-    // eigensolver info), to be read once the block is done
-    int par = 0;
-    hipStream_t qstream = nullptr;
-    hipEvent_t rec_done = nullptr;
-    bool first_in_block = true;
-    int** flags_out = nullptr;
-    // 1: recurrence and block tridiagonal (handle's stream); 2: eigensolve
-    // and quadrature; 3: both. Phase 2 of parity par may be issued after
-    // phase 1 of the next block, so that rocSOLVER's host synchronisations
-    // wait while that block's recurrence is already queued.
-    int phase = 3;
 };
+// Slots (block tridiagonals held for one batched eigensolve) at this
+// lanczos_degree: at most kBkMaxSlots and 1 GiB of them, at least one.
+int block_krylov_slots(int m);
 size_t block_krylov_scratch_bytes(const slq_op* op, int m, int value_bytes);
 int run_block_krylov(const BlockKrylovArgs& a, void* scratch, size_t scratch_bytes);
 // Frees the handle's rocBLAS handle (if the block form created one).
 void release_solver(slq_op* op);
 
-// Plain block SpMM (tests): Y = (A + tB) X or its transpose.// Plain block SpMM (tests): Y = (A + tB) X or its transpose.
+// Plain block SpMM (tests): Y = (A + tB) X or its transpose.
 int run_spmm(slq_op* op, double t, int transpose, const void* X, void* Y,
              int value_dtype);
 

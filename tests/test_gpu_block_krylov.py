@@ -125,7 +125,7 @@ def test_gemm_profile_counts_the_block_products(c1):
 
 def test_affine_general_b_each_t(c1):
     """A + tB with a general B: one block recurrence per t (no Ritz shift),
-    run one after the other on the handle's stream."""
+    each in a slot of its own, the two eigensolved in one batch."""
     n = c1.shape[0]
     B = scipy.sparse.diags(numpy.linspace(0.5, 2.0, n)).tocsr()
     ts = [0.1, 1.5]
@@ -134,6 +134,26 @@ def test_affine_general_b_each_t(c1):
     Z = oracle.rademacher(12, n, 64).T
     for j, t in enumerate(ts):
         want = ref_samples((c1 + t * B).tocsr(), Z, 6, numpy.log)[:, 0]
+        assert _rel(got[:, j], want) < 1e-9
+
+
+@pytest.mark.parametrize('nt,count', [(9, 128), (3, 192)])
+def test_slot_groups(c1, nt, count):
+    """More recurrences per block than slots (9 t: eight eigensolved in one
+    batch, then one), and groups of two blocks with three t each (six of
+    the eight slots) followed by a group of one: every sample equals the
+    NumPy statement of its block and t."""
+    n = c1.shape[0]
+    D = scipy.sparse.diags(numpy.linspace(0.5, 2.0, n)).tocsr()
+    ts = list(numpy.linspace(0.05, 2.0, nt))
+    op = imate_amd.AffineMatrixFunction(c1, D)
+    val, got = _run(imate_amd.traceinv, op, 5, count, 21, parameters=ts)
+    Z = oracle.rademacher(21, n, count).T
+    for j, t in enumerate(ts):
+        At = (c1 + t * D).tocsr()
+        want = numpy.concatenate(
+            [ref_samples(At, Z[:, k:k + B], 5, lambda x: 1.0 / x)[:, 0]
+             for k in range(0, count, B)])
         assert _rel(got[:, j], want) < 1e-9
 
 
