@@ -107,7 +107,10 @@ struct Mma<float> {
 // of the same chunk, so the chunk's rows of W are staged once through LDS
 // (16 rows at a time, double buffered, the next rows loaded during this
 // step's MFMAs) and serve all four; each wave computes the whole 64 x 64
-// block (16 accumulator tiles) and reads its rows of Q_b once. MFMA
+// block (16 accumulator tiles) and reads its rows of Q_b once (FP32: the
+// next 16 rows load during this step's MFMAs, 0.46 -> 0.49 of the matrix
+// peak; the FP64 prefetch registers would halve the waves per SIMD, which
+// measured slower). MFMA
 // operands per four rows: A[r][k] = Q_b[i0 + k][16 pb + r],
 // B[k][c] = W[i0 + k][16 qb + c], with r = c = lane % 16 and k = lane / 16.
 // Chunks are a multiple of 16 rows.
@@ -145,6 +148,19 @@ __global__ __launch_bounds__(kBkThreads) void bk_tn_kernel(const T* __restrict__
             v[k] = i < r1 ? W[i * kB + e % kB] : T(0);
         }
     };
+    constexpr bool kPrefQ = sizeof(T) == 4;
+    auto load_q = [&](int64_t i0, T (&v)[4][4]) {
+#pragma unroll
+        for (int g = 0; g < 4; ++g)
+        {
+            const int64_t i = i0 + 4 * g + kq;
+            const bool ok = active && i < r1;
+#pragma unroll
+            for (int x = 0; x < 4; ++x) v[g][x] = ok ? Qb[i * kB + x * 16 + r] : T(0);
+        }
+    };
+    T a[4][4];
+    if (kPrefQ) load_q(r0, a);
     T wv[kPer];
     load_w(r0, wv);
 #pragma unroll
@@ -155,17 +171,11 @@ __global__ __launch_bounds__(kBkThreads) void bk_tn_kernel(const T* __restrict__
     {
         const bool more = i0 + kRows < r1;
         if (more) load_w(i0 + kRows, wv);
+        T an[4][4];
+        if (kPrefQ && more) load_q(i0 + kRows, an);
+        if (!kPrefQ) load_q(i0, a);
         if (active)
         {
-            T a[4][4];
-#pragma unroll
-            for (int g = 0; g < 4; ++g)
-            {
-                const int64_t i = i0 + 4 * g + kq;
-                const bool ok = i < r1;
-#pragma unroll
-                for (int x = 0; x < 4; ++x) a[g][x] = ok ? Qb[i * kB + x * 16 + r] : T(0);
-            }
             const T* __restrict__ ws = Ws[buf];
 #pragma unroll
             for (int g = 0; g < 4; ++g)
@@ -184,6 +194,11 @@ __global__ __launch_bounds__(kBkThreads) void bk_tn_kernel(const T* __restrict__
         {
 #pragma unroll
             for (int k = 0; k < kPer; ++k) Ws[buf ^ 1][k * kBkThreads + threadIdx.x] = wv[k];
+            if (kPrefQ)
+#pragma unroll
+                for (int g = 0; g < 4; ++g)
+#pragma unroll
+                    for (int x = 0; x < 4; ++x) a[g][x] = an[g][x];
         }
         buf ^= 1;
         __syncthreads();
@@ -235,13 +250,12 @@ __global__ __launch_bounds__(256) void bk_reduce_kernel(const double* __restrict
 // k-step s of block b the MFMA contracts over p = 16 k + s (k = lane / 16),
 // so lane (r, k) reads X_b[row r][16 k .. 16 k + 15]: 128 B (FP64)
 // contiguous per lane. X and Y must not overlap.
-// 16-row tiles of Y per wave in bk_nn_kernel: two for FP32 (four measured
-// slower), one for FP64
-// (whose operand registers for the prefetch would otherwise halve the waves
-// per SIMD).
+// 16-row tiles of Y per wave in bk_nn_kernel: two (each value of H read
+// from LDS and each staging of H_b serve twice the rows; FP64 0.46 -> 0.51
+// of the matrix peak against one tile, four measured slower for FP32).
 template <typename T>
 struct NnTiles {
-    static constexpr int value = sizeof(T) == 8 ? 1 : 2;
+    static constexpr int value = 2;
 };
 
 template <typename T, bool SUB>
