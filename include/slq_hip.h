@@ -297,9 +297,11 @@ int slq_mlp_predict(const float* params, const float* x, int32_t Q,
  * probes of a block (consecutive samples of the thread-0 stream) share one
  * Krylov basis, fully re-orthogonalised by matrix-core GEMMs against all
  * stored basis blocks; lanczos_degree = m block steps give a 64 m x 64 m
- * block tridiagonal (eigensolve by rocSOLVER dsyevd, loaded on first use) and
- * per-probe block Gauss quadrature samples, fed to the reference's
- * convergence test like slq_estimate's. Arguments and outputs as
+ * block tridiagonal and per-probe block Gauss quadrature samples, fed to
+ * the reference's convergence test like slq_estimate's. The blocks that
+ * cannot reach min_num_samples run as a group whose eigensolves are one
+ * batched rocSOLVER call (loaded on first use; the block tridiagonals take
+ * up to 1 GiB of workspace). Arguments and outputs as
  * slq_estimate, without gram (Lanczos only), orthogonalize (always full),
  * lanczos_tol and block_size (64). Requires 64 m <= the matrix size. */
 int slq_estimate_block_krylov(slq_op* op, const double* params, int32_t num_inquiries,
