@@ -700,8 +700,69 @@ __device__ __forceinline__ void store_row4(float* __restrict__ p, const float (&
     else *reinterpret_cast<F4*>(p) = w;
 }
 
-// spmm_epilogue for the four consecutive probes 4 i0 .. 4 i0 + 3 of one
-// row, with 16 B loads and stores (the same arithmetic per probe).
+// The four probes of lane column i0 (0..15) of the MFMA kernels, one per
+// 16-probe block q, and their accesses in a 64-probe row. FP64: probes
+// 2 i0 + q for q < 2 and 30 + 2 i0 + q for q >= 2, so each 16 B access of
+// lanes 0..15 covers 256 contiguous bytes of the 512 B row (two whole
+// 128 B lines; the earlier probes 4 i0 + q took half of each of four lines
+// per access). FP32: probes 4 i0 + q, one 16 B access covering the row.
+template <typename T>
+struct Lanes;
+template <>
+struct Lanes<double> {
+    static __device__ __forceinline__ int probe(int i0, int q)
+    {
+        return q < 2 ? 2 * i0 + q : 30 + 2 * i0 + q;
+    }
+    static __device__ __forceinline__ void load(const double* __restrict__ row, int i0,
+                                                double (&v)[4])
+    {
+        typedef double D2 __attribute__((ext_vector_type(2)));
+        const D2 lo = *reinterpret_cast<const D2*>(row + 2 * i0);
+        const D2 hi = *reinterpret_cast<const D2*>(row + 32 + 2 * i0);
+        v[0] = lo.x;
+        v[1] = lo.y;
+        v[2] = hi.x;
+        v[3] = hi.y;
+    }
+    template <bool NT>
+    static __device__ __forceinline__ void store(double* __restrict__ row, int i0,
+                                                 const double (&v)[4])
+    {
+        typedef double D2 __attribute__((ext_vector_type(2)));
+        const D2 lo = {v[0], v[1]}, hi = {v[2], v[3]};
+        D2* plo = reinterpret_cast<D2*>(row + 2 * i0);
+        D2* phi = reinterpret_cast<D2*>(row + 32 + 2 * i0);
+        if constexpr (NT)
+        {
+            __builtin_nontemporal_store(lo, plo);
+            __builtin_nontemporal_store(hi, phi);
+        }
+        else
+        {
+            *plo = lo;
+            *phi = hi;
+        }
+    }
+};
+template <>
+struct Lanes<float> {
+    static __device__ __forceinline__ int probe(int i0, int q) { return 4 * i0 + q; }
+    static __device__ __forceinline__ void load(const float* __restrict__ row, int i0,
+                                                float (&v)[4])
+    {
+        load_row4(row + 4 * i0, v);
+    }
+    template <bool NT>
+    static __device__ __forceinline__ void store(float* __restrict__ row, int i0,
+                                                 const float (&v)[4])
+    {
+        store_row4<NT>(row + 4 * i0, v);
+    }
+};
+
+// spmm_epilogue for the four probes Lanes<T>::probe(i0, q) of one row, with
+// 16 B loads and stores (the same arithmetic per probe).
 template <typename T, bool NT>
 __device__ __forceinline__ void spmm_epilogue4(const SpmmArgs<T>& a, int64_t row, int i0,
                                                const T (&yin)[4], const T (&s)[4], T tt,
@@ -710,17 +771,17 @@ __device__ __forceinline__ void spmm_epilogue4(const SpmmArgs<T>& a, int64_t row
                                                T* __restrict__ Vout, const T* __restrict__ Sub,
                                                double (&red)[4])
 {
-    const int64_t o = row * kBlock + 4 * i0;
+    const int64_t o = row * kBlock;
     const bool shift_row = a.shift_mode == 1 && row < a.shift_len;
     T v[4] = {T(0), T(0), T(0), T(0)}, y[4], sb[4];
     if (Vout || a.red_mode == 1 || shift_row)
     {
         T xv[4];
-        load_row4(X + o, xv);
+        Lanes<T>::load(X + o, i0, xv);
 #pragma unroll
         for (int q = 0; q < 4; ++q) v[q] = s[q] * xv[q];
     }
-    if (Sub) load_row4(Sub + o, sb);
+    if (Sub) Lanes<T>::load(Sub + o, i0, sb);
 #pragma unroll
     for (int q = 0; q < 4; ++q)
     {
@@ -731,8 +792,8 @@ __device__ __forceinline__ void spmm_epilogue4(const SpmmArgs<T>& a, int64_t row
         if (a.red_mode == 1) red[q] += (double) v[q] * (double) y[q];
         if (Sub) y[q] = y[q] - sub_c[q] * (sub_s[q] * sb[q]);
     }
-    if (Vout) store_row4<NT>(Vout + o, v);
-    store_row4<NT>(Y + o, y);
+    if (Vout) Lanes<T>::template store<NT>(Vout + o, i0, v);
+    Lanes<T>::template store<NT>(Y + o, i0, y);
 #pragma unroll
     for (int q = 0; q < 4; ++q)
         if (a.red_mode == 2) red[q] += (double) y[q] * (double) y[q];
@@ -770,11 +831,12 @@ __global__ __launch_bounds__(kThreads) void mfma_spmm_kernel(
     const unsigned int waves = (unsigned int) ((gridDim.x - x + V - 1) / V) * kWavesPerWG;
     unsigned int* ticket = a.rb.tickets + 64 * x;
     unsigned int* done = ticket + 32;
-    // block q of the four 16-probe MFMA blocks holds probes 4 i0 + q, so a
-    // lane's four B values of one gathered row are contiguous
+    // block q of the four 16-probe MFMA blocks holds probe
+    // Lanes<T>::probe(i0, q), so a lane's four B values of one gathered row
+    // take two 16 B loads (FP64) that with lanes 0..15 read whole lines
     T sq[4];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) sq[q] = a.xscale ? (T) a.xscale[4 * i0 + q] : T(1);
+    for (int q = 0; q < 4; ++q) sq[q] = a.xscale ? (T) a.xscale[Lanes<T>::probe(i0, q)] : T(1);
     const T tt = (T) a.t;
     for (;;)
     {
@@ -806,16 +868,16 @@ __global__ __launch_bounds__(kThreads) void mfma_spmm_kernel(
                 reinterpret_cast<const Pair2<T>*>(tvals + base * 16) + lane;
             Pair2<T> ap = ap2[0], an;
             T b0[4], b1[4];
-            load_row4(X + col_of(0) * kBlock + 4 * i0, b0);
+            Lanes<T>::load(X + col_of(0) * kBlock, i0, b0);
             for (int P = 0; P < np; ++P)
             {
-                load_row4(X + col_of(2 * P + 1) * kBlock + 4 * i0, b1);
+                Lanes<T>::load(X + col_of(2 * P + 1) * kBlock, i0, b1);
 #pragma unroll
                 for (int q = 0; q < 4; ++q) acc[q] = Mfma16<T>::op(ap.x, sq[q] * b0[q], acc[q]);
                 if (P + 1 < np)
                 {
                     an = ap2[(P + 1) * kWave];
-                    load_row4(X + col_of(2 * P + 2) * kBlock + 4 * i0, b0);
+                    Lanes<T>::load(X + col_of(2 * P + 2) * kBlock, i0, b0);
                 }
 #pragma unroll
                 for (int q = 0; q < 4; ++q) acc[q] = Mfma16<T>::op(ap.y, sq[q] * b1[q], acc[q]);
@@ -827,8 +889,8 @@ __global__ __launch_bounds__(kThreads) void mfma_spmm_kernel(
 #pragma unroll
         for (int q = 0; q < 4; ++q)
         {
-            sub_c[q] = Sub ? (T) a.sub_coef[4 * i0 + q] : T(0);
-            sub_s[q] = Sub ? (T) a.sub_scale[4 * i0 + q] : T(0);
+            sub_c[q] = Sub ? (T) a.sub_coef[Lanes<T>::probe(i0, q)] : T(0);
+            sub_s[q] = Sub ? (T) a.sub_scale[Lanes<T>::probe(i0, q)] : T(0);
         }
 #pragma unroll
         for (int m = 0; m < 4; ++m)
@@ -850,7 +912,7 @@ __global__ __launch_bounds__(kThreads) void mfma_spmm_kernel(
                 const double r2 = r1 + __shfl_xor(r1, 32);
                 if (q == kq) mine = r2;
             }
-            a.rb.tile_partial[t * kWave + 4 * i0 + kq] = mine;
+            a.rb.tile_partial[t * kWave + Lanes<T>::probe(i0, kq)] = mine;
         }
     }
     unsigned int d = 0;
@@ -913,7 +975,7 @@ void mfma_spmm_split_kernel(
     unsigned int* done = ticket + 32;
     T sq[4];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) sq[q] = a.xscale ? (T) a.xscale[4 * i0 + q] : T(1);
+    for (int q = 0; q < 4; ++q) sq[q] = a.xscale ? (T) a.xscale[Lanes<T>::probe(i0, q)] : T(1);
     const T tt = (T) a.t;
     auto slot = [](int m, int src) { return m * (S - 1) + (src < m / MS ? src : src - 1); };
     if (j == 0 && lane == 0)
@@ -957,17 +1019,17 @@ void mfma_spmm_split_kernel(
                     reinterpret_cast<const Pair2<T>*>(tvals + base * 16) + lane;
                 Pair2<T> ap = ap2[0], an;
                 T b0[4], b1[4];
-                load_row4(X + col_of(0) * kBlock + 4 * i0, b0);
+                Lanes<T>::load(X + col_of(0) * kBlock, i0, b0);
                 for (int P = 0; P < np; ++P)
                 {
-                    load_row4(X + col_of(2 * P + 1) * kBlock + 4 * i0, b1);
+                    Lanes<T>::load(X + col_of(2 * P + 1) * kBlock, i0, b1);
 #pragma unroll
                     for (int q = 0; q < 4; ++q)
                         acc[q] = Mfma16<T>::op(ap.x, sq[q] * b0[q], acc[q]);
                     if (P + 1 < np)
                     {
                         an = ap2[(P + 1) * kWave];
-                        load_row4(X + col_of(2 * P + 2) * kBlock + 4 * i0, b0);
+                        Lanes<T>::load(X + col_of(2 * P + 2) * kBlock, i0, b0);
                     }
 #pragma unroll
                     for (int q = 0; q < 4; ++q)
@@ -1026,8 +1088,8 @@ void mfma_spmm_split_kernel(
 #pragma unroll
         for (int q = 0; q < 4; ++q)
         {
-            sub_c[q] = Sub ? (T) a.sub_coef[4 * i0 + q] : T(0);
-            sub_s[q] = Sub ? (T) a.sub_scale[4 * i0 + q] : T(0);
+            sub_c[q] = Sub ? (T) a.sub_coef[Lanes<T>::probe(i0, q)] : T(0);
+            sub_s[q] = Sub ? (T) a.sub_scale[Lanes<T>::probe(i0, q)] : T(0);
         }
 #pragma unroll
         for (int mm = 0; mm < MS; ++mm)
@@ -1049,7 +1111,7 @@ void mfma_spmm_split_kernel(
                 const double r2 = r1 + __shfl_xor(r1, 32);
                 if (q == kq) mine = r2;
             }
-            a.rb.tile_partial[(t * S + j) * kWave + 4 * i0 + kq] = mine;
+            a.rb.tile_partial[(t * S + j) * kWave + Lanes<T>::probe(i0, kq)] = mine;
         }
     }
     unsigned int d = 0;
@@ -1073,7 +1135,7 @@ void mfma_spmm_split_kernel(
 // after them, so both loads overlap the matrix-core work. MFMA
 // group (c, g) gives k-slot kq to column 16 c + 4 kq + g of the chunk (A
 // and B alike, so the product is unchanged); the four 16-probe blocks hold
-// probes 4 i + q as in mfma_spmm_kernel, whose epilogue it shares. At 64
+// probes Lanes<T>::probe(i, q) as in mfma_spmm_kernel, whose epilogue it shares. At 64
 // probes the product does 16 flops per FP64 matrix byte: the bound is the
 // matrix cores (78.6 TFLOP/s is 4.9 TB/s of A) as much as HBM.
 constexpr int kDenseKC = 64;
@@ -1085,7 +1147,8 @@ __device__ __forceinline__ void load_vals4(const T* __restrict__ p, T (&v)[4])
 }
 
 // Per-probe reduction terms of a workgroup's rows (red[q] of lane (kq, i0)
-// holds probe 4 i0 + q over that lane's rows) into its partial row.
+// holds probe Lanes<T>::probe(i0, q) over that lane's rows) into its
+// partial row.
 template <typename T>
 __device__ __forceinline__ void dense_red_partial(const SpmmArgs<T>& a, const double (&red)[4],
                                                   int wave, int kq, int i0, int lane)
@@ -1098,10 +1161,10 @@ __device__ __forceinline__ void dense_red_partial(const SpmmArgs<T>& a, const do
         const double r2 = r1 + __shfl_xor(r1, 32);
         if (q == kq) mine = r2;
     }
-    // lane (kq, i0) holds probe 4 i0 + kq: to lane = probe through LDS, then
-    // the wavefronts in order
+    // lane (kq, i0) holds probe Lanes<T>::probe(i0, kq): to lane = probe
+    // through LDS, then the wavefronts in order
     __shared__ double pr[kWavesPerWG][kWave];
-    pr[wave][4 * i0 + kq] = mine;
+    pr[wave][Lanes<T>::probe(i0, kq)] = mine;
     __syncthreads();
     if (wave == 0)
     {
@@ -1138,14 +1201,14 @@ __global__ __launch_bounds__(kThreads) void dense_spmm_kernel(SpmmArgs<T> a,
     for (int u = 0; u < 8; ++u) ssc[u] = a.xscale ? (T) a.xscale[pp + u] : T(1);
     T sq[4];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) sq[q] = a.xscale ? (T) a.xscale[4 * i0 + q] : T(1);
+    for (int q = 0; q < 4; ++q) sq[q] = a.xscale ? (T) a.xscale[Lanes<T>::probe(i0, q)] : T(1);
     const T tt = (T) a.t;
     T sub_c[4], sub_s[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q)
     {
-        sub_c[q] = Sub ? (T) a.sub_coef[4 * i0 + q] : T(0);
-        sub_s[q] = Sub ? (T) a.sub_scale[4 * i0 + q] : T(0);
+        sub_c[q] = Sub ? (T) a.sub_coef[Lanes<T>::probe(i0, q)] : T(0);
+        sub_s[q] = Sub ? (T) a.sub_scale[Lanes<T>::probe(i0, q)] : T(0);
     }
     const int64_t nchunk = (a.ncols + kDenseKC - 1) / kDenseKC;
     const int64_t cps = (nchunk + ksplit - 1) / ksplit;
@@ -1220,7 +1283,7 @@ __global__ __launch_bounds__(kThreads) void dense_spmm_kernel(SpmmArgs<T> a,
                 for (int g = 0; g < 4; ++g)
                 {
                     T bv[4];
-                    load_row4(&xs[buf][16 * c + 4 * kq + g][4 * i0], bv);
+                    Lanes<T>::load(&xs[buf][16 * c + 4 * kq + g][0], i0, bv);
 #pragma unroll
                     for (int q = 0; q < 4; ++q)
                         acc[q] = Mfma16<T>::op(av[c][g], bv[q], acc[q]);
@@ -1243,7 +1306,8 @@ __global__ __launch_bounds__(kThreads) void dense_spmm_kernel(SpmmArgs<T> a,
             const T y[4] = {acc[0][m], acc[1][m], acc[2][m], acc[3][m]};
             if (row >= a.nrows) continue;
             if (ksplit > 1)
-                store_row4<false>(scratch + ((int64_t) sp * a.nrows + row) * kBlock + 4 * i0, y);
+                Lanes<T>::template store<false>(
+                    scratch + ((int64_t) sp * a.nrows + row) * kBlock, i0, y);
             else
                 spmm_epilogue4<T, NT>(a, row, i0, y, sq, tt, sub_c, sub_s, X, Y, Vout, Sub,
                                       red);
@@ -1271,9 +1335,9 @@ __global__ __launch_bounds__(kThreads) void dense_reduce_kernel(SpmmArgs<T> a,
 #pragma unroll
     for (int q = 0; q < 4; ++q)
     {
-        sq[q] = a.xscale ? (T) a.xscale[4 * i0 + q] : T(1);
-        sub_c[q] = Sub ? (T) a.sub_coef[4 * i0 + q] : T(0);
-        sub_s[q] = Sub ? (T) a.sub_scale[4 * i0 + q] : T(0);
+        sq[q] = a.xscale ? (T) a.xscale[Lanes<T>::probe(i0, q)] : T(1);
+        sub_c[q] = Sub ? (T) a.sub_coef[Lanes<T>::probe(i0, q)] : T(0);
+        sub_s[q] = Sub ? (T) a.sub_scale[Lanes<T>::probe(i0, q)] : T(0);
     }
     const T tt = (T) a.t;
     double red[4] = {0.0, 0.0, 0.0, 0.0};
@@ -1289,7 +1353,7 @@ __global__ __launch_bounds__(kThreads) void dense_reduce_kernel(SpmmArgs<T> a,
             for (int sp = 0; sp < ksplit; ++sp)
             {
                 T v[4];
-                load_row4(scratch + ((int64_t) sp * a.nrows + row) * kBlock + 4 * i0, v);
+                Lanes<T>::load(scratch + ((int64_t) sp * a.nrows + row) * kBlock, i0, v);
 #pragma unroll
                 for (int q = 0; q < 4; ++q) y[q] += v[q];
             }
