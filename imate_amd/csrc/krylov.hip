@@ -13,6 +13,8 @@
 // Layout: every vector block is [rows][64] row-major, lane p of a wavefront
 // owns probe p, so a row of a block is one coalesced 512 B (FP64) access and
 // the gather of column c reads the contiguous row c of the operand block.
+// The matrix-core SpMM maps lanes to probes by its MFMA operand layout
+// instead (Lanes<T>), keeping every 16 B access on whole cache lines.
 //
 // Work distribution ("XCD windows"): workgroup ids are dealt round-robin to
 // the 8 XCDs (id mod 8). XCD x owns the contiguous eighth x of the rows (or
@@ -20,7 +22,9 @@
 // (workgroups per XCD) x (wavefronts per workgroup) units at a time. Units
 // processed concurrently on an XCD are therefore neighbours; on a spatially
 // ordered matrix their gathered operand rows overlap and stay in that XCD's
-// 4 MB L2.
+// 4 MB L2. The tiled SpMM hands each XCD's tiles out in order from a
+// per-XCD ticket counter instead, for the same locality with dynamic load
+// balance.
 //
 // Reductions (alpha, beta, norms) are per-probe sums over all rows, computed
 // deterministically and without fences: each workgroup (or, in the tiled
