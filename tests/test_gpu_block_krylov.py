@@ -171,3 +171,19 @@ def test_convergence_stops_early(c1):
     assert numpy.all(numpy.isnan(got[used:]))
     want = _ref(c1, 13, used, 6, lambda x: 1.0 / x)
     assert _rel(got[:used].reshape(used, -1), want) < 1e-9
+
+
+def test_grouping_leaves_samples_unchanged(c1):
+    """Blocks eigensolved one at a time (min_num_samples = 64: every block
+    may stop the loop, so each is its own group) against all four in one
+    batched group (min_num_samples = 256); with error_rtol = 0 neither run
+    stops early, and the samples agree to rounding of the eigensolver."""
+    kw = dict(method='slq', gpu=True, krylov='block', lanczos_degree=6,
+              max_num_samples=256, error_rtol=0.0, error_atol=0.0, seed=17,
+              return_info=True)
+    _, one = imate_amd.logdet(c1, min_num_samples=64, **kw)
+    _, grp = imate_amd.logdet(c1, min_num_samples=256, **kw)
+    a = numpy.asarray(one['convergence']['samples'], dtype=numpy.float64)
+    b = numpy.asarray(grp['convergence']['samples'], dtype=numpy.float64)
+    assert int(one['convergence']['num_samples_used']) == 256
+    assert _rel(a, b) < 1e-12
