@@ -187,3 +187,14 @@ def test_grouping_leaves_samples_unchanged(c1):
     b = numpy.asarray(grp['convergence']['samples'], dtype=numpy.float64)
     assert int(one['convergence']['num_samples_used']) == 256
     assert _rel(a, b) < 1e-12
+
+
+@pytest.mark.parametrize('form', ['dense', 'csc'])
+def test_dense_and_csc_operators(c1, form):
+    """The block form over the dense operator (matrix-core dense SpMM with
+    column splits at this size) and over a CSC input (converted to CSR at
+    operator creation): the NumPy statement's samples at 1e-9."""
+    A = c1.toarray() if form == 'dense' else c1.tocsc()
+    val, got = _run(imate_amd.logdet, A, 6, 64, 23)
+    want = _ref(c1, 23, 64, 6, numpy.log)
+    assert _rel(got, want) < 1e-9
