@@ -100,3 +100,32 @@ def test_precision_form_full_config_vs_exact():
         e3 = i3['error']['absolute_error']
         assert abs(ti - exact['traceinv']) <= 0.01 * exact['traceinv']
         assert ti - exact['traceinv'] <= 3 * e3, (prec, ti, e3)
+
+
+def test_block_krylov_full_config_vs_exact():
+    """The block Lanczos form (krylov='block': FP32 or FP64 shared basis,
+    matrix-core re-orthogonalisation, batched block eigensolve) at the same
+    full configuration: logdet within 1 % and three Monte-Carlo errors of
+    the Cholesky value, tr(A^2) within three Monte-Carlo errors (block Gauss
+    quadrature is exact for x^2), traceinv not above the exact value by more
+    than three Monte-Carlo errors and within 1 %."""
+    with open(os.path.join(HERE, 'golden', 'exact_precision_form.json')) as f:
+        exact = json.load(f)
+    n, m, s = exact['n'], 60, 256
+    A = sparse_correlation_matrix(n, scale=exact['scale'], seed=exact['seed'])
+    M = imate_amd.Matrix(A)
+    for prec in ('mixed', 'native'):
+        kw = dict(method='slq', gpu=True, krylov='block', lanczos_degree=m,
+                  min_num_samples=s, max_num_samples=s, seed=0,
+                  precision=prec, return_info=True)
+        ld, i1 = imate_amd.logdet(M, **kw)
+        e1 = i1['error']['absolute_error']
+        assert abs(ld - exact['logdet']) <= 0.01 * abs(exact['logdet'])
+        assert abs(ld - exact['logdet']) <= 3 * e1, (prec, ld, e1)
+        t2, i2 = imate_amd.trace(M, p=2, **kw)
+        e2 = i2['error']['absolute_error']
+        assert abs(t2 - exact['trace_A2']) <= 3 * e2, (prec, t2, e2)
+        ti, i3 = imate_amd.traceinv(M, **kw)
+        e3 = i3['error']['absolute_error']
+        assert abs(ti - exact['traceinv']) <= 0.01 * exact['traceinv']
+        assert ti - exact['traceinv'] <= 3 * e3, (prec, ti, e3)
