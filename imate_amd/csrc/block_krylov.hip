@@ -367,88 +367,97 @@ __global__ __launch_bounds__(kBkThreads) void bk_nn_kernel(const T* __restrict__
         }
 }
 
-// Value of v in lane l (l uniform across the wavefront).
-__device__ __forceinline__ double readlane_d(double v, int l)
-{
-    const unsigned long long u = (unsigned long long) __double_as_longlong(v);
-    const unsigned int lo = (unsigned int) __builtin_amdgcn_readlane((int) (unsigned int) u, l);
-    const unsigned int hi = (unsigned int) __builtin_amdgcn_readlane((int) (unsigned int) (u >> 32), l);
-    return __longlong_as_double((long long) (((unsigned long long) hi << 32) | lo));
-}
-
 // Cholesky factor of G (64 x 64, its upper triangle read): G = R^T R with R
 // upper triangular. Writes Rinv = R^{-1} and Rout = R Rprev (R when Rprev
 // is null), both row-major; a pivot that is not positive (or not finite)
-// sets *flag. One wavefront with every matrix in registers: lane t holds
-// column t, and the entries of other columns come by readlane (the loops
-// are unrolled, so every register index and lane is a constant); no LDS
-// round trips and no barriers on this chain of 64 dependent steps.
-__global__ __launch_bounds__(64) void bk_chol_kernel(const double* __restrict__ G,
-                                                     const double* __restrict__ Rprev,
-                                                     double* __restrict__ Rinv,
-                                                     double* __restrict__ Rout,
-                                                     int* __restrict__ flag)
+// sets *flag. 256 threads: thread (w, l) holds rows 16 w .. 16 w + 15 of
+// column l in registers. At step k the wavefront holding row k publishes
+// it through a double-buffered LDS row (one barrier per step: a buffer is
+// rewritten two steps later, after the next barrier), every thread scales
+// it by 1 / R[k][k] and updates its rows below k; the rows of R are kept in
+// LDS for the inverse (one wavefront, its column of x in registers) and
+// the product (all four, Rprev staged in LDS). Rows of a column below its
+// diagonal are updated too and never read, which keeps the loops free of
+// lane masks. (A one-wavefront form with readlane broadcasts, 42k
+// instructions, took 110 us.)
+__global__ __launch_bounds__(256) void bk_chol_kernel(const double* __restrict__ G,
+                                                      const double* __restrict__ Rprev,
+                                                      double* __restrict__ Rinv,
+                                                      double* __restrict__ Rout,
+                                                      int* __restrict__ flag)
 {
-    const int t = threadIdx.x;
-    double c[kB];
+    __shared__ double raw[2][kB];
+    __shared__ double Rs[kB][kB];
+    __shared__ double Ps[kB][kB];
+    const int l = threadIdx.x & (kB - 1);
+    const int w = threadIdx.x >> 6;
+    double c[16];
 #pragma unroll
-    for (int i = 0; i < kB; ++i) c[i] = i <= t ? G[i * kB + t] : G[t * kB + i];
+    for (int r = 0; r < 16; ++r) c[r] = G[(16 * w + r) * kB + l];
+    if (Rprev)
+    {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) Ps[16 * w + r][l] = Rprev[(16 * w + r) * kB + l];
+    }
     bool bad = false;
 #pragma unroll
     for (int k = 0; k < kB; ++k)
     {
-        double d = readlane_d(c[k], k);
+        const int b = k & 1;
+        if (w == k / 16) raw[b][l] = c[k % 16];
+        __syncthreads();
+        double d = raw[b][k];
         if (!(d > 0.0) || !isfinite(d))
         {
             bad = true;
             d = 1.0;
         }
         const double rkk = sqrt(d);
-        // row k of R: lane t >= k holds R[k][t] in c[k]
-        if (t == k) c[k] = rkk;
-        else if (t > k) c[k] = c[k] / rkk;
-        // trailing column t: A[i][t] -= R[k][i] R[k][t], k < i <= t
+        const double inv = 1.0 / rkk;
+        const double rkl = l > k ? raw[b][l] * inv : (l == k ? rkk : 0.0);
+        if (w == 0) Rs[k][l] = rkl;
+        // rows i = 16 w + r: A[i][l] -= R[k][i] R[k][l]
 #pragma unroll
-        for (int i = k + 1; i < kB; ++i)
-        {
-            const double rki = readlane_d(c[k], i);
-            if (t >= i) c[i] -= rki * c[k];
-        }
+        for (int r = 0; r < 16; ++r) c[r] -= (raw[b][16 * w + r] * inv) * rkl;
     }
-    if (bad && t == 0) atomicOr(flag, 1);
-    // column t of R^{-1}, bottom row first: R[i][k] is lane k's c[i]
+    if (bad && threadIdx.x == 0) atomicOr(flag, 1);
+    __syncthreads();
+    if (w == 0)
     {
+        // column l of R^{-1}, bottom row first
         double x[kB];
 #pragma unroll
         for (int i = kB - 1; i >= 0; --i)
         {
-            double acc = (i == t) ? 1.0 : 0.0;
+            // four partial sums (k mod 4) shorten the dependent chain
+            double acc[4] = {(i == l) ? 1.0 : 0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-            for (int k = i + 1; k < kB; ++k) acc -= readlane_d(c[i], k) * x[k];
-            x[i] = acc / readlane_d(c[i], i);
+            for (int k = i + 1; k < kB; ++k) acc[k & 3] -= Rs[i][k] * x[k];
+            x[i] = ((acc[0] + acc[1]) + (acc[2] + acc[3])) / Rs[i][i];
         }
 #pragma unroll
-        for (int i = 0; i < kB; ++i) Rinv[i * kB + t] = x[i];
+        for (int i = 0; i < kB; ++i) Rinv[i * kB + l] = x[i];
     }
-    // column t of R Rprev (Rprev upper triangular: zero below its diagonal)
-    if (Rprev)
+    // rows 16 w .. 16 w + 15 of R Rprev (Rprev upper triangular) or of R
+    for (int r = 0; r < 16; ++r)
     {
-        double rp[kB];
-#pragma unroll
-        for (int k = 0; k < kB; ++k) rp[k] = Rprev[k * kB + t];
-#pragma unroll
-        for (int i = 0; i < kB; ++i)
+        const int i = 16 * w + r;
+        double acc;
+        if (Rprev)
         {
-            double acc = 0.0;
-#pragma unroll
-            for (int k = i; k < kB; ++k) acc += readlane_d(c[i], k) * rp[k];
-            Rout[i * kB + t] = acc;
+            double a0 = 0.0, a1 = 0.0;
+            int k = i;
+            for (; k + 1 < kB; k += 2)
+            {
+                a0 += Rs[i][k] * Ps[k][l];
+                a1 += Rs[i][k + 1] * Ps[k + 1][l];
+            }
+            if (k < kB) a0 += Rs[i][k] * Ps[k][l];
+            acc = a0 + a1;
         }
-    }
-    else
-    {
-#pragma unroll
-        for (int i = 0; i < kB; ++i) Rout[i * kB + t] = i <= t ? c[i] : 0.0;
+        else
+            acc = Rs[i][l];
+        Rout[i * kB + l] = acc;
     }
 }
 
@@ -685,11 +694,11 @@ int cholqr2(slq_op* op, const T* X, T* Tmp, T* Y, int64_t n, const BkBuffers& bb
 {
     const hipStream_t s = op->stream;
     SLQ_TRY(tn<T>(op, X, 0, 1, X, n, bb, bb.G));
-    hipLaunchKernelGGL(bk_chol_kernel, dim3(1), dim3(kB), 0, s, (const double*) bb.G,
+    hipLaunchKernelGGL(bk_chol_kernel, dim3(1), dim3(256), 0, s, (const double*) bb.G,
                        (const double*) nullptr, bb.Rinv, bb.R1, bb.flag);
     SLQ_TRY((nn<T, false>(op, X, 0, 1, bb.Rinv, Tmp, n)));
     SLQ_TRY(tn<T>(op, Tmp, 0, 1, Tmp, n, bb, bb.G));
-    hipLaunchKernelGGL(bk_chol_kernel, dim3(1), dim3(kB), 0, s, (const double*) bb.G,
+    hipLaunchKernelGGL(bk_chol_kernel, dim3(1), dim3(256), 0, s, (const double*) bb.G,
                        (const double*) bb.R1, bb.Rinv, Rout, bb.flag);
     SLQ_TRY((nn<T, false>(op, Tmp, 0, 1, bb.Rinv, Y, n)));
     return SLQ_OK;
