@@ -626,13 +626,17 @@ struct BkSlots {
 };
 
 // Row chunks of a Q^T W product over nkb basis blocks (one workgroup per
-// chunk and four blocks): about 1024 workgroups, at least 64 chunks, chunks
-// of at least 16 rows.
+// chunk and four blocks): about 512 workgroups (two wavefronts per SIMD,
+// the kernel's occupancy), at least 32 chunks, chunks of at least 16 rows.
+// Each chunk writes a partial block that bk_reduce_kernel reads back, so
+// fewer chunks move fewer bytes: against about 1024 workgroups, GEMM time
+// per call 415 -> 407 ms FP64 and 210 -> 205 ms FP32; about 256 were
+// slower (462 and 231 ms).
 int tn_chunks(int64_t n, int nkb)
 {
     const int groups = (nkb + 3) / 4;
-    int c = 1024 / groups;
-    if (c < 64) c = 64;
+    int c = 512 / groups;
+    if (c < 32) c = 32;
     const int64_t most = (n + 15) / 16;
     if (c > most) c = (int) most;
     return c < 1 ? 1 : c;
