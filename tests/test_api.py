@@ -15,7 +15,8 @@ import scipy.sparse
 import imate_amd
 from imate_amd import _lib
 from imate_amd.sample_matrices import (toeplitz, sparse_correlation_matrix,
-                                       morton_order, correlation_scale)
+                                       morton_order, hilbert_order,
+                                       correlation_scale)
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
@@ -161,6 +162,31 @@ def test_correlation_matrix_is_spd_and_sparse():
     B = sparse_correlation_matrix(2000, scale=0.02, seed=4)
     assert numpy.linalg.eigvalsh(B.toarray()).min() > 0
     assert correlation_scale(4096, 40) > 0
+
+
+def test_hilbert_order_walks_the_grid_by_edge_neighbours():
+    """On the cell centres of a 32 x 32 grid the curve visits every cell
+    once, each step to an edge neighbour, from (0, 0) to (31, 0); a wrong
+    reflection or swap of the quadrant frames breaks the adjacency."""
+    g = numpy.array([(i, j) for i in range(32) for j in range(32)], float)
+    p = hilbert_order((g + 0.5) / 32, bits=5)
+    assert numpy.array_equal(numpy.sort(p), numpy.arange(1024))
+    q = g[p]
+    assert numpy.all(numpy.abs(numpy.diff(q, axis=0)).sum(axis=1) == 1)
+    assert tuple(q[0]) == (0, 0) and tuple(q[-1]) == (31, 0)
+
+
+def test_curve_orders_give_one_matrix_up_to_permutation():
+    """The Hilbert and Morton orderings of the same seeded points are one
+    matrix under a symmetric permutation: the same spectrum."""
+    A = sparse_correlation_matrix(600, nnz_per_row=20, seed=2)
+    B = sparse_correlation_matrix(600, nnz_per_row=20, seed=2, order='morton')
+    assert A.nnz == B.nnz
+    ea = numpy.linalg.eigvalsh(A.toarray())
+    eb = numpy.linalg.eigvalsh(B.toarray())
+    assert numpy.allclose(ea, eb, rtol=0, atol=1e-12)
+    with pytest.raises(ValueError):
+        sparse_correlation_matrix(600, nnz_per_row=20, order='rows')
 
 
 def test_morton_order_is_a_permutation_with_locality():
