@@ -4,7 +4,7 @@ from a sparse LDL^T factorisation (SciPy SuperLU in symmetric mode with a
 minimum-degree ordering on A^T + A and diagonal pivots), logdet = sum log d_ii.
 This is the oracle the reference uses for logdet (imate/logdet/
 _cholesky_method.py:262-330 with sparse_cholesky.py:23-66). The n = 2^20 case
-takes about 12 minutes and 15 GB of memory.
+takes about 12-34 minutes and 15-23 GB of memory.
 
     python tests/golden/make_cholesky_reference.py [n ...]
 """
