@@ -937,7 +937,8 @@ def main():
         'scaling': 'weak',
         'vs_baseline': None,
         'dtype': 'f64',
-        'data': 'synthetic (seeded sparse correlation matrix)',
+        'data': 'synthetic (seeded sparse correlation matrix, points '
+                'uniform in the unit square sorted along a Hilbert curve)',
         'config': {'workload': 'slq-%s-correlation-n%d-nnz%.0f-m%d-s%d-b64'
                                % (args.func, n, k, m, s),
                    'n': n, 'nnz': int(A.nnz), 'lanczos_degree': m,
